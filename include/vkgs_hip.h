@@ -1,0 +1,228 @@
+/*
+ * vkgs_hip.h -- C ABI of the MI355X (gfx950) splat pipeline.
+ *
+ * The reference (jaesung-cs/vkgs) keeps its splat path private to
+ * Engine::Impl (src/vkgs/engine/engine.cc:101-1682); its stages talk only
+ * through device buffers bound as descriptor sets (engine.cc:128-214).  This
+ * header exports one entry point per reference stage, with the same buffer
+ * layouts, plus an Engine-like renderer that runs a whole frame.
+ *
+ * Rules of the ABI:
+ *  - Plain C: pointers, sizes and a hipStream_t passed as `void*` (NULL = the
+ *    null stream).  No exceptions cross it; every call returns vkgs_status.
+ *  - Pointers named d_* are device memory the caller owns; stage entry points
+ *    never allocate.  Scratch ("storage") is sized by *_storage_bytes queries,
+ *    as vrdxGetSorterKeyValueStorageRequirements does for the sort
+ *    (engine.cc:518-520).
+ *  - Counts produced on the device stay on the device (indirect semantics of
+ *    vrdxCmdSortKeyValueIndirect engine.cc:1218 and vkCmdDrawIndexedIndirect
+ *    engine.cc:1479): stage calls take `max_*` capacities from the host and
+ *    read the live count through a device pointer.
+ *  - Stream-ordered and non-blocking unless stated otherwise.
+ */
+#ifndef VKGS_HIP_H
+#define VKGS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VKGS_ABI_VERSION 1
+
+typedef enum vkgs_status {
+  VKGS_OK = 0,
+  VKGS_ERR_INVALID_ARG = 1, /* null pointer, zero size where one is required, bad enum */
+  VKGS_ERR_CAPACITY = 2,    /* request exceeds a capacity (N > capacity, storage too small) */
+  VKGS_ERR_HIP = 3,         /* a HIP runtime call failed */
+  VKGS_ERR_IO = 4,          /* file missing or unreadable */
+  VKGS_ERR_FORMAT = 5,      /* malformed or unsupported PLY */
+  VKGS_ERR_OVERFLOW = 6,    /* a device-side list outgrew its storage; result incomplete */
+  VKGS_ERR_NO_DEVICE = 7    /* no HIP device */
+} vkgs_status;
+
+/* Byte-compatible with vk::shader::Camera (src/vkgs/vulkan/shader/uniforms.h:10-15):
+ * mat4 projection @0, mat4 view @64, vec3 camera_position @128,
+ * alignas(16) uvec2 screen_size @144; alignas(64) -> sizeof 192. */
+typedef struct vkgs_camera {
+  float projection[16]; /* column-major, already converted to Vulkan clip space */
+  float view[16];
+  float camera_position[3];
+  float pad0;
+  uint32_t screen_size[2]; /* (width, height) */
+  uint32_t pad1[10];
+} vkgs_camera;
+
+const char* vkgs_status_string(int status);
+int vkgs_abi_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Camera producer.  Replaces vkgs::Camera::{ProjectionMatrix, ViewMatrix, Eye}
+ * (src/vkgs/scene/camera.cc:25-45): perspective(fovy, W/H, znear, zfar) with
+ * the GL->Vulkan conversion (camera.cc:29-34), lookAt(eye, center, +y)
+ * (camera.cc:37), eye = center + r (sin phi sin theta, cos phi, sin phi cos theta)
+ * (camera.cc:39-45).  Host-only arithmetic; no device is needed.
+ * ------------------------------------------------------------------------- */
+int vkgs_camera_orbit(float fovy, uint32_t width, uint32_t height, float znear, float zfar,
+                      const float center[3], float r, float phi, float theta, vkgs_camera* out);
+
+/* inverse of a 4x4 column-major matrix (used for the SH view direction,
+ * projection.comp:85).  Returns VKGS_ERR_INVALID_ARG for a singular matrix. */
+int vkgs_mat4_inverse(const float m[16], float out[16]);
+
+/* ---------------------------------------------------------------------------
+ * PLY ingestion.  Replaces the header parse and offset table of
+ * SplatLoadThread (src/vkgs/engine/splat_load_thread.cc:53-135) with checked
+ * parsing: binary_little_endian only, every used property must be `float`,
+ * missing properties and truncated files are errors, N > capacity is an error.
+ * ------------------------------------------------------------------------- */
+typedef struct vkgs_ply_info {
+  uint64_t point_count;   /* element vertex N */
+  uint32_t stride;        /* floats per vertex row (62 for standard 3DGS) */
+  uint32_t offsets[60];   /* offset table, splat_load_thread.cc:115-135; offsets[59] == stride */
+  uint64_t header_bytes;  /* byte offset of the first row */
+} vkgs_ply_info;
+
+int vkgs_ply_read_header(const char* path, vkgs_ply_info* info);
+/* Host buffer of the reference's ply-buffer format: 60 u32 offsets then
+ * N x stride floats (splat_load_thread.cc:92, 157-158).  dst_bytes must be
+ * >= vkgs_ply_buffer_bytes(info). */
+uint64_t vkgs_ply_buffer_bytes(const vkgs_ply_info* info);
+int vkgs_ply_read(const char* path, const vkgs_ply_info* info, void* dst, uint64_t dst_bytes);
+
+/* ---------------------------------------------------------------------------
+ * Stage 0: activation + layout (parse_ply.comp:94-157, dispatch
+ * engine.cc:1137-1161).  d_ply is the device copy of the ply buffer above.
+ * Writes the vkgs splat storage (engine.cc:485-492):
+ *   position f32 (N,3), cov3d f32 (N,6) = (xx,xy,xz,yy,yz,zz), opacity f32 (N),
+ *   sh f16 (N,48) channel-major [R: dc, rest0..14][G: ...][B: ...].
+ * ------------------------------------------------------------------------- */
+int vkgs_parse_ply(const void* d_ply, uint32_t n, float* d_position, float* d_cov3d, float* d_opacity,
+                   uint16_t* d_sh, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Stage 1: rank (rank.comp:27-42, dispatch engine.cc:1165-1194).
+ * Centre-only NDC cull, key = floatBitsToUint(1 - z_ndc), index = splat id.
+ * Unlike the reference's atomicAdd append (rank.comp:38), visible splats are
+ * compacted in ascending id order, so ties in the stable sort resolve by id.
+ * Writes *d_visible_count.  Needs storage of vkgs_rank_storage_bytes(n).
+ * ------------------------------------------------------------------------- */
+size_t vkgs_rank_storage_bytes(uint32_t max_n);
+int vkgs_rank(const vkgs_camera* camera, const float model[16], const float* d_position, uint32_t n,
+              uint32_t* d_visible_count, uint32_t* d_key, uint32_t* d_index, void* d_storage,
+              size_t storage_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Stage 2: sort.  Replaces vrdxGetSorterKeyValueStorageRequirements
+ * (engine.cc:518-520) and vrdxCmdSortKeyValueIndirect (engine.cc:1218-1219):
+ * stable ascending LSD radix sort of (u32 key, u32 value) pairs in place; the
+ * live count is read from *d_count (<= max_n).  key_bits limits the sorted key
+ * width (32 sorts full keys; 30 suffices for rank keys, which are <= 0x3f800000).
+ * ------------------------------------------------------------------------- */
+size_t vkgs_sort_kv_storage_bytes(uint32_t max_n);
+int vkgs_sort_kv(uint32_t max_n, const uint32_t* d_count, uint32_t* d_key, uint32_t* d_value, uint32_t key_bits,
+                 void* d_storage, size_t storage_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Stage 3: inverse index (inverse_index.comp:55-60 after the -1 fill of
+ * engine.cc:1226): d_inverse[d_index[k]] = k for k < *d_count, -1 elsewhere.
+ * ------------------------------------------------------------------------- */
+int vkgs_inverse_index(const uint32_t* d_count, const uint32_t* d_index, uint32_t n, int32_t* d_inverse,
+                       void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Stage 4: projection (projection.comp:60-180, dispatch engine.cc:1255-1274).
+ * Writes the 48-byte instance of every visible splat at its sorted slot:
+ *   [3s+0] = (ndc.xyz, 0), [3s+1] = rot_scale columns (c0.x, c0.y, c1.x, c1.y),
+ *   [3s+2] = (rgb, opacity)
+ * and the 12-word DrawIndexedIndirect/DrawIndirect block (projection.comp:35-48).
+ * d_instances holds float4[3 * max slots].
+ * ------------------------------------------------------------------------- */
+int vkgs_projection(const vkgs_camera* camera, const float model[16], const float* d_position,
+                    const float* d_cov3d, const float* d_opacity, const uint16_t* d_sh,
+                    const int32_t* d_inverse, uint32_t n, const uint32_t* d_visible_count,
+                    float* d_instances, uint32_t* d_draw_indirect, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Stage 5: draw.  Replaces the splat graphics pipeline (splat.vert:10-26,
+ * splat.frag:8-12, blend engine.cc:279-299, raster state
+ * graphics_pipeline.cc:58-85, vkCmdDrawIndexedIndirect engine.cc:1479) with a
+ * tile rasterizer: instances [0, V) are composited in order with the
+ * reference's "over" blend (SRC_ALPHA / ONE_MINUS_SRC_ALPHA on colour and
+ * alpha) into a height x width RGBA fp32 image cleared to (0,0,0,1)
+ * (engine.cc:1382-1387).  V is read from d_draw_indirect[8] (vertexCount1).
+ * max_pairs bounds the (tile, splat) list; overflow is reported by
+ * vkgs_draw_status after the stream completes.
+ * ------------------------------------------------------------------------- */
+typedef struct vkgs_raster_config {
+  uint32_t tile_w, tile_h; /* 8x8, 16x16, 32x8, 16x8, 8x16 */
+  float t_min;             /* stop a pixel once transmittance < t_min (0 = composite every splat) */
+  uint32_t reserved[5];
+} vkgs_raster_config;
+
+/* default: 16x16 tiles, t_min = 1/8192 */
+void vkgs_raster_config_default(vkgs_raster_config* cfg);
+
+size_t vkgs_draw_storage_bytes(uint32_t max_instances, uint32_t width, uint32_t height, uint64_t max_pairs,
+                               const vkgs_raster_config* cfg);
+int vkgs_draw(const float* d_instances, const uint32_t* d_draw_indirect, uint32_t max_instances,
+              uint32_t width, uint32_t height, uint64_t max_pairs, const vkgs_raster_config* cfg,
+              float* d_rgba, void* d_storage, size_t storage_bytes, void* stream);
+/* Synchronous: waits for the stream, then reports VKGS_ERR_OVERFLOW if the
+ * last vkgs_draw on this storage overflowed, else VKGS_OK.  *pairs receives the
+ * (tile, splat) pair count of that draw. */
+int vkgs_draw_status(const void* d_storage, size_t storage_bytes, uint64_t* pairs, void* stream);
+
+/* Tile lists of the last vkgs_draw on this storage (device pointers into the
+ * storage): d_ranges = uint32 [tiles][2] = [start, end), d_list = instance
+ * indices in draw order per tile.  Synchronous (reads the storage header).
+ * For parity checks and tools. */
+int vkgs_draw_lists(const void* d_storage, size_t storage_bytes, const uint32_t** d_ranges,
+                    const uint32_t** d_list, uint32_t* tiles_x, uint32_t* tiles_y);
+
+/* ---------------------------------------------------------------------------
+ * Renderer: the Engine analogue (Engine::Impl::Draw, engine.cc:1164-1301).
+ * Owns the splat storage (preallocated to a capacity, as engine.cc:483-502
+ * does for MAX_SPLAT_COUNT) and the frame workspace, and renders a frame with
+ * the fused MI355X path: cull + project + compact, depth sort, tile binning,
+ * tile sort, ranges, raster.  Frames are captured into a hipGraph per
+ * (N, width, height, raster config) and replayed.
+ * ------------------------------------------------------------------------- */
+typedef struct vkgs_renderer vkgs_renderer;
+
+int vkgs_renderer_create(uint32_t capacity, vkgs_renderer** out);
+int vkgs_renderer_destroy(vkgs_renderer* r);
+/* Engine::LoadSplats (engine.cc:541-544) + parse (engine.cc:1137-1161):
+ * synchronous load of a PLY file into the splat storage. */
+int vkgs_renderer_load_ply(vkgs_renderer* r, const char* path);
+/* Load from a host ply buffer (60 offsets + rows), as produced by vkgs_ply_read. */
+int vkgs_renderer_load_ply_buffer(vkgs_renderer* r, const void* host_ply, uint32_t n);
+uint32_t vkgs_renderer_splat_count(const vkgs_renderer* r);
+/* Device pointers of the splat storage (position, cov3d, opacity, sh). */
+int vkgs_renderer_storage(vkgs_renderer* r, float** d_position, float** d_cov3d, float** d_opacity,
+                          uint16_t** d_sh);
+int vkgs_renderer_set_config(vkgs_renderer* r, const vkgs_raster_config* cfg);
+/* Render one frame into d_rgba (height x width x 4 fp32, device).  camera and
+ * model (NULL = identity) are host structs copied at call time. */
+int vkgs_renderer_render(vkgs_renderer* r, const vkgs_camera* camera, const float model[16], float* d_rgba,
+                         void* stream);
+/* Per-frame results of the last render (synchronous): visible count, pair
+ * count, overflow flag -> VKGS_ERR_OVERFLOW. */
+int vkgs_renderer_frame_info(vkgs_renderer* r, uint32_t* visible, uint64_t* pairs, void* stream);
+/* Device pointers of the last frame's depth-sorted order: d_sorted_ids[k] is
+ * the splat id drawn k-th (ascending key = far to near), d_keys[k] its key. */
+int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const uint32_t** d_keys);
+int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const uint32_t** d_list_ids,
+                             uint32_t* tiles_x, uint32_t* tiles_y);
+/* Enable per-stage hipEvent timing (the reference's 12-timestamp breakdown,
+ * engine.cc:1031-1045); disables graph replay while on. */
+int vkgs_renderer_set_timing(vkgs_renderer* r, int enable);
+/* ms of the last timed frame: [project, sort, bin, tile_sort, raster, total] */
+int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,201 @@
+"""HIP path vs the CPU oracle, stage by stage, on identical input bits.
+
+Bars: bit-exact for visible count, keys, sorted order, inverse map, instances
+(projection uses only correctly rounded + - * / sqrt), tile ranges and tile
+lists; images within 1/255 max abs per channel and PSNR >= 50 dB (and within
+1e-5 when every splat is composited, t_min = 0).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no HIP device", allow_module_level=True)
+
+import oracle  # noqa: E402
+from vkgs_amd import pipeline as P  # noqa: E402
+from vkgs_amd import scene as S  # noqa: E402
+
+
+def psnr(a, b):
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)
+
+
+def u32(t):
+    return P.u32(t)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    sc = S.gaussian_cloud(100_000, seed=1)
+    cam = S.orbit_camera(1600, 900, 6.0)
+    st = P.parse_ply(sc.ply_buffer(), sc.n)
+    torch.cuda.synchronize()
+    host = dict(pos=st.position.cpu().numpy(), cov=st.cov3d.cpu().numpy(), opa=st.opacity.cpu().numpy(),
+                sh=st.sh.cpu().numpy().view(np.uint16))
+    return sc, cam, st, host
+
+
+def test_parse_ply_matches_oracle(c1):
+    sc, cam, st, host = c1
+    pos, cov, opa, sh = oracle.parse_ply(sc.rows, S.standard_offsets())
+    print("parse_ply n=", sc.n, flush=True)
+    np.testing.assert_array_equal(host["pos"], pos)
+    np.testing.assert_array_equal(host["sh"], sh)
+    # exp() differs between libm and the device math library by a few ulp; the
+    # off-diagonal terms cancel, so the bound is relative to each matrix's scale
+    scale = np.abs(cov[:, [0, 3, 5]]).max(axis=1, keepdims=True)
+    assert np.all(np.abs(host["cov"] - cov) <= 4e-6 * scale)
+    np.testing.assert_allclose(host["opa"], opa, rtol=1e-6, atol=1e-7)
+
+
+def test_rank_bit_exact(c1):
+    sc, cam, st, host = c1
+    count, key, index = P.rank(cam, st)
+    v = int(u32(count)[0])
+    ov, okey, oidx = oracle.rank(cam, host["pos"])
+    print("rank visible", v, "oracle", ov, flush=True)
+    assert v == ov
+    np.testing.assert_array_equal(u32(key)[:v], okey)
+    np.testing.assert_array_equal(u32(index)[:v], oidx)
+    assert np.all(np.diff(u32(index)[:v].astype(np.int64)) > 0)  # ascending id order
+
+
+@pytest.mark.parametrize("n,bits,distinct", [(0, 32, 10), (1, 32, 10), (777, 32, 5), (2048, 13, 100),
+                                             (100_003, 30, 1 << 30), (300_000, 32, 7), (1_000_000, 17, 5000)])
+def test_sort_kv_stable(n, bits, distinct):
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(0, min(distinct, 1 << bits), max(n, 1), dtype=np.uint64).astype(np.uint32)
+    vals = np.arange(max(n, 1), dtype=np.uint32)
+    dk = torch.from_numpy(keys.view(np.int32).copy()).cuda()
+    dv = torch.from_numpy(vals.view(np.int32).copy()).cuda()
+    cnt = torch.tensor([n], dtype=torch.int32, device="cuda")
+    print("sort n", n, "bits", bits, flush=True)
+    P.sort_kv(cnt, dk, dv, bits)
+    order = np.argsort(keys[:n], kind="stable")
+    np.testing.assert_array_equal(u32(dk)[:n], keys[:n][order])
+    np.testing.assert_array_equal(u32(dv)[:n], vals[:n][order])
+
+
+def test_sort_kv_count_below_capacity():
+    rng = np.random.default_rng(5)
+    cap, n = 50_000, 31_337
+    keys = rng.integers(0, 1 << 20, cap, dtype=np.uint64).astype(np.uint32)
+    vals = np.arange(cap, dtype=np.uint32)
+    dk = torch.from_numpy(keys.view(np.int32).copy()).cuda()
+    dv = torch.from_numpy(vals.view(np.int32).copy()).cuda()
+    cnt = torch.tensor([n], dtype=torch.int32, device="cuda")
+    P.sort_kv(cnt, dk, dv, 20)
+    order = np.argsort(keys[:n], kind="stable")
+    np.testing.assert_array_equal(u32(dk)[:n], keys[:n][order])
+    np.testing.assert_array_equal(u32(dv)[:n], vals[:n][order])
+    np.testing.assert_array_equal(u32(dk)[n:], keys[n:])  # untouched tail
+
+
+def test_reference_stages_bit_exact(c1):
+    sc, cam, st, host = c1
+    out = P.reference_frame(cam, st, 1600, 900)
+    torch.cuda.synchronize()
+    ref = oracle.frame(cam, host["pos"], host["cov"], host["opa"], host["sh"], 1600, 900)
+    v = int(u32(out["count"])[0])
+    print("reference stages visible", v, flush=True)
+    assert v == ref["visible"]
+    np.testing.assert_array_equal(u32(out["key"])[:v], ref["keys"])
+    np.testing.assert_array_equal(u32(out["index"])[:v], ref["order"])
+    np.testing.assert_array_equal(out["inverse"].cpu().numpy(), ref["inverse"])
+    inst = out["instances"].cpu().numpy()[:v]
+    np.testing.assert_array_equal(inst.view(np.uint32), ref["instances"].view(np.uint32))
+    np.testing.assert_array_equal(u32(out["draw_indirect"]), ref["draw_indirect"])
+    # tile lists of the draw
+    ranges, lst = out["draw_storage"].lists()
+    oranges, olst = oracle.bin_tiles(ref["instances"], 1600, 900, 16, 16)
+    np.testing.assert_array_equal(ranges, oranges.astype(np.int64))
+    np.testing.assert_array_equal(lst, olst)
+    # image
+    img = out["image"].cpu().numpy()
+    diff = np.abs(img - ref["image"]).max()
+    print("image max abs diff", diff, "psnr", psnr(img, ref["image"]), flush=True)
+    assert diff <= 1.0 / 255.0
+    assert psnr(img, ref["image"]) >= 50.0
+
+
+def test_draw_exact_composite_small():
+    """t_min = 0 composites every splat: image within 1e-5 of the back-to-front oracle."""
+    sc = S.gaussian_cloud(2000, seed=11, pos_std=1.0, logscale=(-2.5, 0.4))
+    cam = S.orbit_camera(64, 64, 4.0)
+    st = P.parse_ply(sc.ply_buffer(), sc.n)
+    cfg = P._lib.RasterConfig()
+    cfg.tile_w, cfg.tile_h, cfg.t_min = 16, 16, 0.0
+    out = P.reference_frame(cam, st, 64, 64, cfg=cfg)
+    v = int(u32(out["count"])[0])
+    inst = out["instances"].cpu().numpy()[:v]
+    ref = oracle.draw(inst, 64, 64)
+    brute = oracle.draw_bruteforce(inst, 64, 64)
+    np.testing.assert_array_equal(ref, brute)
+    img = out["image"].cpu().numpy()
+    print("small exact max diff", np.abs(img - ref).max(), flush=True)
+    np.testing.assert_allclose(img, ref, atol=1e-5, rtol=0)
+
+
+@pytest.mark.parametrize("tw,th", [(8, 8), (16, 16), (32, 8), (16, 8), (8, 16)])
+def test_tile_shapes(c1, tw, th):
+    sc, cam, st, host = c1
+    cfg = P._lib.RasterConfig()
+    cfg.tile_w, cfg.tile_h, cfg.t_min = tw, th, 1.0 / 8192
+    out = P.reference_frame(cam, st, 1600, 900, cfg=cfg)
+    v = int(u32(out["count"])[0])
+    inst = out["instances"].cpu().numpy()[:v]
+    ranges, lst = out["draw_storage"].lists()
+    oranges, olst = oracle.bin_tiles(inst, 1600, 900, tw, th)
+    np.testing.assert_array_equal(ranges, oranges.astype(np.int64))
+    np.testing.assert_array_equal(lst, olst)
+    ref = oracle.draw(inst, 1600, 900)
+    img = out["image"].cpu().numpy()
+    assert np.abs(img - ref).max() <= 1.0 / 255.0
+    assert psnr(img, ref) >= 50.0
+
+
+def test_renderer_fused_matches_reference(c1):
+    sc, cam, st, host = c1
+    r = P.Renderer(sc.n)
+    r.load_buffer(sc.ply_buffer(), sc.n)
+    img = r.render(cam)
+    status, v, k = r.frame_info()
+    assert status == 0
+    ids, keys = r.sorted_ids()
+    ref = oracle.frame(cam, host["pos"], host["cov"], host["opa"], host["sh"], 1600, 900)
+    print("fused visible", v, "pairs", k, flush=True)
+    assert v == ref["visible"]
+    np.testing.assert_array_equal(keys, ref["keys"])
+    np.testing.assert_array_equal(ids, ref["order"])
+    ranges, lst_ids = r.tile_lists()
+    oranges, olst = oracle.bin_tiles(ref["instances"], 1600, 900, 16, 16)
+    np.testing.assert_array_equal(ranges, oranges.astype(np.int64))
+    np.testing.assert_array_equal(lst_ids, ref["order"][olst])
+    # the stage path and the fused path composite identical instances in identical order
+    stage = P.reference_frame(cam, st, 1600, 900)
+    np.testing.assert_array_equal(img.cpu().numpy(), stage["image"].cpu().numpy())
+    # graph replay gives the same image
+    img2 = r.render(cam)
+    np.testing.assert_array_equal(img.cpu().numpy(), img2.cpu().numpy())
+    r.close()
+
+
+def test_renderer_empty_and_offscreen():
+    sc = S.gaussian_cloud(1000, seed=3)
+    sc.rows[:, 0] += 1000.0  # everything far outside the frustum
+    cam = S.orbit_camera(320, 240, 6.0)
+    r = P.Renderer(sc.n)
+    r.load_buffer(sc.ply_buffer(), sc.n)
+    img = r.render(cam).cpu().numpy()
+    status, v, k = r.frame_info()
+    assert (status, v, k) == (0, 0, 0)
+    expect = np.zeros((240, 320, 4), np.float32)
+    expect[..., 3] = 1.0  # clear colour (engine.cc:1382-1387)
+    np.testing.assert_array_equal(img, expect)
+    r.close()

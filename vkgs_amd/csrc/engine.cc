@@ -1,0 +1,385 @@
+// vkgs_renderer: the Engine analogue (Engine::Impl, engine.cc:101-1682).
+//
+// Owns the splat storage (preallocated to a capacity like engine.cc:483-502)
+// and every per-frame buffer, and records one frame as
+//   fused cull+project+compact -> depth sort -> bin -> tile sort -> ranges -> raster
+// on a stream.  The recording is captured once per (N, width, height, raster
+// config, output pointer) into a hipGraph and replayed; the camera and model
+// reach the kernels through a device-resident FrameConst written by a
+// stream-ordered copy before each replay, so no host sync happens in a frame.
+#include <string.h>
+
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "sort.h"
+
+namespace vkgs {
+hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
+                    const RasterShape& rs, uint64_t max_pairs, TileRect* rect, uint32_t* cnt, uint32_t* offs,
+                    const ScanPlan& scp, void* scan_storage, uint32_t* keys, uint32_t* vals, const SortPlan& sp,
+                    void* sort_storage, uint32_t* ranges, uint32_t* hdr_status, uint32_t* hdr_pairs, float4* out,
+                    hipStream_t s, hipEvent_t* marks);
+}
+
+using namespace vkgs;
+
+namespace {
+
+enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscWords = 64 };
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, want ? want : 1);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+}  // namespace
+
+struct vkgs_renderer {
+  uint32_t capacity = 0;
+  uint32_t n = 0;
+  DevBuf position, cov3d, opacity, sh;
+
+  // frame buffers sized by n
+  uint32_t frame_n = 0;
+  DevBuf fc, misc, counts, inst, key, val, ids, sort_storage;
+  // binning sized by (n, pairs, tiles)
+  uint64_t max_pairs = 0;
+  DevBuf rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges;
+  DevBuf sorted_ids, list_ids;
+  SortPlan depth_plan{}, pair_plan{};
+  ScanPlan scan{};
+  RasterShape rs{};
+  vkgs_raster_config cfg{};
+
+  FrameConst* h_fc = nullptr;  // pinned staging for the per-frame constants
+  hipStream_t capture_stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  struct Key {
+    uint32_t n, w, h, tw, th;
+    float t_min;
+    uint64_t pairs;
+    void* out;
+  } graph_key{};
+  bool graph_valid = false;
+  bool grow_pairs = false;
+
+  bool timing = false;
+  hipEvent_t ev[6] = {};
+  float stage_ms[6] = {};
+};
+
+namespace {
+
+void drop_graph(vkgs_renderer* r) {
+  if (r->exec) hipGraphExecDestroy(r->exec);
+  if (r->graph) hipGraphDestroy(r->graph);
+  r->exec = nullptr;
+  r->graph = nullptr;
+  r->graph_valid = false;
+}
+
+hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
+  hipError_t e;
+  const uint32_t n = r->n;
+  RasterShape rs;
+  if (!make_raster_shape(width, height, &r->cfg, &rs)) return hipErrorInvalidValue;
+  if (r->max_pairs == 0) r->max_pairs = (uint64_t)(n ? n : 1) * 4 + (1u << 20);
+  if (r->grow_pairs) {
+    r->max_pairs *= 2;
+    r->grow_pairs = false;
+  }
+  if (r->max_pairs > 0xffffffffull) r->max_pairs = 0xffffffffull;
+  r->rs = rs;
+  const uint32_t cap = n ? n : 1;
+  r->depth_plan = sort_plan(cap);
+  r->pair_plan = sort_plan((uint32_t)r->max_pairs);
+  r->scan = scan_plan(cap);
+  const uint32_t tiles = rs.tiles_x * rs.tiles_y;
+  if ((e = r->fc.ensure(sizeof(FrameConst))) != hipSuccess) return e;
+  if ((e = r->misc.ensure(kMiscWords * 4)) != hipSuccess) return e;
+  if ((e = r->counts.ensure((size_t)ceil_div(cap, kCompactBlock) * 4)) != hipSuccess) return e;
+  if ((e = r->inst.ensure((size_t)cap * 48)) != hipSuccess) return e;
+  if ((e = r->key.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  if ((e = r->val.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  if ((e = r->ids.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  if ((e = r->sort_storage.ensure(r->depth_plan.bytes)) != hipSuccess) return e;
+  if ((e = r->rect.ensure((size_t)cap * sizeof(TileRect))) != hipSuccess) return e;
+  if ((e = r->cnt.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  if ((e = r->offs.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  if ((e = r->scan_storage.ensure(r->scan.bytes)) != hipSuccess) return e;
+  if ((e = r->pkeys.ensure((size_t)r->max_pairs * 4)) != hipSuccess) return e;
+  if ((e = r->pvals.ensure((size_t)r->max_pairs * 4)) != hipSuccess) return e;
+  if ((e = r->psort.ensure(r->pair_plan.bytes)) != hipSuccess) return e;
+  if ((e = r->ranges.ensure((size_t)tiles * 8)) != hipSuccess) return e;
+  r->frame_n = n;
+  return hipSuccess;
+}
+
+// Records the frame on stream s (eager, or under capture).
+hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed) {
+  hipError_t e;
+  const uint32_t n = r->n;
+  uint32_t* misc = r->misc.as<uint32_t>();
+  if (timed) hipEventRecord(r->ev[0], s);
+  if ((e = fill_u32_launch(misc, 0u, 4, s)) != hipSuccess) return e;
+  if ((e = launch_fused_project(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
+                                r->opacity.as<float>(), r->sh.as<uint16_t>(), n, misc + kMiscVisible,
+                                r->inst.as<float4>(), r->key.as<uint32_t>(), r->val.as<uint32_t>(),
+                                r->ids.as<uint32_t>(), r->counts.as<uint32_t>(), s)) != hipSuccess)
+    return e;
+  if (timed) hipEventRecord(r->ev[1], s);
+  // keys are bits(1 - z) <= 0x3f800000: 30 significant bits
+  if ((e = sort_kv_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val.as<uint32_t>(), 30u,
+                          r->sort_storage.p, s)) != hipSuccess)
+    return e;
+  if (timed) hipEventRecord(r->ev[2], s);
+  hipEvent_t marks[2] = {r->ev[3], r->ev[4]};
+  if ((e = run_draw(r->inst.as<float4>(), r->val.as<uint32_t>(), misc + kMiscVisible, n ? n : 1, r->rs, r->max_pairs,
+                    r->rect.as<TileRect>(), r->cnt.as<uint32_t>(), r->offs.as<uint32_t>(), r->scan,
+                    r->scan_storage.p, r->pkeys.as<uint32_t>(), r->pvals.as<uint32_t>(), r->pair_plan, r->psort.p,
+                    r->ranges.as<uint32_t>(), misc + kMiscStatus, misc + kMiscPairs, reinterpret_cast<float4*>(out), s,
+                    timed ? marks : nullptr)) != hipSuccess)
+    return e;
+  if (timed) hipEventRecord(r->ev[5], s);
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vkgs_renderer_create(uint32_t capacity, vkgs_renderer** out) {
+  if (!out || capacity == 0 || capacity > (1u << 28)) return VKGS_ERR_INVALID_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return VKGS_ERR_NO_DEVICE;
+  vkgs_renderer* r = new vkgs_renderer();
+  r->capacity = capacity;
+  vkgs_raster_config_default(&r->cfg);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = r->position.ensure((size_t)capacity * 12);
+  if (e == hipSuccess) e = r->cov3d.ensure((size_t)capacity * 24);
+  if (e == hipSuccess) e = r->opacity.ensure((size_t)capacity * 4);
+  if (e == hipSuccess) e = r->sh.ensure((size_t)capacity * 96);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&r->h_fc), sizeof(FrameConst));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->capture_stream, hipStreamNonBlocking);
+  for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipEventCreate(&r->ev[i]);
+  if (e != hipSuccess) {
+    vkgs_renderer_destroy(r);
+    return VKGS_ERR_HIP;
+  }
+  *out = r;
+  return VKGS_OK;
+}
+
+int vkgs_renderer_destroy(vkgs_renderer* r) {
+  if (!r) return VKGS_ERR_INVALID_ARG;
+  hipDeviceSynchronize();
+  drop_graph(r);
+  DevBuf* bufs[] = {&r->position, &r->cov3d, &r->opacity, &r->sh,   &r->fc,   &r->misc,  &r->counts,
+                    &r->inst,     &r->key,   &r->val,     &r->ids,  &r->sort_storage, &r->rect, &r->cnt,
+                    &r->offs,     &r->scan_storage, &r->pkeys, &r->pvals, &r->psort, &r->ranges, &r->sorted_ids,
+                    &r->list_ids};
+  for (DevBuf* b : bufs) b->release();
+  if (r->h_fc) hipHostFree(r->h_fc);
+  if (r->capture_stream) hipStreamDestroy(r->capture_stream);
+  for (int i = 0; i < 6; ++i)
+    if (r->ev[i]) hipEventDestroy(r->ev[i]);
+  delete r;
+  return VKGS_OK;
+}
+
+int vkgs_renderer_load_ply_buffer(vkgs_renderer* r, const void* host_ply, uint32_t n) {
+  if (!r || (!host_ply && n)) return VKGS_ERR_INVALID_ARG;
+  if (n > r->capacity) return VKGS_ERR_CAPACITY;
+  if (n == 0) {
+    r->n = 0;
+    drop_graph(r);
+    return VKGS_OK;
+  }
+  const uint32_t* offsets = static_cast<const uint32_t*>(host_ply);
+  const uint32_t stride = offsets[59];
+  if (stride == 0) return VKGS_ERR_FORMAT;
+  for (int i = 0; i < 59; ++i)
+    if (offsets[i] >= stride) return VKGS_ERR_FORMAT;
+  const size_t bytes = 240 + (size_t)n * stride * 4;
+  void* d = nullptr;
+  VKGS_HIP_CHECK(hipMalloc(&d, bytes));
+  hipError_t e = hipMemcpy(d, host_ply, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_parse_ply(d, n, r->position.as<float>(), r->cov3d.as<float>(), r->opacity.as<float>(),
+                         r->sh.as<uint16_t>(), nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  hipFree(d);
+  if (e != hipSuccess) return VKGS_ERR_HIP;
+  r->n = n;
+  drop_graph(r);
+  return VKGS_OK;
+}
+
+int vkgs_renderer_load_ply(vkgs_renderer* r, const char* path) {
+  if (!r || !path) return VKGS_ERR_INVALID_ARG;
+  vkgs_ply_info info;
+  int st = vkgs_ply_read_header(path, &info);
+  if (st != VKGS_OK) return st;
+  if (info.point_count > r->capacity) return VKGS_ERR_CAPACITY;
+  const uint64_t bytes = vkgs_ply_buffer_bytes(&info);
+  std::vector<char> host(bytes);
+  st = vkgs_ply_read(path, &info, host.data(), bytes);
+  if (st != VKGS_OK) return st;
+  return vkgs_renderer_load_ply_buffer(r, host.data(), (uint32_t)info.point_count);
+}
+
+uint32_t vkgs_renderer_splat_count(const vkgs_renderer* r) { return r ? r->n : 0u; }
+
+int vkgs_renderer_storage(vkgs_renderer* r, float** d_position, float** d_cov3d, float** d_opacity,
+                          uint16_t** d_sh) {
+  if (!r) return VKGS_ERR_INVALID_ARG;
+  if (d_position) *d_position = r->position.as<float>();
+  if (d_cov3d) *d_cov3d = r->cov3d.as<float>();
+  if (d_opacity) *d_opacity = r->opacity.as<float>();
+  if (d_sh) *d_sh = r->sh.as<uint16_t>();
+  return VKGS_OK;
+}
+
+int vkgs_renderer_set_config(vkgs_renderer* r, const vkgs_raster_config* cfg) {
+  if (!r || !cfg) return VKGS_ERR_INVALID_ARG;
+  RasterShape rs;
+  if (!make_raster_shape(16, 16, cfg, &rs)) return VKGS_ERR_INVALID_ARG;
+  r->cfg = *cfg;
+  drop_graph(r);
+  return VKGS_OK;
+}
+
+int vkgs_renderer_render(vkgs_renderer* r, const vkgs_camera* camera, const float model[16], float* d_rgba,
+                         void* stream) {
+  if (!r || !camera || !d_rgba) return VKGS_ERR_INVALID_ARG;
+  const uint32_t width = camera->screen_size[0], height = camera->screen_size[1];
+  RasterShape rs;
+  if (!make_raster_shape(width, height, &r->cfg, &rs)) return VKGS_ERR_INVALID_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const vkgs_renderer::Key key = {r->n, width, height, r->cfg.tile_w, r->cfg.tile_h, r->cfg.t_min,
+                                  r->max_pairs, d_rgba};
+  const bool same = r->graph_valid && memcmp(&key, &r->graph_key, sizeof(key)) == 0 && !r->grow_pairs;
+  if (!same || r->timing) {
+    drop_graph(r);
+    VKGS_HIP_CHECK(hipStreamSynchronize(s));
+    VKGS_HIP_CHECK(ensure_frame(r, width, height));
+  }
+  // per-frame constants: passed by value to a one-wave kernel, so the host
+  // never waits for the previous frame before writing them
+  FrameConst fc;
+  make_frame_const(camera, model, &fc);
+  fc.tile_w = r->rs.tile_w;
+  fc.tile_h = r->rs.tile_h;
+  fc.tiles_x = r->rs.tiles_x;
+  fc.tiles_y = r->rs.tiles_y;
+  VKGS_HIP_CHECK(launch_set_frame_const(r->fc.as<FrameConst>(), fc, s));
+  if (r->timing) {
+    VKGS_HIP_CHECK(record_frame(r, d_rgba, s, true));
+    VKGS_HIP_CHECK(hipEventSynchronize(r->ev[5]));
+    float t[6] = {0};
+    hipEventElapsedTime(&t[0], r->ev[0], r->ev[1]);  // project
+    hipEventElapsedTime(&t[1], r->ev[1], r->ev[2]);  // depth sort
+    hipEventElapsedTime(&t[2], r->ev[2], r->ev[3]);  // bin
+    hipEventElapsedTime(&t[3], r->ev[3], r->ev[4]);  // tile sort + ranges
+    hipEventElapsedTime(&t[4], r->ev[4], r->ev[5]);  // raster
+    hipEventElapsedTime(&t[5], r->ev[0], r->ev[5]);  // total
+    memcpy(r->stage_ms, t, sizeof(t));
+    return VKGS_OK;
+  }
+  if (!r->graph_valid) {
+    VKGS_HIP_CHECK(hipStreamBeginCapture(r->capture_stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = record_frame(r, d_rgba, r->capture_stream, false);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(r->capture_stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      return VKGS_ERR_HIP;
+    }
+    r->graph = g;
+    VKGS_HIP_CHECK(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+    r->graph_key = key;
+    r->graph_valid = true;
+  }
+  VKGS_HIP_CHECK(hipGraphLaunch(r->exec, s));
+  return VKGS_OK;
+}
+
+int vkgs_renderer_frame_info(vkgs_renderer* r, uint32_t* visible, uint64_t* pairs, void* stream) {
+  if (!r || !r->misc.p) return VKGS_ERR_INVALID_ARG;
+  uint32_t m[4];
+  VKGS_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  VKGS_HIP_CHECK(hipMemcpy(m, r->misc.p, sizeof(m), hipMemcpyDeviceToHost));
+  if (visible) *visible = m[kMiscVisible];
+  if (pairs) *pairs = m[kMiscPairs];
+  if (m[kMiscStatus]) {
+    r->grow_pairs = true;  // the next render reallocates with twice the pair capacity
+    return VKGS_ERR_OVERFLOW;
+  }
+  return VKGS_OK;
+}
+
+int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const uint32_t** d_keys) {
+  if (!r || !r->misc.p || !d_sorted_ids) return VKGS_ERR_INVALID_ARG;
+  const uint32_t cap = r->frame_n ? r->frame_n : 1;
+  VKGS_HIP_CHECK(r->sorted_ids.ensure((size_t)cap * 4));
+  uint32_t* misc = r->misc.as<uint32_t>();
+  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val.as<uint32_t>(), misc + kMiscVisible, cap,
+                                   r->sorted_ids.as<uint32_t>(), nullptr));
+  VKGS_HIP_CHECK(hipDeviceSynchronize());
+  *d_sorted_ids = r->sorted_ids.as<uint32_t>();
+  if (d_keys) *d_keys = r->key.as<uint32_t>();
+  return VKGS_OK;
+}
+
+int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const uint32_t** d_list_ids,
+                             uint32_t* tiles_x, uint32_t* tiles_y) {
+  if (!r || !r->misc.p || !d_ranges || !d_list_ids) return VKGS_ERR_INVALID_ARG;
+  const uint64_t cap = r->max_pairs ? r->max_pairs : 1;
+  VKGS_HIP_CHECK(r->list_ids.ensure((size_t)cap * 4));
+  uint32_t* misc = r->misc.as<uint32_t>();
+  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->pvals.as<uint32_t>(), misc + kMiscPairs,
+                                   (uint32_t)cap, r->list_ids.as<uint32_t>(), nullptr));
+  VKGS_HIP_CHECK(hipDeviceSynchronize());
+  *d_ranges = r->ranges.as<uint32_t>();
+  *d_list_ids = r->list_ids.as<uint32_t>();
+  if (tiles_x) *tiles_x = r->rs.tiles_x;
+  if (tiles_y) *tiles_y = r->rs.tiles_y;
+  return VKGS_OK;
+}
+
+int vkgs_renderer_set_timing(vkgs_renderer* r, int enable) {
+  if (!r) return VKGS_ERR_INVALID_ARG;
+  r->timing = enable != 0;
+  drop_graph(r);
+  return VKGS_OK;
+}
+
+int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[6]) {
+  if (!r || !ms) return VKGS_ERR_INVALID_ARG;
+  memcpy(ms, r->stage_ms, sizeof(r->stage_ms));
+  return VKGS_OK;
+}
+
+}  // extern "C"

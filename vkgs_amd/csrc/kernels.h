@@ -1,0 +1,66 @@
+// Host-side launchers of the gfx950 kernels (implemented in *.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace vkgs {
+
+constexpr uint32_t kCompactBlock = 256;  // splats per workgroup in rank/fused compaction
+
+// parse_ply.comp
+hipError_t launch_parse_ply(const void* d_ply, uint32_t n, float* pos, float* cov3d, float* opacity, uint16_t* sh,
+                            hipStream_t s);
+
+// rank.comp with ordered compaction; counts holds ceil(n/256) words.
+hipError_t launch_rank(const FrameConst& fc, const float* pos, uint32_t n, uint32_t* d_count, uint32_t* key,
+                       uint32_t* index, uint32_t* counts, hipStream_t s);
+
+// inverse_index.comp (+ the -1 fill)
+hipError_t launch_inverse(const uint32_t* d_count, const uint32_t* index, uint32_t n, int32_t* inverse,
+                          hipStream_t s);
+
+// projection.comp
+hipError_t launch_projection(const FrameConst& fc, const float* pos, const float* cov3d, const float* opacity,
+                             const uint16_t* sh, const int32_t* inverse, uint32_t n, const uint32_t* d_count,
+                             float4* inst, uint32_t* draw_indirect, hipStream_t s);
+
+// Fused cull + project + compaction (frame constants read from device memory):
+// instance, key = bits(1 - z), val = slot, ids[slot] = splat id.
+hipError_t launch_fused_project(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
+                                const uint16_t* sh, uint32_t n, uint32_t* d_count, float4* inst, uint32_t* key,
+                                uint32_t* val, uint32_t* ids, uint32_t* counts, hipStream_t s);
+
+// Copies a FrameConst passed by value into device memory (stream-ordered).
+hipError_t launch_set_frame_const(FrameConst* d_fc, const FrameConst& fc, hipStream_t s);
+
+// Binning and raster -----------------------------------------------------
+
+struct RasterShape {
+  uint32_t width, height, tile_w, tile_h, tiles_x, tiles_y;
+  uint32_t tile_shift_x, tile_shift_y;
+  float t_min;
+};
+
+bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config* cfg, RasterShape* rs);
+
+// Per draw-order position k < *d_count: tile rectangle and tile count of
+// instance order[k] (order == nullptr: k).
+hipError_t launch_bin_count(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
+                            const RasterShape& rs, TileRect* rect, uint32_t* cnt, hipStream_t s);
+hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* order,
+                           const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
+                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s);
+hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
+                         uint32_t* ranges, hipStream_t s);
+hipError_t launch_rasterize(const float4* inst, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
+                            float4* out, hipStream_t s);
+
+// Gathers out[k] = src[idx[k]] for k < *d_count.
+hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* d_count, uint32_t max_n,
+                             uint32_t* out, hipStream_t s);
+
+}  // namespace vkgs

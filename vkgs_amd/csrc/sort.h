@@ -1,0 +1,54 @@
+// LSD radix sort of (u32 key, u32 value) pairs and an exclusive u32 scan,
+// both with the element count read from device memory, so a hipGraph can be
+// captured once for a capacity and replayed for any live count.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace vkgs {
+
+struct SortPlan {
+  uint32_t max_n = 0;
+  uint32_t blocks = 1;  // fixed grid; each block sorts a contiguous chunk
+  size_t off_alt_key = 0, off_alt_val = 0, off_hist = 0;
+  size_t bytes = 0;
+};
+
+SortPlan sort_plan(uint32_t max_n);
+
+// Stable ascending sort of min(*d_count, max_n) pairs on key bits
+// [0, key_bits), result in (keys, vals).
+hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
+                          uint32_t key_bits, void* storage, hipStream_t stream);
+
+struct ScanPlan {
+  uint32_t max_n = 0;
+  uint32_t blocks = 1;
+  size_t off_partial = 0;
+  size_t bytes = 0;
+};
+
+ScanPlan scan_plan(uint32_t max_n);
+
+// out[i] = sum(in[0..i)) for i < min(*d_count, max_n); *d_total = sum of all.
+// in and out may alias.
+hipError_t scan_exclusive_launch(const ScanPlan& plan, const uint32_t* in, uint32_t* out, const uint32_t* d_count,
+                                 uint32_t* d_total, void* storage, hipStream_t stream);
+
+// Single-workgroup exclusive scan of `total` words in place (total <= 2^22);
+// *d_sum (optional) receives the sum.
+hipError_t scan_small_launch(uint32_t* data, uint32_t total, uint32_t* d_sum, hipStream_t stream);
+
+// Fills n words with a value (graph-safe replacement for hipMemsetAsync).
+hipError_t fill_u32_launch(uint32_t* dst, uint32_t value, size_t n, hipStream_t stream);
+
+// Writes up to 16 words passed by value (graph-safe replacement for a small
+// host-to-device copy).
+struct Words16 {
+  uint32_t w[16];
+};
+hipError_t set_words_launch(uint32_t* dst, const Words16& words, uint32_t n, hipStream_t stream);
+
+}  // namespace vkgs

@@ -1,0 +1,384 @@
+// LSD radix sort (reduce-then-scan) and device-wide exclusive scan for gfx950.
+//
+// Replaces the reference's external sorter (vrdxCmdSortKeyValueIndirect,
+// engine.cc:1218-1219; "reduce-then-scan", DETAILS.md:26-30).  Each pass of
+// up to 8 key bits runs three kernels:
+//   upsweep   : every block counts the digits of its contiguous chunk into a
+//               per-wave LDS histogram (one LDS atomic per distinct digit per
+//               64 keys, found with 8 wave ballots), then writes hist[d][block];
+//   scan      : one workgroup turns hist into exclusive global offsets;
+//   downsweep : every block walks its chunk in 2048-key sub-tiles, ranks keys
+//               stably (wave multisplit: ballots + LDS counters, then a
+//               cross-wave prefix), reorders the sub-tile in LDS and writes
+//               runs of equal digits contiguously to global memory.
+// No block ever waits on another block, so the sort cannot hang on forward
+// progress.  Counts are read from device memory (indirect semantics); the grid
+// is fixed by the capacity and each block derives its chunk from the count.
+#pragma clang fp contract(off)
+
+#include "common.h"
+#include "sort.h"
+
+namespace vkgs {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kIpt = 8;
+constexpr int kTile = kThreads * kIpt;  // keys per sub-tile
+constexpr int kMaxBlocks = 1024;
+constexpr int kScanThreads = 1024;
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Mask of the lanes whose `bits`-bit digit equals this lane's (valid lanes only).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint32_t bits, bool valid) {
+  uint64_t m = __ballot(valid);
+  for (uint32_t b = 0; b < bits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan across the workgroup; s_tmp holds one word per wave.
+template <int kBlock>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+  constexpr int kNw = kBlock / kWave;
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  const uint32_t incl = wave_inclusive_scan(v);
+  if (lane == kWave - 1) s_tmp[w] = incl;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kNw; ++i) {
+    const uint32_t t = s_tmp[i];
+    off += (i < w) ? t : 0u;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+// Contiguous chunk of block blockIdx.x; chunks are multiples of kTile.
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t& begin, uint32_t& end) {
+  const uint64_t per = ((uint64_t)n + gridDim.x - 1) / gridDim.x;
+  const uint64_t chunk = (per + kTile - 1) / kTile * kTile;
+  const uint64_t b = (uint64_t)blockIdx.x * chunk;
+  begin = (uint32_t)(b < n ? b : n);
+  end = (uint32_t)(b + chunk < n ? b + chunk : n);
+}
+
+__global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                         uint32_t shift, uint32_t bits,
+                                                         uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[kWaves][256];
+  const uint32_t tid = threadIdx.x, w = tid / kWave;
+  for (uint32_t i = tid; i < kWaves * 256; i += kThreads) (&s_h[0][0])[i] = 0u;
+  __syncthreads();
+  const uint32_t n = min(*d_count, max_n);
+  uint32_t begin, end;
+  block_range(n, begin, end);
+  const uint32_t mask = (1u << bits) - 1u;
+  for (uint32_t base = begin; base < end; base += kTile) {
+    uint32_t k[kIpt];
+#pragma unroll
+    for (int r = 0; r < kIpt; ++r) {
+      const uint32_t i = base + r * kThreads + tid;
+      k[r] = i < end ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kIpt; ++r) {
+      const bool valid = base + r * kThreads + tid < end;
+      const uint32_t d = (k[r] >> shift) & mask;
+      const uint64_t peers = match_digit(d, bits, valid);
+      if (valid && mbcnt(peers) == 0) atomicAdd(&s_h[w][d], (uint32_t)__popcll(peers));
+    }
+  }
+  __syncthreads();
+  if (tid <= mask) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) s += s_h[i][tid];
+    hist[tid * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                           uint32_t shift, uint32_t bits,
+                                                           const uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_wcnt[kWaves][256];
+  __shared__ uint32_t s_start[256];
+  __shared__ uint32_t s_tmp[kWaves];
+  __shared__ uint32_t s_key[kTile];
+  __shared__ uint32_t s_val[kTile];
+
+  const uint32_t tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+  const uint32_t n = min(*d_count, max_n);
+  uint32_t begin, end;
+  block_range(n, begin, end);
+  const uint32_t mask = (1u << bits) - 1u;
+  s_base[tid] = tid <= mask ? hist[tid * gridDim.x + blockIdx.x] : 0u;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) s_wcnt[i][tid] = 0u;
+  __syncthreads();
+
+  for (uint32_t base = begin; base < end; base += kTile) {
+    const uint32_t cnt = min((uint32_t)kTile, end - base);
+    const uint32_t seg = base + w * (kWave * kIpt);
+    uint32_t k[kIpt], v[kIpt], rank[kIpt];
+#pragma unroll
+    for (int r = 0; r < kIpt; ++r) {
+      const uint32_t i = seg + r * kWave + lane;
+      const bool valid = i < end;
+      k[r] = valid ? kin[i] : 0u;
+      v[r] = valid ? vin[i] : 0u;
+    }
+    // stable rank inside this wave's segment, 64 keys at a time
+#pragma unroll
+    for (int r = 0; r < kIpt; ++r) {
+      const bool valid = seg + r * kWave + lane < end;
+      const uint32_t d = (k[r] >> shift) & mask;
+      const uint64_t peers = match_digit(d, bits, valid);
+      const uint32_t below = mbcnt(peers);
+      uint32_t old = 0;
+      if (valid && below == 0) old = atomicAdd(&s_wcnt[w][d], (uint32_t)__popcll(peers));
+      const int leader = peers ? (int)__ffsll((unsigned long long)peers) - 1 : (int)lane;
+      old = __shfl(old, leader, kWave);
+      rank[r] = old + below;
+    }
+    __syncthreads();
+    // per digit: exclusive prefix over waves, then over digits
+    uint32_t total_d = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) {
+      const uint32_t c = s_wcnt[i][tid];
+      s_wcnt[i][tid] = total_d;
+      total_d += c;
+    }
+    uint32_t all;
+    const uint32_t start = block_exclusive_scan<kThreads>(total_d, s_tmp, &all);
+    s_start[tid] = start;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kIpt; ++r) {
+      const bool valid = seg + r * kWave + lane < end;
+      if (valid) {
+        const uint32_t d = (k[r] >> shift) & mask;
+        const uint32_t pos = s_start[d] + s_wcnt[w][d] + rank[r];
+        if (pos < (uint32_t)kTile) {
+          s_key[pos] = k[r];
+          s_val[pos] = v[r];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+      const uint32_t idx = j * kThreads + tid;
+      if (idx < cnt) {
+        const uint32_t kk = s_key[idx];
+        const uint32_t d = (kk >> shift) & mask;
+        const uint32_t dst = s_base[d] + idx - s_start[d];
+        if (dst < n) {
+          kout[dst] = kk;
+          vout[dst] = s_val[idx];
+        }
+      }
+    }
+    __syncthreads();
+    s_base[tid] += total_d;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) s_wcnt[i][tid] = 0u;
+    __syncthreads();
+  }
+}
+
+// Exclusive scan of `total` words in place by one workgroup.
+__global__ __launch_bounds__(kScanThreads) void scan_single_block(uint32_t* __restrict__ data, uint32_t total,
+                                                                  uint32_t* __restrict__ d_sum) {
+  __shared__ uint32_t s_tmp[kScanThreads / kWave];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per = (total + kScanThreads - 1) / kScanThreads;
+  const uint32_t start = min(tid * per, total), stop = min(start + per, total);
+  uint32_t s = 0;
+  for (uint32_t i = start; i < stop; ++i) s += data[i];
+  uint32_t all;
+  uint32_t run = block_exclusive_scan<kScanThreads>(s, s_tmp, &all);
+  for (uint32_t i = start; i < stop; ++i) {
+    const uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+  if (d_sum != nullptr && tid == 0) *d_sum = all;
+}
+
+__global__ __launch_bounds__(kThreads) void scan_reduce(const uint32_t* __restrict__ in,
+                                                        const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                        uint32_t* __restrict__ partial) {
+  __shared__ uint32_t s_tmp[kWaves];
+  const uint32_t n = min(*d_count, max_n);
+  uint32_t begin, end;
+  block_range(n, begin, end);
+  uint32_t s = 0;
+  for (uint32_t i = begin + threadIdx.x; i < end; i += kThreads) s += in[i];
+  uint32_t all;
+  block_exclusive_scan<kThreads>(s, s_tmp, &all);
+  if (threadIdx.x == 0) partial[blockIdx.x] = all;
+}
+
+__global__ __launch_bounds__(kThreads) void scan_apply(const uint32_t* in, uint32_t* out,
+                                                       const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                       const uint32_t* __restrict__ partial) {
+  __shared__ uint32_t s_tmp[kWaves];
+  const uint32_t n = min(*d_count, max_n);
+  uint32_t begin, end;
+  block_range(n, begin, end);
+  uint32_t carry = partial[blockIdx.x];
+  for (uint32_t base = begin; base < end; base += kTile) {
+    const uint32_t first = base + threadIdx.x * kIpt;
+    uint32_t x[kIpt];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+      x[j] = first + j < end ? in[first + j] : 0u;
+      s += x[j];
+    }
+    uint32_t all;
+    uint32_t run = carry + block_exclusive_scan<kThreads>(s, s_tmp, &all);
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+      if (first + j < end) out[first + j] = run;
+      run += x[j];
+    }
+    carry += all;
+  }
+}
+
+__global__ void copy_pairs(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ vs, uint32_t* __restrict__ kd,
+                           uint32_t* __restrict__ vd, const uint32_t* __restrict__ d_count, uint32_t max_n) {
+  const uint32_t n = min(*d_count, max_n);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    kd[i] = ks[i];
+    vd[i] = vs[i];
+  }
+}
+
+__global__ void fill_u32(uint32_t* __restrict__ dst, uint32_t value, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = value;
+}
+
+__global__ void set_words(uint32_t* __restrict__ dst, Words16 words, uint32_t n) {
+  if (threadIdx.x < n) dst[threadIdx.x] = words.w[threadIdx.x];
+}
+
+}  // namespace
+
+SortPlan sort_plan(uint32_t max_n) {
+  SortPlan p;
+  p.max_n = max_n;
+  uint32_t tiles = ceil_div(max_n ? max_n : 1, kTile);
+  p.blocks = tiles < (uint32_t)kMaxBlocks ? tiles : (uint32_t)kMaxBlocks;
+  size_t off = 0;
+  p.off_alt_key = off;
+  off = align_up(off + (size_t)max_n * 4, 256);
+  p.off_alt_val = off;
+  off = align_up(off + (size_t)max_n * 4, 256);
+  p.off_hist = off;
+  off = align_up(off + (size_t)256 * p.blocks * 4, 256);
+  p.bytes = off;
+  return p;
+}
+
+hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
+                          uint32_t key_bits, void* storage, hipStream_t stream) {
+  if (plan.max_n == 0 || key_bits == 0) return hipSuccess;
+  char* base = static_cast<char*>(storage);
+  uint32_t* alt_k = reinterpret_cast<uint32_t*>(base + plan.off_alt_key);
+  uint32_t* alt_v = reinterpret_cast<uint32_t*>(base + plan.off_alt_val);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(base + plan.off_hist);
+  uint32_t* src_k = keys;
+  uint32_t* src_v = vals;
+  uint32_t* dst_k = alt_k;
+  uint32_t* dst_v = alt_v;
+  if (key_bits > 32) key_bits = 32;
+  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+    const uint32_t bits = key_bits - shift < 8 ? key_bits - shift : 8;
+    sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
+    scan_single_block<<<1, kScanThreads, 0, stream>>>(hist, plan.blocks << bits, nullptr);
+    sort_downsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, plan.max_n, shift,
+                                                         bits, hist);
+    uint32_t* tk = src_k;
+    uint32_t* tv = src_v;
+    src_k = dst_k;
+    src_v = dst_v;
+    dst_k = tk;
+    dst_v = tv;
+  }
+  if (src_k != keys) {
+    copy_pairs<<<1024, 256, 0, stream>>>(src_k, src_v, keys, vals, d_count, plan.max_n);
+  }
+  return hipGetLastError();
+}
+
+ScanPlan scan_plan(uint32_t max_n) {
+  ScanPlan p;
+  p.max_n = max_n;
+  uint32_t tiles = ceil_div(max_n ? max_n : 1, kTile);
+  p.blocks = tiles < (uint32_t)kMaxBlocks ? tiles : (uint32_t)kMaxBlocks;
+  p.off_partial = 0;
+  p.bytes = align_up((size_t)p.blocks * 4, 256);
+  return p;
+}
+
+hipError_t scan_exclusive_launch(const ScanPlan& plan, const uint32_t* in, uint32_t* out, const uint32_t* d_count,
+                                 uint32_t* d_total, void* storage, hipStream_t stream) {
+  uint32_t* partial = reinterpret_cast<uint32_t*>(static_cast<char*>(storage) + plan.off_partial);
+  scan_reduce<<<plan.blocks, kThreads, 0, stream>>>(in, d_count, plan.max_n, partial);
+  scan_single_block<<<1, kScanThreads, 0, stream>>>(partial, plan.blocks, d_total);
+  scan_apply<<<plan.blocks, kThreads, 0, stream>>>(in, out, d_count, plan.max_n, partial);
+  return hipGetLastError();
+}
+
+hipError_t scan_small_launch(uint32_t* data, uint32_t total, uint32_t* d_sum, hipStream_t stream) {
+  scan_single_block<<<1, kScanThreads, 0, stream>>>(data, total, d_sum);
+  return hipGetLastError();
+}
+
+hipError_t fill_u32_launch(uint32_t* dst, uint32_t value, size_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  fill_u32<<<(uint32_t)blocks, 256, 0, stream>>>(dst, value, n);
+  return hipGetLastError();
+}
+
+hipError_t set_words_launch(uint32_t* dst, const Words16& words, uint32_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  if (n > 16) return hipErrorInvalidValue;
+  set_words<<<1, 64, 0, stream>>>(dst, words, n);
+  return hipGetLastError();
+}
+
+}  // namespace vkgs
