@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the splat path on one MI355X.
+
+Workload (default): plain forward frame of 6M splats at 1600x900 -- 256
+anisotropic clusters in an 80 x 20 x 80 m box plus 20% ground-plane points,
+orbit camera at r = 60 m, phi = theta = 45 deg, fovy 60 deg.  One step is one
+frame: cull + project + compact, depth sort, tile binning, tile sort, ranges,
+raster; replayed from a hipGraph with the splat storage resident in HBM.
+
+Prints one JSON line with FPS (`value`), the per-stage split, the roofline of
+the dominant kernel measured with HIP events over the timed region, and the
+CPU oracle timed on the host cores as `cpu_baseline`.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+METRIC = "FPS @6M splats 1600x900; batched vs plain fwd+bwd views/s; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+RASTER_FLOPS_PER_EVAL = 20  # per (pixel, splat) evaluation, exp counted as 1
+
+
+def parse_args(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="c3_6m")
+    ap.add_argument("--tile", default="16x16")
+    ap.add_argument("--t-min", type=float, default=1.0 / 8192.0)
+    ap.add_argument("--timed-frames", type=int, default=20, help="eager frames with per-stage HIP events")
+    ap.add_argument("--cpu-frames", type=int, default=2, help="oracle frames for cpu_baseline (0 = skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args(argv)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2] if xs else float("nan")
+
+
+def main(argv=None) -> int:
+    args = parse_args(argv)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N with N > 1 is not supported",
+                          "n_gpus": args.gpus}))
+        return 2
+    import numpy as np
+    import torch
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        return 1
+    from vkgs_amd import pipeline as P
+    from vkgs_amd import scene as S
+
+    wl = S.WORKLOADS[args.workload]
+    tw, th = (int(x) for x in args.tile.split("x"))
+    log(f"generating {wl.name}: {wl.n} splats")
+    t0 = time.perf_counter()
+    sc = wl.make()
+    log(f"generated in {time.perf_counter() - t0:.1f}s")
+    cam = S.orbit_camera(wl.width, wl.height, wl.radius)
+
+    r = P.Renderer(sc.n)
+    r.load_buffer(sc.ply_buffer(), sc.n)
+    r.set_config(tw, th, args.t_min)
+    out = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
+    for _ in range(3):  # grows the pair storage if the first frame overflowed it
+        r.render(cam, out)
+        status, visible, pairs = r.frame_info()
+        if status == 0:
+            break
+    if status != 0:
+        print(json.dumps({"error": f"pair list overflow (status {status})"}))
+        return 1
+    log(f"visible {visible} pairs {pairs}")
+
+    # ---- timed region: graph replays ----
+    for _ in range(args.warmup):
+        r.render(cam, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.render(cam, out)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    fps = 1e3 / ms
+    log(f"{ms:.3f} ms/frame, {fps:.1f} FPS")
+
+    # ---- per-stage split with HIP events on the renderer's stream ----
+    r.set_timing(True)
+    stages = {k: [] for k in P.STAGES}
+    for _ in range(args.timed_frames):
+        r.render(cam, out)
+        for k, v in r.stage_ms().items():
+            if k in stages:
+                stages[k].append(v)
+    r.set_timing(False)
+    stage_ms = {k: round(median(v), 5) for k, v in stages.items()}
+    log("stage ms", stage_ms)
+
+    # ---- algorithmic work of the frame ----
+    ranges, _ = r.tile_lists()
+    lens = (ranges[:, 1] - ranges[:, 0]).astype(np.float64)
+    tiles_x = (wl.width + tw - 1) // tw
+    tx = np.arange(ranges.shape[0]) % tiles_x
+    ty = np.arange(ranges.shape[0]) // tiles_x
+    pix = (np.minimum(tw, wl.width - tx * tw) * np.minimum(th, wl.height - ty * th)).astype(np.float64)
+    p_eval = float((pix * lens).sum())
+    n, V, K = sc.n, visible, pairs
+    b_proj = 12.0 * n + 172.0 * V  # positions of all + cov3d/opacity/SH read + 48-B record per visible
+    t_proj = stage_ms["project"] * 1e-3
+    f_raster = RASTER_FLOPS_PER_EVAL * p_eval
+    t_raster = stage_ms["raster"] * 1e-3
+    proj_gbs = b_proj / t_proj / 1e9
+    raster_tflops = f_raster / t_raster / 1e12
+    # reference-shaped sort traffic: 4 passes x (read + write) of 8-B pairs over V, 2 passes over K
+    b_sort = 64.0 * V + 32.0 * K
+    b_bin = 12.0 * K + 8.0 * K + 8.0 * ranges.shape[0]
+    t_ps = (stage_ms["count"] + stage_ms["project"] + stage_ms["depth_sort"] + stage_ms["bin"]
+            + stage_ms["tile_sort"]) * 1e-3
+    ps_gbs = (b_proj + b_sort + b_bin) / t_ps / 1e9
+
+    if t_raster >= t_proj:
+        roof = {"kernel": "rasterize_kernel", "bound": "valu-fp32", "achieved": round(raster_tflops, 3),
+                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(raster_tflops / FP32_PEAK_TFLOPS, 4),
+                "traffic": None, "avg_ms": stage_ms["raster"],
+                "work": f"{RASTER_FLOPS_PER_EVAL} flop x {p_eval:.4g} (pixel, splat) evaluations"}
+    else:
+        roof = {"kernel": "fused_write_kernel", "bound": "hbm", "achieved": round(proj_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_ms": stage_ms["project"], "work": f"12N + 172V = {b_proj:.4g} B"}
+    roof_proj = {"kernel": "fused_write_kernel", "bound": "hbm", "achieved": round(proj_gbs, 1),
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4)}
+
+    result = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain forward {wl.name}: {wl.n} splats, {wl.width}x{wl.height}, orbit r={wl.radius} m",
+                   "splats": n, "width": wl.width, "height": wl.height, "visible": V, "pairs": K,
+                   "tile": f"{tw}x{th}", "t_min": args.t_min, "graph": True},
+        "stages_ms": stage_ms,
+        "roofline": roof,
+        "roofline_projection": roof_proj,
+        "hbm_frac_projection_sort": round(ps_gbs / HBM_PEAK_GBS, 4),
+    }
+
+    # ---- CPU baseline: the oracle's full frame on the host cores ----
+    if not args.no_cpu_baseline and args.cpu_frames > 0:
+        try:
+            import oracle
+
+            st = r.storage()
+            pos = st.position.cpu().numpy()
+            cov = st.cov3d.cpu().numpy()
+            opa = st.opacity.cpu().numpy()
+            sh = st.sh.cpu().numpy().view(np.uint16)
+            del st
+            oracle.lib()
+            times = []
+            for _ in range(args.cpu_frames):
+                c0 = time.perf_counter()
+                ref = oracle.frame(cam, pos, cov, opa, sh, wl.width, wl.height)
+                times.append(time.perf_counter() - c0)
+            assert ref["visible"] == V
+            cpu_fps = 1.0 / median(times)
+            result["cpu_baseline"] = {
+                "value": round(cpu_fps, 4), "unit": "frames/s", "cores": oracle.num_threads(), "kind": "port",
+                "sample": f"{args.cpu_frames} full frames of the same {wl.name} workload (rank, stable sort, inverse, "
+                          "projection, back-to-front draw) by the C oracle, median"}
+            log(f"cpu oracle {median(times):.2f} s/frame on {oracle.num_threads()} threads")
+        except Exception as e:  # the baseline never blocks the GPU figure
+            result["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(result), flush=True)
+    r.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

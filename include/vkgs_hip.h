@@ -219,8 +219,9 @@ int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const 
 /* Enable per-stage hipEvent timing (the reference's 12-timestamp breakdown,
  * engine.cc:1031-1045); disables graph replay while on. */
 int vkgs_renderer_set_timing(vkgs_renderer* r, int enable);
-/* ms of the last timed frame: [project, sort, bin, tile_sort, raster, total] */
-int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[6]);
+/* ms of the last timed frame: [visible count + scan, project + compact,
+ * depth sort, bin, tile sort + ranges, raster, total, 0] */
+int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[8]);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# VKGS_ORACLE_LIB points the tests at another build of vkgs_oracle.c (used to
+# check that the oracle tests fail on a deliberately mutated oracle)
+LIB_PATH = os.environ.get("VKGS_ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
 
 _lib = None
 

@@ -17,8 +17,8 @@ struct DrawPlan {
   RasterShape rs{};
   SortPlan sort{};
   ScanPlan scan{};
-  size_t off_hdr = 0, off_rect = 0, off_cnt = 0, off_offs = 0, off_scan = 0, off_keys = 0, off_vals = 0,
-         off_sort = 0, off_ranges = 0, bytes = 0;
+  size_t off_hdr = 0, off_rec = 0, off_rect = 0, off_cnt = 0, off_offs = 0, off_scan = 0, off_keys = 0,
+         off_vals = 0, off_sort = 0, off_ranges = 0, bytes = 0;
 };
 
 // Header words at the start of the draw storage.
@@ -45,6 +45,8 @@ bool draw_plan(uint32_t max_inst, uint32_t width, uint32_t height, uint64_t max_
   size_t off = 0;
   p->off_hdr = off;
   off = align_up(off + kHdrWords * 4, 256);
+  p->off_rec = off;
+  off = align_up(off + (size_t)max_inst * 48, 256);
   p->off_rect = off;
   off = align_up(off + (size_t)max_inst * sizeof(TileRect), 256);
   p->off_cnt = off;
@@ -75,24 +77,23 @@ int bits_for(uint32_t v) {  // bits needed for values in [0, v)
 
 // Shared by vkgs_draw and the renderer: bin, sort by tile, ranges, raster.
 hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
-                    const RasterShape& rs, uint64_t max_pairs, TileRect* rect, uint32_t* cnt, uint32_t* offs,
-                    const ScanPlan& scp, void* scan_storage, uint32_t* keys, uint32_t* vals, const SortPlan& sp,
-                    void* sort_storage, uint32_t* ranges, uint32_t* hdr_status, uint32_t* hdr_pairs, float4* out,
-                    hipStream_t s, hipEvent_t* marks) {
+                    const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
+                    hipEvent_t* marks) {
   hipError_t e;
-  if ((e = fill_u32_launch(hdr_status, 0u, 1, s)) != hipSuccess) return e;
-  if ((e = launch_bin_count(inst, order, d_count, max_inst, rs, rect, cnt, s)) != hipSuccess) return e;
-  if ((e = scan_exclusive_launch(scp, cnt, offs, d_count, hdr_pairs, scan_storage, s)) != hipSuccess) return e;
-  if ((e = launch_bin_emit(rect, cnt, offs, order, d_count, max_inst, rs, max_pairs, keys, vals, hdr_status, s)) !=
+  if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
+  if ((e = launch_bin_count(inst, order, d_count, max_inst, rs, b.rec, b.rect, b.cnt, s)) != hipSuccess) return e;
+  if ((e = scan_exclusive_launch(b.scan, b.cnt, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess) return e;
+  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, d_count, max_inst, rs, max_pairs, b.keys, b.vals, b.status, s)) !=
       hipSuccess)
     return e;
   if (marks) hipEventRecord(marks[0], s);
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
-  if ((e = sort_kv_launch(sp, hdr_pairs, keys, vals, (uint32_t)bits_for(tiles), sort_storage, s)) != hipSuccess)
+  if ((e = sort_kv_launch(b.sort, b.pairs, b.keys, b.vals, (uint32_t)bits_for(tiles), b.sort_storage, s)) !=
+      hipSuccess)
     return e;
-  if ((e = launch_ranges(keys, hdr_pairs, max_pairs, tiles, ranges, s)) != hipSuccess) return e;
+  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
   if (marks) hipEventRecord(marks[1], s);
-  return launch_rasterize(inst, vals, ranges, rs, out, s);
+  return launch_rasterize(b.rec, b.vals, b.ranges, rs, out, s);
 }
 
 }  // namespace vkgs
@@ -109,9 +110,7 @@ int vkgs_parse_ply(const void* d_ply, uint32_t n, float* d_position, float* d_co
   return VKGS_OK;
 }
 
-size_t vkgs_rank_storage_bytes(uint32_t max_n) {
-  return align_up((size_t)ceil_div(max_n ? max_n : 1, kCompactBlock) * 4, 256);
-}
+size_t vkgs_rank_storage_bytes(uint32_t max_n) { return compact_storage_bytes(max_n); }
 
 int vkgs_rank(const vkgs_camera* camera, const float model[16], const float* d_position, uint32_t n,
               uint32_t* d_visible_count, uint32_t* d_key, uint32_t* d_index, void* d_storage, size_t storage_bytes,
@@ -188,13 +187,22 @@ int vkgs_draw(const float* d_instances, const uint32_t* d_draw_indirect, uint32_
   hw.w[kHdrOffRanges + 1] = (uint32_t)((uint64_t)p.off_ranges >> 32);
   hipStream_t s = as_stream(stream);
   VKGS_HIP_CHECK(set_words_launch(hdr, hw, 9, s));
+  DrawBuffers db;
+  db.rec = reinterpret_cast<float4*>(b + p.off_rec);
+  db.rect = reinterpret_cast<TileRect*>(b + p.off_rect);
+  db.cnt = reinterpret_cast<uint32_t*>(b + p.off_cnt);
+  db.offs = reinterpret_cast<uint32_t*>(b + p.off_offs);
+  db.scan = p.scan;
+  db.scan_storage = b + p.off_scan;
+  db.keys = reinterpret_cast<uint32_t*>(b + p.off_keys);
+  db.vals = reinterpret_cast<uint32_t*>(b + p.off_vals);
+  db.sort = p.sort;
+  db.sort_storage = b + p.off_sort;
+  db.ranges = reinterpret_cast<uint32_t*>(b + p.off_ranges);
+  db.status = hdr + kHdrStatus;
+  db.pairs = hdr + kHdrPairs;
   VKGS_HIP_CHECK(run_draw(reinterpret_cast<const float4*>(d_instances), nullptr, d_draw_indirect + 8, max_instances,
-                          p.rs, p.max_pairs, reinterpret_cast<TileRect*>(b + p.off_rect),
-                          reinterpret_cast<uint32_t*>(b + p.off_cnt), reinterpret_cast<uint32_t*>(b + p.off_offs),
-                          p.scan, b + p.off_scan, reinterpret_cast<uint32_t*>(b + p.off_keys),
-                          reinterpret_cast<uint32_t*>(b + p.off_vals), p.sort, b + p.off_sort,
-                          reinterpret_cast<uint32_t*>(b + p.off_ranges), hdr + kHdrStatus, hdr + kHdrPairs,
-                          reinterpret_cast<float4*>(d_rgba), s, nullptr));
+                          p.rs, p.max_pairs, db, reinterpret_cast<float4*>(d_rgba), s, nullptr));
   return VKGS_OK;
 }
 

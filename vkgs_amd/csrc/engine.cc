@@ -6,7 +6,7 @@
 // on a stream.  The recording is captured once per (N, width, height, raster
 // config, output pointer) into a hipGraph and replayed; the camera and model
 // reach the kernels through a device-resident FrameConst written by a
-// stream-ordered copy before each replay, so no host sync happens in a frame.
+// one-wave kernel launched before each replay, so no host sync happens in a frame.
 #include <string.h>
 
 #include <vector>
@@ -15,19 +15,13 @@
 #include "kernels.h"
 #include "sort.h"
 
-namespace vkgs {
-hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
-                    const RasterShape& rs, uint64_t max_pairs, TileRect* rect, uint32_t* cnt, uint32_t* offs,
-                    const ScanPlan& scp, void* scan_storage, uint32_t* keys, uint32_t* vals, const SortPlan& sp,
-                    void* sort_storage, uint32_t* ranges, uint32_t* hdr_status, uint32_t* hdr_pairs, float4* out,
-                    hipStream_t s, hipEvent_t* marks);
-}
-
 using namespace vkgs;
 
 namespace {
 
 enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscWords = 64 };
+constexpr int kEvents = 7;
+constexpr int kStages = 8;
 
 struct DevBuf {
   void* p = nullptr;
@@ -64,14 +58,13 @@ struct vkgs_renderer {
   DevBuf fc, misc, counts, inst, key, val, ids, sort_storage;
   // binning sized by (n, pairs, tiles)
   uint64_t max_pairs = 0;
-  DevBuf rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges;
+  DevBuf rec, rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges;
   DevBuf sorted_ids, list_ids;
   SortPlan depth_plan{}, pair_plan{};
   ScanPlan scan{};
   RasterShape rs{};
   vkgs_raster_config cfg{};
 
-  FrameConst* h_fc = nullptr;  // pinned staging for the per-frame constants
   hipStream_t capture_stream = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -85,8 +78,8 @@ struct vkgs_renderer {
   bool grow_pairs = false;
 
   bool timing = false;
-  hipEvent_t ev[6] = {};
-  float stage_ms[6] = {};
+  hipEvent_t ev[kEvents] = {};
+  float stage_ms[kStages] = {};
 };
 
 namespace {
@@ -118,12 +111,13 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   if ((e = r->fc.ensure(sizeof(FrameConst))) != hipSuccess) return e;
   if ((e = r->misc.ensure(kMiscWords * 4)) != hipSuccess) return e;
-  if ((e = r->counts.ensure((size_t)ceil_div(cap, kCompactBlock) * 4)) != hipSuccess) return e;
+  if ((e = r->counts.ensure(compact_storage_bytes(cap))) != hipSuccess) return e;
   if ((e = r->inst.ensure((size_t)cap * 48)) != hipSuccess) return e;
   if ((e = r->key.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->val.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->ids.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->sort_storage.ensure(r->depth_plan.bytes)) != hipSuccess) return e;
+  if ((e = r->rec.ensure((size_t)cap * 48)) != hipSuccess) return e;
   if ((e = r->rect.ensure((size_t)cap * sizeof(TileRect))) != hipSuccess) return e;
   if ((e = r->cnt.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->offs.ensure((size_t)cap * 4)) != hipSuccess) return e;
@@ -136,6 +130,25 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   return hipSuccess;
 }
 
+DrawBuffers draw_buffers(vkgs_renderer* r) {
+  DrawBuffers b;
+  uint32_t* misc = r->misc.as<uint32_t>();
+  b.rec = r->rec.as<float4>();
+  b.rect = r->rect.as<TileRect>();
+  b.cnt = r->cnt.as<uint32_t>();
+  b.offs = r->offs.as<uint32_t>();
+  b.scan = r->scan;
+  b.scan_storage = r->scan_storage.p;
+  b.keys = r->pkeys.as<uint32_t>();
+  b.vals = r->pvals.as<uint32_t>();
+  b.sort = r->pair_plan;
+  b.sort_storage = r->psort.p;
+  b.ranges = r->ranges.as<uint32_t>();
+  b.status = misc + kMiscStatus;
+  b.pairs = misc + kMiscPairs;
+  return b;
+}
+
 // Records the frame on stream s (eager, or under capture).
 hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed) {
   hipError_t e;
@@ -143,25 +156,26 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
   uint32_t* misc = r->misc.as<uint32_t>();
   if (timed) hipEventRecord(r->ev[0], s);
   if ((e = fill_u32_launch(misc, 0u, 4, s)) != hipSuccess) return e;
-  if ((e = launch_fused_project(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
-                                r->opacity.as<float>(), r->sh.as<uint16_t>(), n, misc + kMiscVisible,
-                                r->inst.as<float4>(), r->key.as<uint32_t>(), r->val.as<uint32_t>(),
-                                r->ids.as<uint32_t>(), r->counts.as<uint32_t>(), s)) != hipSuccess)
+  if ((e = launch_fused_count(r->fc.as<FrameConst>(), r->position.as<float>(), n, misc + kMiscVisible,
+                              r->counts.as<uint32_t>(), s)) != hipSuccess)
     return e;
   if (timed) hipEventRecord(r->ev[1], s);
+  if ((e = launch_fused_write(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
+                              r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->inst.as<float4>(),
+                              r->key.as<uint32_t>(), r->val.as<uint32_t>(), r->ids.as<uint32_t>(),
+                              r->counts.as<uint32_t>(), s)) != hipSuccess)
+    return e;
+  if (timed) hipEventRecord(r->ev[2], s);
   // keys are bits(1 - z) <= 0x3f800000: 30 significant bits
   if ((e = sort_kv_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val.as<uint32_t>(), 30u,
                           r->sort_storage.p, s)) != hipSuccess)
     return e;
-  if (timed) hipEventRecord(r->ev[2], s);
-  hipEvent_t marks[2] = {r->ev[3], r->ev[4]};
+  if (timed) hipEventRecord(r->ev[3], s);
+  hipEvent_t marks[2] = {r->ev[4], r->ev[5]};
   if ((e = run_draw(r->inst.as<float4>(), r->val.as<uint32_t>(), misc + kMiscVisible, n ? n : 1, r->rs, r->max_pairs,
-                    r->rect.as<TileRect>(), r->cnt.as<uint32_t>(), r->offs.as<uint32_t>(), r->scan,
-                    r->scan_storage.p, r->pkeys.as<uint32_t>(), r->pvals.as<uint32_t>(), r->pair_plan, r->psort.p,
-                    r->ranges.as<uint32_t>(), misc + kMiscStatus, misc + kMiscPairs, reinterpret_cast<float4*>(out), s,
-                    timed ? marks : nullptr)) != hipSuccess)
+                    draw_buffers(r), reinterpret_cast<float4*>(out), s, timed ? marks : nullptr)) != hipSuccess)
     return e;
-  if (timed) hipEventRecord(r->ev[5], s);
+  if (timed) hipEventRecord(r->ev[6], s);
   return hipSuccess;
 }
 
@@ -181,9 +195,8 @@ int vkgs_renderer_create(uint32_t capacity, vkgs_renderer** out) {
   if (e == hipSuccess) e = r->cov3d.ensure((size_t)capacity * 24);
   if (e == hipSuccess) e = r->opacity.ensure((size_t)capacity * 4);
   if (e == hipSuccess) e = r->sh.ensure((size_t)capacity * 96);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&r->h_fc), sizeof(FrameConst));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->capture_stream, hipStreamNonBlocking);
-  for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipEventCreate(&r->ev[i]);
+  for (int i = 0; i < kEvents && e == hipSuccess; ++i) e = hipEventCreate(&r->ev[i]);
   if (e != hipSuccess) {
     vkgs_renderer_destroy(r);
     return VKGS_ERR_HIP;
@@ -197,13 +210,12 @@ int vkgs_renderer_destroy(vkgs_renderer* r) {
   hipDeviceSynchronize();
   drop_graph(r);
   DevBuf* bufs[] = {&r->position, &r->cov3d, &r->opacity, &r->sh,   &r->fc,   &r->misc,  &r->counts,
-                    &r->inst,     &r->key,   &r->val,     &r->ids,  &r->sort_storage, &r->rect, &r->cnt,
+                    &r->inst,     &r->key,   &r->val,     &r->ids,  &r->sort_storage, &r->rec, &r->rect, &r->cnt,
                     &r->offs,     &r->scan_storage, &r->pkeys, &r->pvals, &r->psort, &r->ranges, &r->sorted_ids,
                     &r->list_ids};
   for (DevBuf* b : bufs) b->release();
-  if (r->h_fc) hipHostFree(r->h_fc);
   if (r->capture_stream) hipStreamDestroy(r->capture_stream);
-  for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < kEvents; ++i)
     if (r->ev[i]) hipEventDestroy(r->ev[i]);
   delete r;
   return VKGS_OK;
@@ -297,14 +309,15 @@ int vkgs_renderer_render(vkgs_renderer* r, const vkgs_camera* camera, const floa
   VKGS_HIP_CHECK(launch_set_frame_const(r->fc.as<FrameConst>(), fc, s));
   if (r->timing) {
     VKGS_HIP_CHECK(record_frame(r, d_rgba, s, true));
-    VKGS_HIP_CHECK(hipEventSynchronize(r->ev[5]));
-    float t[6] = {0};
-    hipEventElapsedTime(&t[0], r->ev[0], r->ev[1]);  // project
-    hipEventElapsedTime(&t[1], r->ev[1], r->ev[2]);  // depth sort
-    hipEventElapsedTime(&t[2], r->ev[2], r->ev[3]);  // bin
-    hipEventElapsedTime(&t[3], r->ev[3], r->ev[4]);  // tile sort + ranges
-    hipEventElapsedTime(&t[4], r->ev[4], r->ev[5]);  // raster
-    hipEventElapsedTime(&t[5], r->ev[0], r->ev[5]);  // total
+    VKGS_HIP_CHECK(hipEventSynchronize(r->ev[6]));
+    float t[kStages] = {0};
+    hipEventElapsedTime(&t[0], r->ev[0], r->ev[1]);  // visible count + scan
+    hipEventElapsedTime(&t[1], r->ev[1], r->ev[2]);  // project + compact (one kernel)
+    hipEventElapsedTime(&t[2], r->ev[2], r->ev[3]);  // depth sort
+    hipEventElapsedTime(&t[3], r->ev[3], r->ev[4]);  // bin: count, scan, emit
+    hipEventElapsedTime(&t[4], r->ev[4], r->ev[5]);  // tile sort + ranges
+    hipEventElapsedTime(&t[5], r->ev[5], r->ev[6]);  // raster (one kernel)
+    hipEventElapsedTime(&t[6], r->ev[0], r->ev[6]);  // total
     memcpy(r->stage_ms, t, sizeof(t));
     return VKGS_OK;
   }
@@ -345,7 +358,7 @@ int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const 
   const uint32_t cap = r->frame_n ? r->frame_n : 1;
   VKGS_HIP_CHECK(r->sorted_ids.ensure((size_t)cap * 4));
   uint32_t* misc = r->misc.as<uint32_t>();
-  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val.as<uint32_t>(), misc + kMiscVisible, cap,
+  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val.as<uint32_t>(), nullptr, misc + kMiscVisible, cap,
                                    r->sorted_ids.as<uint32_t>(), nullptr));
   VKGS_HIP_CHECK(hipDeviceSynchronize());
   *d_sorted_ids = r->sorted_ids.as<uint32_t>();
@@ -359,8 +372,9 @@ int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const 
   const uint64_t cap = r->max_pairs ? r->max_pairs : 1;
   VKGS_HIP_CHECK(r->list_ids.ensure((size_t)cap * 4));
   uint32_t* misc = r->misc.as<uint32_t>();
-  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->pvals.as<uint32_t>(), misc + kMiscPairs,
-                                   (uint32_t)cap, r->list_ids.as<uint32_t>(), nullptr));
+  // pairs hold draw-order positions k; k -> slot (depth-sorted val) -> splat id
+  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val.as<uint32_t>(), r->pvals.as<uint32_t>(),
+                                   misc + kMiscPairs, (uint32_t)cap, r->list_ids.as<uint32_t>(), nullptr));
   VKGS_HIP_CHECK(hipDeviceSynchronize());
   *d_ranges = r->ranges.as<uint32_t>();
   *d_list_ids = r->list_ids.as<uint32_t>();
@@ -376,7 +390,7 @@ int vkgs_renderer_set_timing(vkgs_renderer* r, int enable) {
   return VKGS_OK;
 }
 
-int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[6]) {
+int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[8]) {
   if (!r || !ms) return VKGS_ERR_INVALID_ARG;
   memcpy(ms, r->stage_ms, sizeof(r->stage_ms));
   return VKGS_OK;

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "common.h"
+#include "sort.h"
 
 namespace vkgs {
 
@@ -15,7 +16,11 @@ constexpr uint32_t kCompactBlock = 256;  // splats per workgroup in rank/fused c
 hipError_t launch_parse_ply(const void* d_ply, uint32_t n, float* pos, float* cov3d, float* opacity, uint16_t* sh,
                             hipStream_t s);
 
-// rank.comp with ordered compaction; counts holds ceil(n/256) words.
+// Bytes of the `counts` storage of launch_rank / launch_fused_count: one word
+// per 256 splats plus the partials of their scan.
+size_t compact_storage_bytes(uint32_t n);
+
+// rank.comp with ordered compaction; counts is compact_storage_bytes(n) bytes.
 hipError_t launch_rank(const FrameConst& fc, const float* pos, uint32_t n, uint32_t* d_count, uint32_t* key,
                        uint32_t* index, uint32_t* counts, hipStream_t s);
 
@@ -28,11 +33,14 @@ hipError_t launch_projection(const FrameConst& fc, const float* pos, const float
                              const uint16_t* sh, const int32_t* inverse, uint32_t n, const uint32_t* d_count,
                              float4* inst, uint32_t* draw_indirect, hipStream_t s);
 
-// Fused cull + project + compaction (frame constants read from device memory):
-// instance, key = bits(1 - z), val = slot, ids[slot] = splat id.
-hipError_t launch_fused_project(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
-                                const uint16_t* sh, uint32_t n, uint32_t* d_count, float4* inst, uint32_t* key,
-                                uint32_t* val, uint32_t* ids, uint32_t* counts, hipStream_t s);
+// Fused cull + project + compaction (frame constants read from device memory),
+// in two launches: visible counts per workgroup + their scan (-> *d_count),
+// then instance, key = bits(1 - z), val = slot, ids[slot] = splat id.
+hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t n, uint32_t* d_count,
+                              uint32_t* counts, hipStream_t s);
+hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
+                              const uint16_t* sh, uint32_t n, float4* inst, uint32_t* key, uint32_t* val,
+                              uint32_t* ids, const uint32_t* counts, hipStream_t s);
 
 // Copies a FrameConst passed by value into device memory (stream-ordered).
 hipError_t launch_set_frame_const(FrameConst* d_fc, const FrameConst& fc, hipStream_t s);
@@ -47,20 +55,45 @@ struct RasterShape {
 
 bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config* cfg, RasterShape* rs);
 
-// Per draw-order position k < *d_count: tile rectangle and tile count of
-// instance order[k] (order == nullptr: k).
+// Per draw-order position k < *d_count: raster record rec[3k..3k+2] of
+// instance order[k] (order == nullptr: k), its tile rectangle and tile count.
 hipError_t launch_bin_count(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
-                            const RasterShape& rs, TileRect* rect, uint32_t* cnt, hipStream_t s);
-hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* order,
-                           const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
-                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s);
+                            const RasterShape& rs, float4* rec, TileRect* rect, uint32_t* cnt, hipStream_t s);
+// (tile, k) pairs at offs[k]; pairs beyond max_pairs set *status.
+hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* d_count,
+                           uint32_t max_n, const RasterShape& rs, uint64_t max_pairs, uint32_t* keys, uint32_t* vals,
+                           uint32_t* status, hipStream_t s);
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s);
-hipError_t launch_rasterize(const float4* inst, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
+hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
                             float4* out, hipStream_t s);
 
-// Gathers out[k] = src[idx[k]] for k < *d_count.
-hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* d_count, uint32_t max_n,
-                             uint32_t* out, hipStream_t s);
+// Device buffers of one draw (bin, tile sort, ranges, raster).
+struct DrawBuffers {
+  float4* rec;       // 3 x float4 raster record per draw-order position
+  TileRect* rect;    // tile rectangle per position
+  uint32_t* cnt;     // tiles per position
+  uint32_t* offs;    // exclusive scan of cnt
+  ScanPlan scan;
+  void* scan_storage;
+  uint32_t* keys;    // tile id per pair
+  uint32_t* vals;    // draw-order position per pair
+  SortPlan sort;
+  void* sort_storage;
+  uint32_t* ranges;  // [start, end) per tile
+  uint32_t* status;  // overflow flag
+  uint32_t* pairs;   // pair count
+};
+
+// Bins instances order[k] (k < *d_count), sorts the pairs by tile, marks the
+// tile ranges and rasterizes; marks[0..1] (optional) are recorded after the
+// binning and after the tile sort.
+hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
+                    const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
+                    hipEvent_t* marks);
+
+// out[k] = src[idx[k]] (idx2 == nullptr) or src[idx[idx2[k]]] for k < *d_count.
+hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* idx2, const uint32_t* d_count,
+                             uint32_t max_n, uint32_t* out, hipStream_t s);
 
 }  // namespace vkgs

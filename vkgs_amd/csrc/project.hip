@@ -354,12 +354,25 @@ hipError_t launch_parse_ply(const void* d_ply, uint32_t n, float* pos, float* co
   return hipGetLastError();
 }
 
+size_t compact_storage_bytes(uint32_t n) {
+  const uint32_t nb = ceil_div(n ? n : 1, kCompactBlock);
+  return align_up((size_t)nb * 4, 256) + scan_plan(nb).bytes;
+}
+
+// Exclusive scan of the per-workgroup visible counts in place; the scan's
+// partials live right after the counts (see compact_storage_bytes).
+static hipError_t compact_scan(uint32_t* counts, uint32_t nb, uint32_t* d_count, hipStream_t s) {
+  const ScanPlan p = scan_plan(nb);
+  void* partials = reinterpret_cast<char*>(counts) + align_up((size_t)nb * 4, 256);
+  return scan_exclusive_launch(p, counts, counts, nullptr, d_count, partials, s);
+}
+
 hipError_t launch_rank(const FrameConst& fc, const float* pos, uint32_t n, uint32_t* d_count, uint32_t* key,
                        uint32_t* index, uint32_t* counts, hipStream_t s) {
   if (n == 0) return fill_u32_launch(d_count, 0u, 1, s);
   const uint32_t nb = ceil_div(n, kCompactBlock);
   visible_count_kernel<FrameConst><<<nb, kCompactBlock, 0, s>>>(fc, pos, n, counts);
-  hipError_t e = scan_small_launch(counts, nb, d_count, s);
+  hipError_t e = compact_scan(counts, nb, d_count, s);
   if (e != hipSuccess) return e;
   rank_write_kernel<<<nb, kCompactBlock, 0, s>>>(fc, pos, n, counts, key, index);
   return hipGetLastError();
@@ -382,14 +395,19 @@ hipError_t launch_projection(const FrameConst& fc, const float* pos, const float
   return hipGetLastError();
 }
 
-hipError_t launch_fused_project(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
-                                const uint16_t* sh, uint32_t n, uint32_t* d_count, float4* inst, uint32_t* key,
-                                uint32_t* val, uint32_t* ids, uint32_t* counts, hipStream_t s) {
+hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t n, uint32_t* d_count,
+                              uint32_t* counts, hipStream_t s) {
   if (n == 0) return fill_u32_launch(d_count, 0u, 1, s);
   const uint32_t nb = ceil_div(n, kCompactBlock);
   visible_count_kernel<const FrameConst*><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, n, counts);
-  hipError_t e = scan_small_launch(counts, nb, d_count, s);
-  if (e != hipSuccess) return e;
+  return compact_scan(counts, nb, d_count, s);
+}
+
+hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
+                              const uint16_t* sh, uint32_t n, float4* inst, uint32_t* key, uint32_t* val,
+                              uint32_t* ids, const uint32_t* counts, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t nb = ceil_div(n, kCompactBlock);
   fused_write_kernel<<<nb, kCompactBlock, 0, s>>>(d_fc, pos, cov3d, opacity, sh, n, counts, inst, key, val, ids);
   return hipGetLastError();
 }

@@ -1,19 +1,28 @@
 // Tile binning and tile rasterizer for gfx950.
 //
 // Replaces the hardware draw of the reference (splat.vert:10-26,
-// splat.frag:8-12, "over" blend engine.cc:279-299).  Instances are binned into
-// screen tiles by their conservative pixel rectangle (bin_count, exclusive
-// scan, bin_emit), the (tile, instance) pairs are stably sorted by tile so each
-// tile's list keeps draw order, and one workgroup per tile composites its list.
+// splat.frag:8-12, "over" blend engine.cc:279-299).
+//
+// bin_count  : one lane per draw-order position k: raster set-up of instance
+//              order[k] (centre, inverse pixel-space rot_scale, colour, pixel
+//              rectangle) into a 48-byte raster record rec[k], plus its tile
+//              rectangle and tile count;
+// scan       : exclusive scan of the tile counts;
+// bin_emit   : (tile, k) pairs, tiles of one splat in row-major order;
+// tile sort  : stable radix sort by tile, so each tile's list keeps draw order;
+// ranges     : [start, end) of every tile;
+// rasterize  : one workgroup per tile, one lane per pixel.  Records are staged
+//              in LDS in batches of one per lane; the next batch is gathered
+//              into registers while the current one is composited, so global
+//              latency overlaps compositing and there is one barrier per batch.
+//              A wave skips a record whose rectangle misses its pixel rows with
+//              a scalar branch.
 //
 // Compositing runs front to back (nearest first) with transmittance T:
 //   C += c * alpha * T;  A += alpha * alpha * T;  T *= 1 - alpha;  A_out = A + T
 // which is the closed form of the reference's back-to-front blend with clear
 // colour (0,0,0,1).  A pixel stops once T < t_min (t_min = 0 composites every
 // splat); the skipped splats contribute at most t_min * max(colour, 1).
-// Each batch of up to tile_w*tile_h instances is set up once (one lane per
-// instance) into LDS and then read by every lane as a broadcast; a wave skips
-// an instance whose rectangle misses its rows with a scalar branch.
 #pragma clang fp contract(off)
 
 #include "common.h"
@@ -22,11 +31,13 @@
 namespace vkgs {
 namespace {
 
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+
 __global__ __launch_bounds__(256) void bin_count_kernel(const float4* __restrict__ inst,
                                                         const uint32_t* __restrict__ order,
                                                         const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                        RasterShape rs, TileRect* __restrict__ rect,
-                                                        uint32_t* __restrict__ cnt) {
+                                                        RasterShape rs, float4* __restrict__ rec,
+                                                        TileRect* __restrict__ rect, uint32_t* __restrict__ cnt) {
   const uint32_t n = min(*d_count, max_n);
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const uint32_t idx = order ? order[k] : k;
@@ -41,7 +52,15 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const float4* __restrict
       r.x = tx0 | (tx1 << 16);
       r.y = ty0 | (ty1 << 16);
       count = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    } else {  // never binned; an empty rectangle keeps the record inert
+      s.x0 = 1;
+      s.x1 = 0;
+      s.y0 = 1;
+      s.y1 = 0;
     }
+    rec[3 * (size_t)k + 0] = make_float4(s.cx, s.cy, s.i00, s.i01);
+    rec[3 * (size_t)k + 1] = make_float4(s.i10, s.i11, s.o, __uint_as_float(pack16(s.x0, s.x1)));
+    rec[3 * (size_t)k + 2] = make_float4(s.r, s.g, s.b, __uint_as_float(pack16(s.y0, s.y1)));
     rect[k] = r;
     cnt[k] = count;
   }
@@ -50,7 +69,6 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const float4* __restrict
 __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restrict__ rect,
                                                        const uint32_t* __restrict__ cnt,
                                                        const uint32_t* __restrict__ offs,
-                                                       const uint32_t* __restrict__ order,
                                                        const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                        uint32_t tiles_x, uint64_t max_pairs,
                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
@@ -61,7 +79,6 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
     if (c == 0) continue;
     const uint64_t o = offs[k];
     const TileRect r = rect[k];
-    const uint32_t idx = order ? order[k] : k;
     if (o + c > max_pairs) atomicOr(status, 1u);
     const uint32_t tx0 = r.x & 0xffffu, tx1 = r.x >> 16, ty0 = r.y & 0xffffu, ty1 = r.y >> 16;
     uint64_t j = o;
@@ -69,7 +86,7 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
       for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j) {
         if (j < max_pairs) {
           keys[j] = ty * tiles_x + tx;
-          vals[j] = idx;
+          vals[j] = k;
         }
       }
   }
@@ -91,89 +108,102 @@ __global__ void ranges_mark_kernel(const uint32_t* __restrict__ keys, const uint
 }
 
 template <int TW, int TH>
-__global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restrict__ inst,
+__global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restrict__ rec,
                                                            const uint32_t* __restrict__ vals,
                                                            const uint32_t* __restrict__ ranges, RasterShape rs,
                                                            float4* __restrict__ out) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
-  __shared__ float4 s_a[kN];  // cx, cy, i00, i01
-  __shared__ float4 s_b[kN];  // i10, i11, opacity, -
-  __shared__ float4 s_c[kN];  // r, g, b, -
-  __shared__ int4 s_box[kN];  // x0, x1, y0, y1 (pixels)
+  __shared__ float4 s_rec[2][3][kN];
 
   const uint32_t tile = blockIdx.x;
   const uint32_t tx = tile % rs.tiles_x, ty = tile / rs.tiles_x;
-  const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
+  const int t = threadIdx.x;
+  const int lx = t % TW, ly = t / TW;
   const int px = (int)tx * TW + lx, py = (int)ty * TH + ly;
   const bool inside = px < (int)rs.width && py < (int)rs.height;
-  const int col0 = (int)tx * TW, col1 = col0 + TW - 1;
-  const int row0 = (int)ty * TH + (int)(threadIdx.x / kWave) * kRowsPerWave, row1 = row0 + kRowsPerWave - 1;
+  // pixel bounds of this wave (wave-uniform, held in scalar registers)
+  const int col0 = __builtin_amdgcn_readfirstlane((int)tx * TW), col1 = col0 + TW - 1;
+  const int row0 = __builtin_amdgcn_readfirstlane((int)ty * TH + (t / kWave) * kRowsPerWave);
+  const int row1 = row0 + kRowsPerWave - 1;
   const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
+  const uint32_t items = end - start;
+  const uint32_t batches = (items + kN - 1) / kN;
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   const float t_min = rs.t_min;
 
+  // batch b holds list positions end-1-b*kN-t (nearest first)
+  float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
+  if ((uint32_t)t < items) {
+    const uint32_t k = vals[end - 1 - t];
+    p0 = rec[3 * (size_t)k];
+    p1 = rec[3 * (size_t)k + 1];
+    p2 = rec[3 * (size_t)k + 2];
+  }
+
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
   bool done = !inside;
-  for (uint32_t hi = end; hi > start;) {
-    const uint32_t lo = hi - start > (uint32_t)kN ? hi - kN : start;
-    const int cnt = (int)(hi - lo);
-    __syncthreads();
-    if ((int)threadIdx.x < cnt) {
-      const uint32_t idx = vals[hi - 1 - threadIdx.x];  // nearest first
-      const float4 a = inst[3 * (size_t)idx], b = inst[3 * (size_t)idx + 1], c = inst[3 * (size_t)idx + 2];
-      const float f[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
-      Splat2D s;
-      if (!setup_splat(f, rs.width, rs.height, &s)) {  // never binned; keep the slot inert
-        s.x0 = 1;
-        s.x1 = 0;
-        s.y0 = 1;
-        s.y1 = 0;
-      }
-      s_a[threadIdx.x] = make_float4(s.cx, s.cy, s.i00, s.i01);
-      s_b[threadIdx.x] = make_float4(s.i10, s.i11, s.o, 0.f);
-      s_c[threadIdx.x] = make_float4(s.r, s.g, s.b, 0.f);
-      s_box[threadIdx.x] = make_int4(s.x0, s.x1, s.y0, s.y1);
+  for (uint32_t b = 0; b < batches; ++b) {
+    const int buf = b & 1;
+    s_rec[buf][0][t] = p0;
+    s_rec[buf][1][t] = p1;
+    s_rec[buf][2][t] = p2;
+    if (__syncthreads_count(!done) == 0) break;
+    // gather the next batch while this one is composited
+    const uint32_t next = (b + 1) * kN + t;
+    if (next < items) {
+      const uint32_t k = vals[end - 1 - next];
+      p0 = rec[3 * (size_t)k];
+      p1 = rec[3 * (size_t)k + 1];
+      p2 = rec[3 * (size_t)k + 2];
     }
-    __syncthreads();
-    if (!done) {
+    const int cnt = (int)min((uint32_t)kN, items - b * kN);
+    if (__ballot(!done) != 0) {
+      const float4* s0 = s_rec[buf][0];
+      const float4* s1 = s_rec[buf][1];
+      const float4* s2 = s_rec[buf][2];
+      float4 a = s0[0], bb = s1[0], c = s2[0];
       for (int e = 0; e < cnt; ++e) {
-        const int4 box = s_box[e];
-        const int bx0 = __builtin_amdgcn_readfirstlane(box.x), bx1 = __builtin_amdgcn_readfirstlane(box.y);
-        const int by0 = __builtin_amdgcn_readfirstlane(box.z), by1 = __builtin_amdgcn_readfirstlane(box.w);
-        if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
-        const float4 a = s_a[e];
-        const float4 b = s_b[e];
-        const float dx = fx - a.x, dy = fy - a.y;
-        const float ux = fmaf(a.z, dx, a.w * dy);
-        const float uy = fmaf(b.x, dx, b.y * dy);
-        if (fabsf(ux) <= 3.f && fabsf(uy) <= 3.f) {
+        // LDS reads of the next record overlap this record's arithmetic
+        const int en = e + 1 < cnt ? e + 1 : e;
+        const float4 na = s0[en], nb = s1[en], nc = s2[en];
+        const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
+        const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
+        const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
+        const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
+        if (!(by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1)) {
+          const float dx = fx - a.x, dy = fy - a.y;
+          const float ux = fmaf(a.z, dx, a.w * dy);
+          const float uy = fmaf(bb.x, dx, bb.y * dy);
+          const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && !done;
           const float q = fmaf(ux, ux, uy * uy);
-          const float alpha = b.z * __expf(-0.5f * q);
-          const float4 c = s_c[e];
+          const float alpha = in ? bb.z * __expf(-0.5f * q) : 0.f;
           const float ta = T * alpha;
           cr = fmaf(c.x, ta, cr);
           cg = fmaf(c.y, ta, cg);
           cb = fmaf(c.z, ta, cb);
           aa = fmaf(alpha, ta, aa);
           T = T - ta;
-          if (T < t_min) {
-            done = true;
-            break;
-          }
+          done = done || T < t_min;
+          if (__ballot(!done) == 0) break;
         }
+        a = na;
+        bb = nb;
+        c = nc;
       }
     }
-    hi = lo;
-    if (__syncthreads_count(!done) == 0) break;
   }
   if (inside) out[(size_t)py * rs.width + px] = make_float4(cr, cg, cb, aa + T);
 }
 
 __global__ void gather_u32_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx,
-                                  const uint32_t* __restrict__ d_count, uint32_t max_n, uint32_t* __restrict__ out) {
+                                  const uint32_t* __restrict__ idx2, const uint32_t* __restrict__ d_count,
+                                  uint32_t max_n, uint32_t* __restrict__ out) {
   const uint32_t n = min(*d_count, max_n);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = src[idx[k]];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t i = idx2 ? idx[idx2[k]] : idx[k];
+    out[k] = src[i];
+  }
 }
 
 uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
@@ -215,18 +245,18 @@ bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config
 }
 
 hipError_t launch_bin_count(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
-                            const RasterShape& rs, TileRect* rect, uint32_t* cnt, hipStream_t s) {
+                            const RasterShape& rs, float4* rec, TileRect* rect, uint32_t* cnt, hipStream_t s) {
   if (max_n == 0) return hipSuccess;
-  bin_count_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(inst, order, d_count, max_n, rs, rect, cnt);
+  bin_count_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(inst, order, d_count, max_n, rs, rec, rect, cnt);
   return hipGetLastError();
 }
 
-hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* order,
-                           const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
-                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s) {
+hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* d_count,
+                           uint32_t max_n, const RasterShape& rs, uint64_t max_pairs, uint32_t* keys, uint32_t* vals,
+                           uint32_t* status, hipStream_t s) {
   if (max_n == 0) return hipSuccess;
-  bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rect, cnt, offs, order, d_count, max_n, rs.tiles_x,
-                                                              max_pairs, keys, vals, status);
+  bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rect, cnt, offs, d_count, max_n, rs.tiles_x, max_pairs,
+                                                              keys, vals, status);
   return hipGetLastError();
 }
 
@@ -238,14 +268,14 @@ hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t
   return hipGetLastError();
 }
 
-hipError_t launch_rasterize(const float4* inst, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
+hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
                             float4* out, hipStream_t s) {
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tw = rs.tile_w, th = rs.tile_h;
-#define VKGS_RASTER_CASE(W_, H_)                                                          \
-  if (tw == W_ && th == H_) {                                                             \
-    rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(inst, vals, ranges, rs, out);      \
-    return hipGetLastError();                                                             \
+#define VKGS_RASTER_CASE(W_, H_)                                                     \
+  if (tw == W_ && th == H_) {                                                        \
+    rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out);  \
+    return hipGetLastError();                                                        \
   }
   VKGS_RASTER_CASE(16, 16)
   VKGS_RASTER_CASE(8, 8)
@@ -259,10 +289,10 @@ hipError_t launch_rasterize(const float4* inst, const uint32_t* vals, const uint
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* d_count, uint32_t max_n,
-                             uint32_t* out, hipStream_t s) {
+hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* idx2, const uint32_t* d_count,
+                             uint32_t max_n, uint32_t* out, hipStream_t s) {
   if (max_n == 0) return hipSuccess;
-  gather_u32_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(src, idx, d_count, max_n, out);
+  gather_u32_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(src, idx, idx2, d_count, max_n, out);
   return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ namespace vkgs {
 struct SortPlan {
   uint32_t max_n = 0;
   uint32_t blocks = 1;  // fixed grid; each block sorts a contiguous chunk
-  size_t off_alt_key = 0, off_alt_val = 0, off_hist = 0;
+  size_t off_alt_key = 0, off_alt_val = 0, off_hist = 0, off_digit_total = 0;
   size_t bytes = 0;
 };
 
@@ -32,8 +32,8 @@ struct ScanPlan {
 
 ScanPlan scan_plan(uint32_t max_n);
 
-// out[i] = sum(in[0..i)) for i < min(*d_count, max_n); *d_total = sum of all.
-// in and out may alias.
+// out[i] = sum(in[0..i)) for i < min(*d_count, max_n) (d_count == nullptr:
+// i < max_n); *d_total = sum of all.  in and out may alias.
 hipError_t scan_exclusive_launch(const ScanPlan& plan, const uint32_t* in, uint32_t* out, const uint32_t* d_count,
                                  uint32_t* d_total, void* storage, hipStream_t stream);
 

@@ -120,12 +120,40 @@ __global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restr
   }
 }
 
+// Exclusive scan of one digit's row hist[d][0..blocks) in place (one workgroup
+// per digit); digit_total[d] receives the row sum.
+__global__ __launch_bounds__(kThreads) void sort_scan_rows(uint32_t* __restrict__ hist, uint32_t blocks,
+                                                           uint32_t* __restrict__ digit_total) {
+  __shared__ uint32_t s_tmp[kWaves];
+  uint32_t* row = hist + (size_t)blockIdx.x * blocks;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < blocks; base += kThreads * 4) {
+    const uint32_t first = base + threadIdx.x * 4;
+    uint32_t x[4], s = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      x[j] = first + j < blocks ? row[first + j] : 0u;
+      s += x[j];
+    }
+    uint32_t all;
+    uint32_t run = carry + block_exclusive_scan<kThreads>(s, s_tmp, &all);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (first + j < blocks) row[first + j] = run;
+      run += x[j];
+    }
+    carry += all;
+  }
+  if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
+}
+
 __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                            uint32_t shift, uint32_t bits,
-                                                           const uint32_t* __restrict__ hist) {
+                                                           const uint32_t* __restrict__ hist,
+                                                           const uint32_t* __restrict__ digit_total) {
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_wcnt[kWaves][256];
   __shared__ uint32_t s_start[256];
@@ -138,7 +166,13 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
   uint32_t begin, end;
   block_range(n, begin, end);
   const uint32_t mask = (1u << bits) - 1u;
-  s_base[tid] = tid <= mask ? hist[tid * gridDim.x + blockIdx.x] : 0u;
+  {
+    // global offset of digit tid for this block: keys of smaller digits
+    // everywhere plus keys of this digit in earlier blocks
+    uint32_t all;
+    const uint32_t smaller = block_exclusive_scan<kThreads>(tid <= mask ? digit_total[tid] : 0u, s_tmp, &all);
+    s_base[tid] = tid <= mask ? smaller + hist[tid * gridDim.x + blockIdx.x] : 0u;
+  }
 #pragma unroll
   for (int i = 0; i < kWaves; ++i) s_wcnt[i][tid] = 0u;
   __syncthreads();
@@ -237,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void scan_reduce(const uint32_t* __restri
                                                         const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                         uint32_t* __restrict__ partial) {
   __shared__ uint32_t s_tmp[kWaves];
-  const uint32_t n = min(*d_count, max_n);
+  const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
   uint32_t s = 0;
@@ -251,7 +285,7 @@ __global__ __launch_bounds__(kThreads) void scan_apply(const uint32_t* in, uint3
                                                        const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                        const uint32_t* __restrict__ partial) {
   __shared__ uint32_t s_tmp[kWaves];
-  const uint32_t n = min(*d_count, max_n);
+  const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
   uint32_t carry = partial[blockIdx.x];
@@ -307,6 +341,8 @@ SortPlan sort_plan(uint32_t max_n) {
   off = align_up(off + (size_t)max_n * 4, 256);
   p.off_hist = off;
   off = align_up(off + (size_t)256 * p.blocks * 4, 256);
+  p.off_digit_total = off;
+  off = align_up(off + 256 * 4, 256);
   p.bytes = off;
   return p;
 }
@@ -318,6 +354,7 @@ hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_
   uint32_t* alt_k = reinterpret_cast<uint32_t*>(base + plan.off_alt_key);
   uint32_t* alt_v = reinterpret_cast<uint32_t*>(base + plan.off_alt_val);
   uint32_t* hist = reinterpret_cast<uint32_t*>(base + plan.off_hist);
+  uint32_t* digit_total = reinterpret_cast<uint32_t*>(base + plan.off_digit_total);
   uint32_t* src_k = keys;
   uint32_t* src_v = vals;
   uint32_t* dst_k = alt_k;
@@ -326,9 +363,9 @@ hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_
   for (uint32_t shift = 0; shift < key_bits; shift += 8) {
     const uint32_t bits = key_bits - shift < 8 ? key_bits - shift : 8;
     sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
-    scan_single_block<<<1, kScanThreads, 0, stream>>>(hist, plan.blocks << bits, nullptr);
+    sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total);
     sort_downsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, plan.max_n, shift,
-                                                         bits, hist);
+                                                         bits, hist, digit_total);
     uint32_t* tk = src_k;
     uint32_t* tv = src_v;
     src_k = dst_k;

@@ -16,6 +16,7 @@ from . import _lib
 from ._lib import check, lib, ptr
 
 IDENTITY = np.eye(4, dtype=np.float32).ravel()
+STAGES = ["count", "project", "depth_sort", "bin", "tile_sort", "raster", "total"]
 
 
 def _stream() -> int:
@@ -252,9 +253,9 @@ class Renderer:
         check(lib().vkgs_renderer_set_timing(self._h, 1 if on else 0), "vkgs_renderer_set_timing")
 
     def stage_ms(self):
-        a = (C.c_float * 6)()
+        a = (C.c_float * 8)()
         check(lib().vkgs_renderer_stage_ms(self._h, a), "vkgs_renderer_stage_ms")
-        return dict(zip(["project", "depth_sort", "bin", "tile_sort", "raster", "total"], list(a)))
+        return dict(zip(STAGES, list(a)))
 
 
 def _copy_device(dst: int, src: int, nbytes: int) -> None:
