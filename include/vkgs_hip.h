@@ -162,7 +162,7 @@ typedef struct vkgs_raster_config {
   uint32_t reserved[5];
 } vkgs_raster_config;
 
-/* default: 16x16 tiles, t_min = 1/8192 */
+/* default: 8x8 tiles, t_min = 1/8192 */
 void vkgs_raster_config_default(vkgs_raster_config* cfg);
 
 size_t vkgs_draw_storage_bytes(uint32_t max_instances, uint32_t width, uint32_t height, uint64_t max_pairs,
@@ -216,12 +216,67 @@ int vkgs_renderer_frame_info(vkgs_renderer* r, uint32_t* visible, uint64_t* pair
 int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const uint32_t** d_keys);
 int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const uint32_t** d_list_ids,
                              uint32_t* tiles_x, uint32_t* tiles_y);
+/* Autotuner: renders `frames` graph replays of this camera for each tile shape
+ * (8x8, 16x8, 8x16, 16x4, 16x16, 32x4), times them with HIP events and keeps
+ * the fastest (t_min unchanged).  Synchronous.  The chosen config and its
+ * ms/frame are returned when the pointers are non-NULL. */
+int vkgs_renderer_autotune(vkgs_renderer* r, const vkgs_camera* camera, const float model[16], float* d_rgba,
+                           uint32_t frames, vkgs_raster_config* chosen, float* best_ms, void* stream);
 /* Enable per-stage hipEvent timing (the reference's 12-timestamp breakdown,
  * engine.cc:1031-1045); disables graph replay while on. */
 int vkgs_renderer_set_timing(vkgs_renderer* r, int enable);
 /* ms of the last timed frame: [visible count + scan, project + compact,
  * depth sort, bin, tile sort + ranges, raster, total, 0] */
 int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[8]);
+
+/* ---------------------------------------------------------------------------
+ * Differentiable, batched path (no reference counterpart; the reference only
+ * renders).  Parameters are the raw 3DGS ones in the PLY's conventions:
+ * means (N,3), log-scales (N,3) (scale_0..2), quaternions (N,4) in PLY order
+ * (rot_0..rot_3 = w,x,y,z), opacity logits (N), SH (N,48) in the vkgs
+ * channel-major order; activation as parse_ply.comp:106-156 (SH kept in fp32),
+ * projection as projection.comp:80-179, blend as engine.cc:279-299 with alpha
+ * clamped to 0.999.  Many views of many scenes render in one set of launches:
+ * tiles of all views share one id space, so the tile sort key carries the view
+ * in its high bits.  A context keeps one forward's state until its backward,
+ * which re-walks each tile list back to front from the stored per-pixel
+ * transmittance and processed-entry count.
+ * ------------------------------------------------------------------------- */
+typedef struct vkgs_diff_scene {
+  const float* means;
+  const float* log_scales;
+  const float* quats;
+  const float* opacity_logits;
+  const float* sh;
+  float* grad_means; /* gradients are accumulated (+=) by vkgs_diff_backward */
+  float* grad_log_scales;
+  float* grad_quats;
+  float* grad_opacity_logits;
+  float* grad_sh;
+  uint32_t n;
+  uint32_t reserved[3];
+} vkgs_diff_scene;
+
+typedef struct vkgs_diff_view {
+  vkgs_camera camera;   /* screen_size gives the image size */
+  float model[16];      /* column-major model matrix */
+  uint32_t scene;       /* index into the scene table */
+  uint32_t reserved[3];
+  float* rgba;          /* height x width x 4 output (device) */
+  const float* grad_rgba; /* dL/d rgba, same shape (backward only) */
+} vkgs_diff_view;
+
+typedef struct vkgs_diff_ctx vkgs_diff_ctx;
+
+int vkgs_diff_create(vkgs_diff_ctx** out);
+int vkgs_diff_destroy(vkgs_diff_ctx* ctx);
+int vkgs_diff_forward(vkgs_diff_ctx* ctx, const vkgs_diff_scene* scenes, uint32_t n_scenes,
+                      const vkgs_diff_view* views, uint32_t n_views, const vkgs_raster_config* cfg, void* stream);
+int vkgs_diff_backward(vkgs_diff_ctx* ctx, const vkgs_diff_scene* scenes, uint32_t n_scenes,
+                       const vkgs_diff_view* views, uint32_t n_views, void* stream);
+/* Synchronous: visible items and pairs of the last forward; VKGS_ERR_OVERFLOW
+ * if its pair list overflowed (the next forward doubles the pair storage). */
+int vkgs_diff_info(vkgs_diff_ctx* ctx, uint32_t* visible, uint64_t* pairs, void* stream);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,9 @@ from vkgs_amd import pipeline as P  # noqa: E402
 from vkgs_amd import scene as S  # noqa: E402
 
 
+DEFAULT_TILE = (8, 8)  # vkgs_raster_config_default
+
+
 def psnr(a, b):
     mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
     return float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)
@@ -113,7 +116,7 @@ def test_reference_stages_bit_exact(c1):
     np.testing.assert_array_equal(u32(out["draw_indirect"]), ref["draw_indirect"])
     # tile lists of the draw
     ranges, lst = out["draw_storage"].lists()
-    oranges, olst = oracle.bin_tiles(ref["instances"], 1600, 900, 16, 16)
+    oranges, olst = oracle.bin_tiles(ref["instances"], 1600, 900, *DEFAULT_TILE)
     np.testing.assert_array_equal(ranges, oranges.astype(np.int64))
     np.testing.assert_array_equal(lst, olst)
     # image
@@ -174,7 +177,7 @@ def test_renderer_fused_matches_reference(c1):
     np.testing.assert_array_equal(keys, ref["keys"])
     np.testing.assert_array_equal(ids, ref["order"])
     ranges, lst_ids = r.tile_lists()
-    oranges, olst = oracle.bin_tiles(ref["instances"], 1600, 900, 16, 16)
+    oranges, olst = oracle.bin_tiles(ref["instances"], 1600, 900, *DEFAULT_TILE)
     np.testing.assert_array_equal(ranges, oranges.astype(np.int64))
     np.testing.assert_array_equal(lst_ids, ref["order"][olst])
     # the stage path and the fused path composite identical instances in identical order
@@ -198,4 +201,16 @@ def test_renderer_empty_and_offscreen():
     expect = np.zeros((240, 320, 4), np.float32)
     expect[..., 3] = 1.0  # clear colour (engine.cc:1382-1387)
     np.testing.assert_array_equal(img, expect)
+    r.close()
+
+
+def test_autotune_keeps_the_image(c1):
+    sc, cam, st, host = c1
+    r = P.Renderer(sc.n)
+    r.load_buffer(sc.ply_buffer(), sc.n)
+    before = r.render(cam).cpu().numpy()
+    tw, th, ms = r.autotune(cam, frames=3)
+    assert (tw, th) in {(8, 8), (16, 8), (8, 16), (16, 4), (16, 16), (32, 4)} and ms > 0
+    after = r.render(cam).cpu().numpy()
+    np.testing.assert_array_equal(before, after)  # compositing per pixel does not depend on the tile shape
     r.close()

@@ -106,6 +106,12 @@ SIGNATURES = {
     "vkgs_renderer_tile_lists": (_I, [_P, _P, _P, _P, _P]),
     "vkgs_renderer_set_timing": (_I, [_P, _I]),
     "vkgs_renderer_stage_ms": (_I, [_P, _P]),
+    "vkgs_renderer_autotune": (_I, [_P, _P, _P, _P, _U32, _P, _P, _P]),
+    "vkgs_diff_create": (_I, [_P]),
+    "vkgs_diff_destroy": (_I, [_P]),
+    "vkgs_diff_forward": (_I, [_P, _P, _U32, _P, _U32, _P, _P]),
+    "vkgs_diff_backward": (_I, [_P, _P, _U32, _P, _U32, _P]),
+    "vkgs_diff_info": (_I, [_P, _P, _P, _P]),
 }
 
 _lib = None

@@ -1,0 +1,835 @@
+// Differentiable, batched splat path for gfx950 (forward and backward).
+//
+// Forward: per (view, splat) item the activation of parse_ply.comp:106-156
+// (fp32 SH, no fp16 rounding) and the projection of projection.comp:80-179,
+// ordered compaction, depth sort, binning into global tiles, tile sort, and a
+// raster that also stores, per pixel, the transmittance after the last
+// processed splat and how many list entries it processed.
+//
+// Backward: the raster backward re-walks each tile list back to front from
+// that stored state, recomputing alpha per entry and the transmittance before
+// it as T_i = T_{i+1} / (1 - alpha_i); per entry it accumulates 9 gradient
+// words (colour 3, opacity 1, conic 3, centre 2) in LDS with ds_add_f32 and
+// flushes them with one global float atomic per word per tile.  The
+// projection backward then chains them to the raw parameters of each splat.
+//
+// alpha is clamped to kAlphaMax so the division above stays bounded; this is
+// the only departure of this path from the reference's blend.
+#pragma clang fp contract(off)
+
+#include "common.h"
+#include "diff.h"
+#include "kernels.h"
+
+namespace vkgs {
+namespace {
+
+constexpr float kAlphaMax = 0.999f;
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+
+// ------------------------------------------------------ forward projection ---
+
+// Rotation (row-major) of the normalized quaternion q = (x, y, z, w), as built
+// by parse_ply.comp:114-132 (the shader stores it column-major).
+__device__ __forceinline__ void quat_rot(const float q[4], float R[9]) {
+  const float x = q[0], y = q[1], z = q[2], w = q[3];
+  const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y,
+              wz = w * z;
+  R[0] = 1.f - 2.f * (yy + zz);
+  R[1] = 2.f * (xy - wz);
+  R[2] = 2.f * (xz + wy);
+  R[3] = 2.f * (xy + wz);
+  R[4] = 1.f - 2.f * (xx + zz);
+  R[5] = 2.f * (yz - wx);
+  R[6] = 2.f * (xz - wy);
+  R[7] = 2.f * (yz + wx);
+  R[8] = 1.f - 2.f * (xx + yy);
+}
+
+// Normalized (x, y, z, w) from the PLY order (w, x, y, z); returns |q|.
+__device__ __forceinline__ float load_quat(const DiffScene& sc, uint32_t id, float q[4]) {
+  const float w = sc.quats[4 * (size_t)id + 0], x = sc.quats[4 * (size_t)id + 1];
+  const float y = sc.quats[4 * (size_t)id + 2], z = sc.quats[4 * (size_t)id + 3];
+  const float len = sqrtf(((x * x + y * y) + z * z) + w * w);
+  q[0] = x / len;
+  q[1] = y / len;
+  q[2] = z / len;
+  q[3] = w / len;
+  return len;
+}
+
+// cov3d in the shader's column-major layout (parse_ply.comp:106-138).
+__device__ __forceinline__ void activate_cov(const DiffScene& sc, uint32_t id, float S[9]) {
+  float q[4];
+  load_quat(sc, id, q);
+  const float s0 = expf(sc.log_scales[3 * (size_t)id]), s1 = expf(sc.log_scales[3 * (size_t)id + 1]);
+  const float s2 = expf(sc.log_scales[3 * (size_t)id + 2]);
+  float Rm[9];
+  quat_rot(q, Rm);
+  float rot[9];  // column-major
+  m3_transpose(Rm, rot);
+  const float ss[9] = {s0 * s0, 0.f, 0.f, 0.f, s1 * s1, 0.f, 0.f, 0.f, s2 * s2};
+  float rs[9], rt[9];
+  m3_mul(rot, ss, rs);
+  m3_transpose(rot, rt);
+  m3_mul(rs, rt, S);
+}
+
+__device__ __forceinline__ void sh_basis(float x, float y, float z, float b[16]) {
+  const float C0 = 0.28209479177387814f, C1 = 0.4886025119029199f, C20 = 1.0925484305920792f,
+              C21 = 0.31539156525252005f, C22 = 0.5462742152960396f, C30 = 0.5900435899266435f,
+              C31 = 2.890611442640554f, C32 = 0.4570457994644658f, C33 = 0.3731763325901154f,
+              C34 = 1.445305721320277f;
+  const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+  b[0] = C0;
+  b[1] = -C1 * y;
+  b[2] = C1 * z;
+  b[3] = -C1 * x;
+  b[4] = C20 * xy;
+  b[5] = -C20 * yz;
+  b[6] = C21 * ((2.f * zz - xx) - yy);
+  b[7] = -C20 * xz;
+  b[8] = C22 * (xx - yy);
+  b[9] = (-C30 * y) * (3.f * xx - yy);
+  b[10] = (C31 * xy) * z;
+  b[11] = (-C32 * y) * ((4.f * zz - xx) - yy);
+  b[12] = (C33 * z) * ((2.f * zz - 3.f * xx) - 3.f * yy);
+  b[13] = (-C32 * x) * ((4.f * zz - xx) - yy);
+  b[14] = (C34 * z) * (xx - yy);
+  b[15] = (-C30 * x) * (xx - 3.f * yy);
+}
+
+// projection.comp:80-179 on raw parameters; fills the 48-byte instance.
+__device__ void diff_project(const FrameConst& fc, const DiffScene& sc, uint32_t id, float4 out[3]) {
+  float S[9];
+  activate_cov(sc, id, S);
+  float p4[4] = {sc.means[3 * (size_t)id], sc.means[3 * (size_t)id + 1], sc.means[3 * (size_t)id + 2], 1.f};
+  const float d0 = p4[0] - fc.eye_model[0], d1 = p4[1] - fc.eye_model[1], d2 = p4[2] - fc.eye_model[2];
+  const float dl = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+  const float dx = d0 / dl, dy = d1 / dl, dz = d2 / dl;
+  float m3[9], t4[4];
+  m3_from_m4(fc.M, m3);
+  m3_sandwich(m3, S);
+  m4_mulv(fc.M, p4, t4);
+  m3_from_m4(fc.V, m3);
+  m3_sandwich(m3, S);
+  m4_mulv(fc.V, t4, p4);
+  const float x = p4[0], y = p4[1], z = p4[2];
+  const float r = sqrtf((x * x + y * y) + z * z);
+  float J[9] = {-1.f / z, 0.f, (-2.f * x) / r, 0.f, -1.f / z, (-2.f * y) / r, (x / z) / z, (y / z) / z,
+                (-2.f * z) / r};
+  m3_sandwich(J, S);
+  const float Ps[4] = {fc.P[0], fc.P[1], fc.P[4], fc.P[5]};
+  const float S2[4] = {S[0], S[1], S[3], S[4]};
+  float cov2[4];
+  m2_mul(Ps, S2, cov2);
+  m2_mul(cov2, Ps, cov2);
+  cov2[0] = cov2[0] + fc.inv_w2;
+  cov2[3] = cov2[3] + fc.inv_h2;
+  const float a = cov2[0], b = cov2[3], c = cov2[2];
+  const float D = sqrtf((a - b) * (a - b) + (4.f * c) * c);
+  const float s0 = sqrtf(0.5f * ((a + b) + D));
+  const float s1 = sqrtf(0.5f * ((a + b) - D));
+  const float sin2t = (2.f * c) / D;
+  const float cos2t = (a - b) / D;
+  float ct = (1.f + cos2t) * 0.5f;
+  float st = (1.f - cos2t) * 0.5f;
+  ct = ct < 0.f ? 0.f : ct;
+  st = st < 0.f ? 0.f : st;
+  float cos_t = sqrtf(ct);
+  float sin_t = copysignf(sqrtf(st), sin2t);
+  if (sin2t != sin2t || cos2t != cos2t) {
+    cos_t = __builtin_nanf("");
+    sin_t = __builtin_nanf("");
+  }
+  m4_mulv(fc.P, p4, t4);
+  const float nx = t4[0] / t4[3], ny = t4[1] / t4[3], nz = t4[2] / t4[3];
+  float basis[16];
+  sh_basis(dx, dy, dz, basis);
+  const float* shp = sc.sh + 48 * (size_t)id;
+  float color[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    float acc = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float dot = basis[4 * j + 0] * shp[ch * 16 + 4 * j + 0];
+      dot = dot + basis[4 * j + 1] * shp[ch * 16 + 4 * j + 1];
+      dot = dot + basis[4 * j + 2] * shp[ch * 16 + 4 * j + 2];
+      dot = dot + basis[4 * j + 3] * shp[ch * 16 + 4 * j + 3];
+      acc = (j == 0) ? dot : acc + dot;
+    }
+    color[ch] = glsl_max(acc + 0.5f, 0.f);
+  }
+  const float opacity = 1.f / (1.f + expf(-sc.opacity[id]));
+  out[0] = make_float4(nx, ny, nz, 0.f);
+  out[1] = make_float4(s0 * cos_t, s0 * sin_t, (-s1) * sin_t, s1 * cos_t);
+  out[2] = make_float4(color[0], color[1], color[2], opacity);
+}
+
+// --------------------------------------------------------------- forward ---
+
+__global__ __launch_bounds__(kCompactBlock) void diff_count_kernel(DiffLaunch L, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t s_w[kCompactBlock / kWave];
+  const uint32_t b = blockIdx.x;
+  const uint32_t v = L.block_view[b];
+  const DiffView& view = L.views[v];
+  const DiffScene& sc = L.scenes[view.scene];
+  const uint32_t id = (b - view.block_base) * kCompactBlock + threadIdx.x;
+  bool vis = false;
+  if (id < sc.n) {
+    float omz;
+    vis = rank_visible(view.fc.PVM, sc.means[3 * (size_t)id], sc.means[3 * (size_t)id + 1],
+                       sc.means[3 * (size_t)id + 2], &omz);
+  }
+  const uint64_t m = __ballot(vis);
+  if (__lane_id() == 0) s_w[threadIdx.x / kWave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < kCompactBlock / kWave; ++i) s += s_w[i];
+    counts[b] = s;
+  }
+}
+
+__global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L, const uint32_t* __restrict__ offsets,
+                                                                   float4* __restrict__ inst, uint32_t* __restrict__ key,
+                                                                   uint32_t* __restrict__ val,
+                                                                   uint32_t* __restrict__ ids,
+                                                                   uint32_t* __restrict__ vid) {
+  __shared__ uint32_t s_w[kCompactBlock / kWave];
+  const uint32_t b = blockIdx.x;
+  const uint32_t v = L.block_view[b];
+  const DiffView& view = L.views[v];
+  const DiffScene& sc = L.scenes[view.scene];
+  const uint32_t id = (b - view.block_base) * kCompactBlock + threadIdx.x;
+  bool vis = false;
+  float omz = 0.f;
+  if (id < sc.n)
+    vis = rank_visible(view.fc.PVM, sc.means[3 * (size_t)id], sc.means[3 * (size_t)id + 1],
+                       sc.means[3 * (size_t)id + 2], &omz);
+  const uint64_t m = __ballot(vis);
+  const uint32_t w = threadIdx.x / kWave;
+  if (__lane_id() == 0) s_w[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t slot = offsets[b];
+  for (uint32_t i = 0; i < w; ++i) slot += s_w[i];
+  slot += mbcnt(m);
+  if (vis) {
+    float4 o[3];
+    diff_project(view.fc, sc, id, o);
+    inst[3 * (size_t)slot + 0] = o[0];
+    inst[3 * (size_t)slot + 1] = o[1];
+    inst[3 * (size_t)slot + 2] = o[2];
+    key[slot] = __float_as_uint(omz);
+    val[slot] = slot;
+    ids[slot] = id;
+    vid[slot] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void diff_bin_count_kernel(DiffLaunch L, const float4* __restrict__ inst,
+                                                             const uint32_t* __restrict__ order,
+                                                             const uint32_t* __restrict__ vid,
+                                                             const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                             float4* __restrict__ rec, TileRect* __restrict__ rect,
+                                                             uint32_t* __restrict__ cnt,
+                                                             uint32_t* __restrict__ kview) {
+  const uint32_t n = min(*d_count, max_n);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t slot = order[k];
+    const uint32_t v = vid[slot];
+    const FrameConst& fc = L.views[v].fc;
+    const float4 a = inst[3 * (size_t)slot], b = inst[3 * (size_t)slot + 1], c = inst[3 * (size_t)slot + 2];
+    const float f[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+    Splat2D s;
+    uint32_t count = 0;
+    TileRect r = {0u, 0u};
+    if (setup_splat(f, fc.width, fc.height, &s)) {
+      const uint32_t tx0 = (uint32_t)s.x0 >> L.tile_shift_x, tx1 = (uint32_t)s.x1 >> L.tile_shift_x;
+      const uint32_t ty0 = (uint32_t)s.y0 >> L.tile_shift_y, ty1 = (uint32_t)s.y1 >> L.tile_shift_y;
+      r.x = tx0 | (tx1 << 16);
+      r.y = ty0 | (ty1 << 16);
+      count = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    } else {
+      s.x0 = 1;
+      s.x1 = 0;
+      s.y0 = 1;
+      s.y1 = 0;
+    }
+    rec[3 * (size_t)k + 0] = make_float4(s.cx, s.cy, s.i00, s.i01);
+    rec[3 * (size_t)k + 1] = make_float4(s.i10, s.i11, s.o, __uint_as_float(pack16(s.x0, s.x1)));
+    rec[3 * (size_t)k + 2] = make_float4(s.r, s.g, s.b, __uint_as_float(pack16(s.y0, s.y1)));
+    rect[k] = r;
+    cnt[k] = count;
+    kview[k] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const TileRect* __restrict__ rect,
+                                                            const uint32_t* __restrict__ cnt,
+                                                            const uint32_t* __restrict__ offs,
+                                                            const uint32_t* __restrict__ kview,
+                                                            const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                            uint64_t max_pairs, uint32_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ vals,
+                                                            uint32_t* __restrict__ status) {
+  const uint32_t n = min(*d_count, max_n);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t c = cnt[k];
+    if (c == 0) continue;
+    const uint64_t o = offs[k];
+    const TileRect r = rect[k];
+    const DiffView& view = L.views[kview[k]];
+    if (o + c > max_pairs) atomicOr(status, 1u);
+    const uint32_t tx0 = r.x & 0xffffu, tx1 = r.x >> 16, ty0 = r.y & 0xffffu, ty1 = r.y >> 16;
+    uint64_t j = o;
+    for (uint32_t ty = ty0; ty <= ty1; ++ty)
+      for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j) {
+        if (j < max_pairs) {
+          keys[j] = view.tile_base + ty * view.fc.tiles_x + tx;
+          vals[j] = k;
+        }
+      }
+  }
+}
+
+// Forward raster of one global tile; optionally keeps the per-pixel state the
+// backward needs.  Same compositing as the plain raster, with alpha clamped.
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const float4* __restrict__ rec,
+                                                             const uint32_t* __restrict__ vals,
+                                                             const uint32_t* __restrict__ ranges, int keep_state) {
+  constexpr int kN = TW * TH;
+  constexpr int kRowsPerWave = kWave / TW;
+  __shared__ float4 s_rec[2][3][kN];
+
+  const uint32_t gtile = blockIdx.x;
+  const DiffView& view = L.views[L.tile_view[gtile]];
+  const uint32_t tile = gtile - view.tile_base;
+  const uint32_t W = view.fc.width, H = view.fc.height;
+  const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
+  const int t = threadIdx.x;
+  const int px = (int)tx * TW + t % TW, py = (int)ty * TH + t / TW;
+  const bool inside = px < (int)W && py < (int)H;
+  const int col0 = __builtin_amdgcn_readfirstlane((int)tx * TW), col1 = col0 + TW - 1;
+  const int row0 = __builtin_amdgcn_readfirstlane((int)ty * TH + (t / kWave) * kRowsPerWave);
+  const int row1 = row0 + kRowsPerWave - 1;
+  const uint32_t start = ranges[2 * gtile], end = ranges[2 * gtile + 1];
+  const uint32_t items = end - start;
+  const uint32_t batches = (items + kN - 1) / kN;
+  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+  const float t_min = L.t_min;
+
+  float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
+  if ((uint32_t)t < items) {
+    const uint32_t k = vals[end - 1 - t];
+    p0 = rec[3 * (size_t)k];
+    p1 = rec[3 * (size_t)k + 1];
+    p2 = rec[3 * (size_t)k + 2];
+  }
+  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
+  bool done = !inside;
+  uint32_t last = items;
+  for (uint32_t b = 0; b < batches; ++b) {
+    const int buf = b & 1;
+    s_rec[buf][0][t] = p0;
+    s_rec[buf][1][t] = p1;
+    s_rec[buf][2][t] = p2;
+    if (__syncthreads_count(!done) == 0) break;
+    const uint32_t next = (b + 1) * kN + t;
+    if (next < items) {
+      const uint32_t k = vals[end - 1 - next];
+      p0 = rec[3 * (size_t)k];
+      p1 = rec[3 * (size_t)k + 1];
+      p2 = rec[3 * (size_t)k + 2];
+    }
+    const int cnt = (int)min((uint32_t)kN, items - b * kN);
+    if (__ballot(!done) != 0) {
+      for (int e = 0; e < cnt; ++e) {
+        const float4 a = s_rec[buf][0][e];
+        const float4 bb = s_rec[buf][1][e];
+        const float4 c = s_rec[buf][2][e];
+        const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
+        const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
+        const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
+        const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
+        if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
+        const float dx = fx - a.x, dy = fy - a.y;
+        const float ux = fmaf(a.z, dx, a.w * dy);
+        const float uy = fmaf(bb.x, dx, bb.y * dy);
+        const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && !done;
+        const float q = fmaf(ux, ux, uy * uy);
+        const float alpha = in ? fminf(bb.z * __expf(-0.5f * q), kAlphaMax) : 0.f;
+        const float ta = T * alpha;
+        cr = fmaf(c.x, ta, cr);
+        cg = fmaf(c.y, ta, cg);
+        cb = fmaf(c.z, ta, cb);
+        aa = fmaf(alpha, ta, aa);
+        T = T - ta;
+        if (in && T < t_min) {
+          done = true;
+          last = b * kN + e + 1;
+        }
+        if (__ballot(!done) == 0) break;
+      }
+    }
+  }
+  if (inside) {
+    const size_t pix = (size_t)py * W + px;
+    view.out[pix] = make_float4(cr, cg, cb, aa + T);
+    if (keep_state) {
+      view.final_t[pix] = T;
+      view.last[pix] = last;
+    }
+  }
+}
+
+// -------------------------------------------------------------- backward ---
+
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
+                                                                      const uint32_t* __restrict__ vals,
+                                                                      const uint32_t* __restrict__ ranges,
+                                                                      float* __restrict__ g9) {
+  constexpr int kN = TW * TH;
+  constexpr int kRowsPerWave = kWave / TW;
+  __shared__ float4 s_rec[3][kN];
+  __shared__ float s_g[9][kN];
+  __shared__ uint32_t s_k[kN];
+  __shared__ uint32_t s_max;
+
+  const uint32_t gtile = blockIdx.x;
+  const DiffView& view = L.views[L.tile_view[gtile]];
+  const uint32_t tile = gtile - view.tile_base;
+  const uint32_t W = view.fc.width, H = view.fc.height;
+  const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
+  const int t = threadIdx.x;
+  const int px = (int)tx * TW + t % TW, py = (int)ty * TH + t / TW;
+  const bool inside = px < (int)W && py < (int)H;
+  const int col0 = __builtin_amdgcn_readfirstlane((int)tx * TW), col1 = col0 + TW - 1;
+  const int row0 = __builtin_amdgcn_readfirstlane((int)ty * TH + (t / kWave) * kRowsPerWave);
+  const int row1 = row0 + kRowsPerWave - 1;
+  const uint32_t end = ranges[2 * gtile + 1];
+  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+
+  uint32_t last = 0;
+  float T = 1.f;
+  float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (inside) {
+    const size_t pix = (size_t)py * W + px;
+    last = view.last[pix];
+    T = view.final_t[pix];
+    g = view.grad_out[pix];
+  }
+  if (t == 0) s_max = 0;
+  __syncthreads();
+  if (last) atomicMax(&s_max, last);
+  __syncthreads();
+  const uint32_t max_last = s_max;
+  float rc = 0.f, rg = 0.f, rb = 0.f, ra = 1.f;  // colour / alpha composited behind the current entry
+
+  // traversal index i (0 = nearest) maps to list position end - 1 - i; walk i downwards
+  for (uint32_t hi = max_last; hi > 0;) {
+    const uint32_t lo = hi > (uint32_t)kN ? hi - kN : 0u;
+    const int cnt = (int)(hi - lo);
+    __syncthreads();
+    if (t < cnt) {
+      const uint32_t i = hi - 1 - t;
+      const uint32_t k = vals[end - 1 - i];
+      s_k[t] = k;
+      s_rec[0][t] = rec[3 * (size_t)k];
+      s_rec[1][t] = rec[3 * (size_t)k + 1];
+      s_rec[2][t] = rec[3 * (size_t)k + 2];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) s_g[j][t] = 0.f;
+    }
+    __syncthreads();
+    for (int e = 0; e < cnt; ++e) {
+      const uint32_t i = hi - 1 - e;
+      const float4 a = s_rec[0][e];
+      const float4 bb = s_rec[1][e];
+      const float4 c = s_rec[2][e];
+      const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
+      const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
+      const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
+      const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
+      if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
+      const float dx = fx - a.x, dy = fy - a.y;
+      const float ux = fmaf(a.z, dx, a.w * dy);
+      const float uy = fmaf(bb.x, dx, bb.y * dy);
+      const bool in = i < last && fabsf(ux) <= 3.f && fabsf(uy) <= 3.f;
+      if (!in) continue;
+      const float q = fmaf(ux, ux, uy * uy);
+      const float G = __expf(-0.5f * q);
+      const float raw = bb.z * G;
+      const float alpha = fminf(raw, kAlphaMax);
+      const float om = 1.f - alpha;
+      T = T / om;  // transmittance in front of entry i
+      const float ta = T * alpha;
+      const float dalpha = T * (g.x * (c.x - rc) + g.y * (c.y - rg) + g.z * (c.z - rb) + g.w * (2.f * alpha - ra));
+      rc = alpha * c.x + om * rc;
+      rg = alpha * c.y + om * rg;
+      rb = alpha * c.z + om * rb;
+      ra = alpha * alpha + om * ra;
+      const bool clamped = raw > kAlphaMax;
+      const float dop = clamped ? 0.f : dalpha * G;
+      const float dq = clamped ? 0.f : -0.5f * dalpha * alpha;
+      atomicAdd(&s_g[0][e], g.x * ta);
+      atomicAdd(&s_g[1][e], g.y * ta);
+      atomicAdd(&s_g[2][e], g.z * ta);
+      atomicAdd(&s_g[3][e], dop);
+      atomicAdd(&s_g[4][e], dq * dx * dx);
+      atomicAdd(&s_g[5][e], dq * 2.f * dx * dy);
+      atomicAdd(&s_g[6][e], dq * dy * dy);
+      atomicAdd(&s_g[7][e], -2.f * dq * (a.z * ux + bb.x * uy));
+      atomicAdd(&s_g[8][e], -2.f * dq * (a.w * ux + bb.y * uy));
+    }
+    __syncthreads();
+    if (t < cnt) {
+      const uint32_t k = s_k[t];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const float v = s_g[j][t];
+        if (v != 0.f) unsafeAtomicAdd(&g9[9 * (size_t)k + j], v);
+      }
+    }
+    hi = lo;
+  }
+}
+
+// Chains the 9 per-entry gradient words of draw-order position k to the raw
+// parameters of its splat.  Row-major 3x3 helpers below.
+__device__ __forceinline__ void mm3(const float* A, const float* B, float* C) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+}
+__device__ __forceinline__ void mm3_tn(const float* A, const float* B, float* C) {  // A^T B
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) C[3 * r + c] = A[r] * B[c] + A[3 + r] * B[3 + c] + A[6 + r] * B[6 + c];
+}
+__device__ __forceinline__ void mm3_nt(const float* A, const float* B, float* C) {  // A B^T
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[3 * c] + A[3 * r + 1] * B[3 * c + 1] + A[3 * r + 2] * B[3 * c + 2];
+}
+
+__global__ __launch_bounds__(256) void diff_project_backward_kernel(DiffLaunch L, const uint32_t* __restrict__ order,
+                                                                    const uint32_t* __restrict__ ids,
+                                                                    const uint32_t* __restrict__ vid,
+                                                                    const float* __restrict__ g9,
+                                                                    const uint32_t* __restrict__ d_count,
+                                                                    uint32_t max_n) {
+  const uint32_t n = min(*d_count, max_n);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const float* gk = g9 + 9 * (size_t)k;
+    float gv[9];
+    bool any = false;
+    for (int j = 0; j < 9; ++j) {
+      gv[j] = gk[j];
+      any = any || gv[j] != 0.f;
+    }
+    if (!any) continue;
+    const uint32_t slot = order[k];
+    const uint32_t id = ids[slot];
+    const DiffView& view = L.views[vid[slot]];
+    const DiffScene& sc = L.scenes[view.scene];
+    const FrameConst& fc = view.fc;
+
+    // ---- recompute the forward (row-major) ----
+    float q[4];
+    const float qlen = load_quat(sc, id, q);
+    float R[9];
+    quat_rot(q, R);
+    float s[3];
+    for (int i = 0; i < 3; ++i) s[i] = expf(sc.log_scales[3 * (size_t)id + i]);
+    float RS2[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) RS2[3 * r + c] = R[3 * r + c] * s[c] * s[c];
+    float Sig[9];
+    mm3_nt(RS2, R, Sig);
+    // W = V3 * M3 (row-major from the column-major matrices)
+    float V3[9], M3[9], Wm[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        V3[3 * r + c] = fc.V[4 * c + r];
+        M3[3 * r + c] = fc.M[4 * c + r];
+      }
+    mm3(V3, M3, Wm);
+    const float mx = sc.means[3 * (size_t)id], my = sc.means[3 * (size_t)id + 1], mz = sc.means[3 * (size_t)id + 2];
+    float p4[4] = {mx, my, mz, 1.f}, t4[4], pv[4];
+    m4_mulv(fc.M, p4, t4);
+    m4_mulv(fc.V, t4, pv);
+    const float x = pv[0], y = pv[1], z = pv[2];
+    float WS[9], Sp[9];
+    mm3(Wm, Sig, WS);
+    mm3_nt(WS, Wm, Sp);
+    const float iz = 1.f / z, iz2 = iz * iz;
+    const float J2[6] = {-iz, 0.f, x * iz2, 0.f, -iz, y * iz2};
+    float JS[6];  // J2 Sp (2x3)
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 3; ++c) JS[3 * r + c] = J2[3 * r] * Sp[c] + J2[3 * r + 1] * Sp[3 + c] + J2[3 * r + 2] * Sp[6 + c];
+    float S2[4];  // J2 Sp J2^T
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c)
+        S2[2 * r + c] = JS[3 * r] * J2[3 * c] + JS[3 * r + 1] * J2[3 * c + 1] + JS[3 * r + 2] * J2[3 * c + 2];
+    const float Pm[4] = {fc.P[0], fc.P[4], fc.P[1], fc.P[5]};  // mat2(projection), row-major
+    float PS[4], cov[4];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) PS[2 * r + c] = Pm[2 * r] * S2[c] + Pm[2 * r + 1] * S2[2 + c];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) cov[2 * r + c] = PS[2 * r] * Pm[c] + PS[2 * r + 1] * Pm[2 + c];
+    cov[0] += fc.inv_w2;
+    cov[3] += fc.inv_h2;
+    const float hw = 0.5f * (float)fc.width, hh = 0.5f * (float)fc.height;
+    const float cp00 = cov[0] * hw * hw, cp01 = cov[1] * hw * hh, cp10 = cov[2] * hh * hw, cp11 = cov[3] * hh * hh;
+    const float det = cp00 * cp11 - cp01 * cp10;
+    const float Q00 = cp11 / det, Q01 = -cp01 / det, Q10 = -cp10 / det, Q11 = cp00 / det;
+
+    // ---- conic -> cov2d (pixels) -> cov2d (NDC) -> S2 ----
+    const float GQ00 = gv[4], GQ01 = 0.5f * gv[5], GQ11 = gv[6];
+    // G_px = -Q GQ Q
+    const float A00 = Q00 * GQ00 + Q01 * GQ01, A01 = Q00 * GQ01 + Q01 * GQ11;
+    const float A10 = Q10 * GQ00 + Q11 * GQ01, A11 = Q10 * GQ01 + Q11 * GQ11;
+    const float Gp00 = -(A00 * Q00 + A01 * Q10), Gp01 = -(A00 * Q01 + A01 * Q11);
+    const float Gp10 = -(A10 * Q00 + A11 * Q10), Gp11 = -(A10 * Q01 + A11 * Q11);
+    const float Gn[4] = {Gp00 * hw * hw, Gp01 * hw * hh, Gp10 * hh * hw, Gp11 * hh * hh};
+    // cov = Pm S2 Pm  ->  dS2 = Pm^T Gn Pm^T
+    float T1[4], GS2[4];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) T1[2 * r + c] = Pm[r] * Gn[c] + Pm[2 + r] * Gn[2 + c];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) GS2[2 * r + c] = T1[2 * r] * Pm[c * 2] + T1[2 * r + 1] * Pm[c * 2 + 1];
+    // S2 = J2 Sp J2^T -> dSp = J2^T GS2 J2, dJ2 = (GS2 + GS2^T) J2 Sp
+    float GSp[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        float acc = 0.f;
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) acc += J2[3 * i + r] * GS2[2 * i + j] * J2[3 * j + c];
+        GSp[3 * r + c] = acc;
+      }
+    float GJ2[6];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 3; ++c) {
+        float acc = 0.f;
+        for (int i = 0; i < 2; ++i) acc += (GS2[2 * r + i] + GS2[2 * i + r]) * JS[3 * i + c];
+        GJ2[3 * r + c] = acc;
+      }
+    // Sp = W Sig W^T -> dSig = W^T GSp W
+    float TT[9], GSig[9];
+    mm3_tn(Wm, GSp, TT);
+    mm3(TT, Wm, GSig);
+
+    // ---- view-space position: J2 and the projected centre ----
+    float gpv[3];
+    gpv[0] = GJ2[2] * iz2;
+    gpv[1] = GJ2[5] * iz2;
+    gpv[2] = (GJ2[0] + GJ2[4]) * iz2 - 2.f * (x * GJ2[2] + y * GJ2[5]) * iz2 * iz;
+    {
+      float clip[4];
+      float pv4[4] = {x, y, z, 1.f};
+      m4_mulv(fc.P, pv4, clip);
+      const float iw = 1.f / clip[3];
+      const float ndx = clip[0] * iw, ndy = clip[1] * iw;
+      const float gx = gv[7] * hw, gy = gv[8] * hh;  // d/d ndc
+      for (int j = 0; j < 3; ++j) {
+        gpv[j] += gx * (fc.P[4 * j + 0] - ndx * fc.P[4 * j + 3]) * iw;
+        gpv[j] += gy * (fc.P[4 * j + 1] - ndy * fc.P[4 * j + 3]) * iw;
+      }
+    }
+    // mean <- W^T gpv
+    float gm[3];
+    for (int j = 0; j < 3; ++j) gm[j] = Wm[j] * gpv[0] + Wm[3 + j] * gpv[1] + Wm[6 + j] * gpv[2];
+
+    // ---- colour: SH coefficients and view direction ----
+    const float d0 = mx - fc.eye_model[0], d1 = my - fc.eye_model[1], d2 = mz - fc.eye_model[2];
+    const float dl = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+    const float dx = d0 / dl, dy = d1 / dl, dz = d2 / dl;
+    float basis[16];
+    sh_basis(dx, dy, dz, basis);
+    const float* shp = sc.sh + 48 * (size_t)id;
+    float gcol[3];
+    for (int ch = 0; ch < 3; ++ch) {
+      float acc = 0.f;
+      for (int j = 0; j < 16; ++j) acc += basis[j] * shp[ch * 16 + j];
+      gcol[ch] = (acc + 0.5f < 0.f) ? 0.f : gv[ch];
+    }
+    float* gsh = sc.g_sh + 48 * (size_t)id;
+    float gb[16];
+    for (int j = 0; j < 16; ++j) {
+      gb[j] = gcol[0] * shp[j] + gcol[1] * shp[16 + j] + gcol[2] * shp[32 + j];
+      for (int ch = 0; ch < 3; ++ch)
+        if (gcol[ch] != 0.f) unsafeAtomicAdd(&gsh[ch * 16 + j], gcol[ch] * basis[j]);
+    }
+    {
+      const float C1 = 0.4886025119029199f, C20 = 1.0925484305920792f, C21 = 0.31539156525252005f,
+                  C22 = 0.5462742152960396f, C30 = 0.5900435899266435f, C31 = 2.890611442640554f,
+                  C32 = 0.4570457994644658f, C33 = 0.3731763325901154f, C34 = 1.445305721320277f;
+      const float X = dx, Y = dy, Z = dz, XX = X * X, YY = Y * Y, ZZ = Z * Z;
+      float gdx = 0.f, gdy = 0.f, gdz = 0.f;
+      gdy += gb[1] * -C1;
+      gdz += gb[2] * C1;
+      gdx += gb[3] * -C1;
+      gdx += gb[4] * C20 * Y;
+      gdy += gb[4] * C20 * X;
+      gdy += gb[5] * -C20 * Z;
+      gdz += gb[5] * -C20 * Y;
+      gdx += gb[6] * -2.f * C21 * X;
+      gdy += gb[6] * -2.f * C21 * Y;
+      gdz += gb[6] * 4.f * C21 * Z;
+      gdx += gb[7] * -C20 * Z;
+      gdz += gb[7] * -C20 * X;
+      gdx += gb[8] * 2.f * C22 * X;
+      gdy += gb[8] * -2.f * C22 * Y;
+      gdx += gb[9] * -6.f * C30 * X * Y;
+      gdy += gb[9] * -3.f * C30 * (XX - YY);
+      gdx += gb[10] * C31 * Y * Z;
+      gdy += gb[10] * C31 * X * Z;
+      gdz += gb[10] * C31 * X * Y;
+      gdx += gb[11] * 2.f * C32 * X * Y;
+      gdy += gb[11] * -C32 * (4.f * ZZ - XX - 3.f * YY);
+      gdz += gb[11] * -8.f * C32 * Y * Z;
+      gdx += gb[12] * -6.f * C33 * X * Z;
+      gdy += gb[12] * -6.f * C33 * Y * Z;
+      gdz += gb[12] * C33 * (6.f * ZZ - 3.f * XX - 3.f * YY);
+      gdx += gb[13] * -C32 * (4.f * ZZ - 3.f * XX - YY);
+      gdy += gb[13] * 2.f * C32 * X * Y;
+      gdz += gb[13] * -8.f * C32 * X * Z;
+      gdx += gb[14] * 2.f * C34 * X * Z;
+      gdy += gb[14] * -2.f * C34 * Y * Z;
+      gdz += gb[14] * C34 * (XX - YY);
+      gdx += gb[15] * -3.f * C30 * (XX - YY);
+      gdy += gb[15] * 6.f * C30 * X * Y;
+      const float dd = gdx * X + gdy * Y + gdz * Z;
+      gm[0] += (gdx - X * dd) / dl;
+      gm[1] += (gdy - Y * dd) / dl;
+      gm[2] += (gdz - Z * dd) / dl;
+    }
+    unsafeAtomicAdd(&sc.g_means[3 * (size_t)id + 0], gm[0]);
+    unsafeAtomicAdd(&sc.g_means[3 * (size_t)id + 1], gm[1]);
+    unsafeAtomicAdd(&sc.g_means[3 * (size_t)id + 2], gm[2]);
+
+    // ---- opacity ----
+    const float op = 1.f / (1.f + expf(-sc.opacity[id]));
+    unsafeAtomicAdd(&sc.g_opacity[id], gv[3] * op * (1.f - op));
+
+    // ---- Sig = R diag(s^2) R^T -> scales and quaternion ----
+    float GR[9];  // 2 GSig R diag(s^2)  (GSig symmetric)
+    float GSR[9];
+    float Gs[9];
+    for (int i = 0; i < 9; ++i) Gs[i] = 0.5f * (GSig[i] + GSig[3 * (i % 3) + i / 3]);
+    mm3(Gs, R, GSR);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) GR[3 * r + c] = 2.f * GSR[3 * r + c] * s[c] * s[c];
+    for (int c = 0; c < 3; ++c) {
+      const float rgr = R[c] * GSR[c] + R[3 + c] * GSR[3 + c] + R[6 + c] * GSR[6 + c];  // (R^T Gs R)_cc
+      unsafeAtomicAdd(&sc.g_log_scales[3 * (size_t)id + c], 2.f * rgr * s[c] * s[c]);
+    }
+    const float qx = q[0], qy = q[1], qz = q[2], qw = q[3];
+    const float gqx = 2.f * (qy * (GR[1] + GR[3]) + qz * (GR[2] + GR[6]) - 2.f * qx * (GR[4] + GR[8]) +
+                             qw * (GR[7] - GR[5]));
+    const float gqy = 2.f * (qx * (GR[1] + GR[3]) + qz * (GR[5] + GR[7]) - 2.f * qy * (GR[0] + GR[8]) +
+                             qw * (GR[2] - GR[6]));
+    const float gqz = 2.f * (qx * (GR[2] + GR[6]) + qy * (GR[5] + GR[7]) - 2.f * qz * (GR[0] + GR[4]) +
+                             qw * (GR[3] - GR[1]));
+    const float gqw = 2.f * (qx * (GR[7] - GR[5]) + qy * (GR[2] - GR[6]) + qz * (GR[3] - GR[1]));
+    const float dot = gqx * qx + gqy * qy + gqz * qz + gqw * qw;
+    float* gq = sc.g_quats + 4 * (size_t)id;  // PLY order (w, x, y, z)
+    unsafeAtomicAdd(&gq[0], (gqw - qw * dot) / qlen);
+    unsafeAtomicAdd(&gq[1], (gqx - qx * dot) / qlen);
+    unsafeAtomicAdd(&gq[2], (gqy - qy * dot) / qlen);
+    unsafeAtomicAdd(&gq[3], (gqz - qz * dot) / qlen);
+  }
+}
+
+uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return (uint32_t)(g < cap ? g : cap);
+}
+
+}  // namespace
+
+hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint32_t* d_visible, hipStream_t s) {
+  if (L.n_blocks == 0) return fill_u32_launch(d_visible, 0u, 1, s);
+  diff_count_kernel<<<L.n_blocks, kCompactBlock, 0, s>>>(L, counts);
+  const ScanPlan p = scan_plan(L.n_blocks);
+  void* partials = reinterpret_cast<char*>(counts) + align_up((size_t)L.n_blocks * 4, 256);
+  return scan_exclusive_launch(p, counts, counts, nullptr, d_visible, partials, s);
+}
+
+hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* counts, float4* inst, uint32_t* key, uint32_t* val,
+                             uint32_t* ids, uint32_t* vid, hipStream_t s) {
+  if (L.n_blocks == 0) return hipSuccess;
+  diff_write_kernel<<<L.n_blocks, kCompactBlock, 0, s>>>(L, counts, inst, key, val, ids, vid);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_bin_count(const DiffLaunch& L, const float4* inst, const uint32_t* order, const uint32_t* vid,
+                                 const uint32_t* d_count, uint32_t max_n, float4* rec, TileRect* rect, uint32_t* cnt,
+                                 uint32_t* kview, hipStream_t s) {
+  if (max_n == 0) return hipSuccess;
+  diff_bin_count_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, inst, order, vid, d_count, max_n, rec, rect,
+                                                                   cnt, kview);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const uint32_t* cnt, const uint32_t* offs,
+                                const uint32_t* kview, const uint32_t* d_count, uint32_t max_n, uint64_t max_pairs,
+                                uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s) {
+  if (max_n == 0) return hipSuccess;
+  diff_bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, rect, cnt, offs, kview, d_count, max_n,
+                                                                  max_pairs, keys, vals, status);
+  return hipGetLastError();
+}
+
+#define VKGS_DIFF_CASES(MACRO) \
+  MACRO(16, 16)                \
+  MACRO(8, 8)                  \
+  MACRO(32, 8)                 \
+  MACRO(16, 8)                 \
+  MACRO(8, 16)                 \
+  MACRO(16, 4)                 \
+  MACRO(32, 4)                 \
+  MACRO(64, 4)
+
+hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
+                              bool keep_state, hipStream_t s) {
+  if (L.n_tiles == 0) return hipSuccess;
+#define VKGS_CASE(W_, H_)                                                                             \
+  if (L.tile_w == W_ && L.tile_h == H_) {                                                             \
+    diff_raster_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, keep_state ? 1 : 0); \
+    return hipGetLastError();                                                                         \
+  }
+  VKGS_DIFF_CASES(VKGS_CASE)
+#undef VKGS_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
+                                       const uint32_t* ranges, float* g9, hipStream_t s) {
+  if (L.n_tiles == 0) return hipSuccess;
+#define VKGS_CASE(W_, H_)                                                                          \
+  if (L.tile_w == W_ && L.tile_h == H_) {                                                          \
+    diff_raster_backward_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, g9);   \
+    return hipGetLastError();                                                                      \
+  }
+  VKGS_DIFF_CASES(VKGS_CASE)
+#undef VKGS_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* order, const uint32_t* ids,
+                                        const uint32_t* vid, const float* g9, const uint32_t* d_count, uint32_t max_n,
+                                        hipStream_t s) {
+  if (max_n == 0) return hipSuccess;
+  diff_project_backward_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, order, ids, vid, g9, d_count, max_n);
+  return hipGetLastError();
+}
+
+}  // namespace vkgs

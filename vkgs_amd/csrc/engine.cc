@@ -28,7 +28,7 @@ struct DevBuf {
   size_t bytes = 0;
   hipError_t ensure(size_t want) {
     if (want <= bytes) return hipSuccess;
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
     hipError_t e = hipMalloc(&p, want ? want : 1);
@@ -36,7 +36,7 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -207,7 +207,7 @@ int vkgs_renderer_create(uint32_t capacity, vkgs_renderer** out) {
 
 int vkgs_renderer_destroy(vkgs_renderer* r) {
   if (!r) return VKGS_ERR_INVALID_ARG;
-  hipDeviceSynchronize();
+  (void)hipDeviceSynchronize();
   drop_graph(r);
   DevBuf* bufs[] = {&r->position, &r->cov3d, &r->opacity, &r->sh,   &r->fc,   &r->misc,  &r->counts,
                     &r->inst,     &r->key,   &r->val,     &r->ids,  &r->sort_storage, &r->rec, &r->rect, &r->cnt,
@@ -380,6 +380,64 @@ int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const 
   *d_list_ids = r->list_ids.as<uint32_t>();
   if (tiles_x) *tiles_x = r->rs.tiles_x;
   if (tiles_y) *tiles_y = r->rs.tiles_y;
+  return VKGS_OK;
+}
+
+int vkgs_renderer_autotune(vkgs_renderer* r, const vkgs_camera* camera, const float model[16], float* d_rgba,
+                           uint32_t frames, vkgs_raster_config* chosen, float* best_ms, void* stream) {
+  if (!r || !camera || !d_rgba) return VKGS_ERR_INVALID_ARG;
+  static const uint32_t kShapes[][2] = {{8, 8}, {16, 8}, {8, 16}, {16, 4}, {16, 16}, {32, 4}};
+  if (frames == 0) frames = 10;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const vkgs_raster_config base = r->cfg;
+  const bool timing = r->timing;
+  r->timing = false;
+  vkgs_raster_config best = base;
+  float best_t = 1e30f;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  VKGS_HIP_CHECK(hipEventCreate(&e0));
+  VKGS_HIP_CHECK(hipEventCreate(&e1));
+  int status = VKGS_OK;
+  for (const auto& sh : kShapes) {
+    vkgs_raster_config c = base;
+    c.tile_w = sh[0];
+    c.tile_h = sh[1];
+    if ((status = vkgs_renderer_set_config(r, &c)) != VKGS_OK) break;
+    // first frame sizes the pair storage; repeat while it overflows
+    for (int i = 0; i < 4; ++i) {
+      if ((status = vkgs_renderer_render(r, camera, model, d_rgba, stream)) != VKGS_OK) break;
+      status = vkgs_renderer_frame_info(r, nullptr, nullptr, stream);
+      if (status != VKGS_ERR_OVERFLOW) break;
+    }
+    if (status != VKGS_OK) break;
+    for (uint32_t i = 0; i < 3 && status == VKGS_OK; ++i) status = vkgs_renderer_render(r, camera, model, d_rgba, stream);
+    hipEventRecord(e0, s);
+    for (uint32_t i = 0; i < frames && status == VKGS_OK; ++i)
+      status = vkgs_renderer_render(r, camera, model, d_rgba, stream);
+    hipEventRecord(e1, s);
+    if (status != VKGS_OK) break;
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      status = VKGS_ERR_HIP;
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)frames;
+    if (ms < best_t) {
+      best_t = ms;
+      best = c;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  r->timing = timing;
+  if (status != VKGS_OK) {
+    vkgs_renderer_set_config(r, &base);
+    return status;
+  }
+  vkgs_renderer_set_config(r, &best);
+  if (chosen) *chosen = best;
+  if (best_ms) *best_ms = best_t;
   return VKGS_OK;
 }
 

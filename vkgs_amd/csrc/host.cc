@@ -63,8 +63,8 @@ int vkgs_abi_version(void) { return VKGS_ABI_VERSION; }
 void vkgs_raster_config_default(vkgs_raster_config* cfg) {
   if (!cfg) return;
   memset(cfg, 0, sizeof(*cfg));
-  cfg->tile_w = 16;
-  cfg->tile_h = 16;
+  cfg->tile_w = 8;
+  cfg->tile_h = 8;
   cfg->t_min = 1.f / 8192.f;
 }
 

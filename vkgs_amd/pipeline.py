@@ -249,6 +249,18 @@ class Renderer:
         ranges = _download_u32(rp.value, 2 * tiles).reshape(tiles, 2).astype(np.int64)
         return ranges, _download_u32(lp.value, k)
 
+    def autotune(self, cam, out: torch.Tensor | None = None, frames: int = 10, model=None):
+        """Picks the fastest tile shape for this camera; returns (tile_w, tile_h, ms_per_frame)."""
+        w, h = cam.screen_size[0], cam.screen_size[1]
+        if out is None:
+            out = torch.empty((h, w, 4), dtype=torch.float32, device=_dev())
+        m, mp = _model_ptr(model)
+        cfg = _lib.RasterConfig()
+        ms = C.c_float()
+        check(lib().vkgs_renderer_autotune(self._h, C.byref(cam), mp, ptr(out), frames, C.byref(cfg), C.byref(ms),
+                                           _stream()), "vkgs_renderer_autotune")
+        return cfg.tile_w, cfg.tile_h, float(ms.value)
+
     def set_timing(self, on: bool) -> None:
         check(lib().vkgs_renderer_set_timing(self._h, 1 if on else 0), "vkgs_renderer_set_timing")
 
