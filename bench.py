@@ -36,12 +36,79 @@ def parse_args(argv=None):
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c3_6m")
-    ap.add_argument("--tile", default="16x16")
+    ap.add_argument("--tile", default="auto", help="WxH, or auto (vkgs_renderer_autotune)")
     ap.add_argument("--t-min", type=float, default=1.0 / 8192.0)
     ap.add_argument("--timed-frames", type=int, default=20, help="eager frames with per-stage HIP events")
     ap.add_argument("--cpu-frames", type=int, default=2, help="oracle frames for cpu_baseline (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="timed fwd+bwd steps of the batch workload (0 = skip)")
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="scales every scene size of the batch workload")
     return ap.parse_args(argv)
+
+
+def batch_workload(scale: float = 1.0, seed: int = 5):
+    """32 views of a 3M-splat scene at resolutions drawn from {256^2, 512^2, 800^2, 1600x900} on orbit
+    cameras of radius 8-30 m, plus 8 scenes of 50k-2M splats (log-uniform) with 2 views each."""
+    import numpy as np
+    from vkgs_amd import scene as S
+
+    rng = np.random.default_rng(seed)
+    big = S.clustered(max(1000, int(3_000_000 * scale)), seed=50, clusters=64, box=(40.0, 10.0, 40.0))
+    scenes = [big]
+    counts = np.exp(rng.uniform(np.log(50_000), np.log(2_000_000), 8))
+    for i, c in enumerate(counts):
+        scenes.append(S.gaussian_cloud(max(1000, int(c * scale)), seed=60 + i, pos_std=4.0))
+    res = [(256, 256), (512, 512), (800, 800), (1600, 900)]
+    views = []
+
+    def cam(w, h):
+        r = rng.uniform(8.0, 30.0)
+        phi = rng.uniform(0.35, 1.35)
+        theta = rng.uniform(0.0, 2 * np.pi)
+        return S.orbit_camera(w, h, r, phi=phi, theta=theta)
+
+    for _ in range(32):
+        views.append((0, cam(*res[rng.integers(0, 4)])))
+    for i in range(8):
+        for _ in range(2):
+            views.append((1 + i, cam(*res[rng.integers(0, 4)])))
+    return scenes, views
+
+
+def bench_fwd_bwd(steps: int, scale: float):
+    """views/s of fwd+bwd (L1 loss) for the batch workload: one launch set per view
+    (plain) and one launch set for all views (batched)."""
+    import torch
+    from vkgs_amd import diff as D
+
+    scenes, views = batch_workload(scale)
+    params = [D.scene_tensors(s) for s in scenes]
+    targets = [torch.rand((c.screen_size[1], c.screen_size[0], 4), device="cuda") for _, c in views]
+
+    def plain_step():
+        for (si, c), tgt in zip(views, targets):
+            img = D.render(**params[si], camera=c)
+            (img - tgt).abs().mean().backward()
+
+    def batched_step():
+        imgs = D.render_batch(params, views)
+        sum((img - tgt).abs().mean() for img, tgt in zip(imgs, targets)).backward()
+
+    out = {}
+    for name, fn in (("plain", plain_step), ("batched", batched_step)):
+        fn()  # warm-up: sizes every buffer
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[f"{name}_views_per_s"] = round(len(views) / dt, 2)
+        out[f"{name}_ms_per_view"] = round(dt * 1e3 / len(views), 3)
+    out["batched_over_plain"] = round(out["batched_views_per_s"] / out["plain_views_per_s"], 3)
+    out["views"] = len(views)
+    out["splats"] = [s.n for s in scenes]
+    return out
 
 
 def log(*a):
@@ -69,7 +136,6 @@ def main(argv=None) -> int:
     from vkgs_amd import scene as S
 
     wl = S.WORKLOADS[args.workload]
-    tw, th = (int(x) for x in args.tile.split("x"))
     log(f"generating {wl.name}: {wl.n} splats")
     t0 = time.perf_counter()
     sc = wl.make()
@@ -78,8 +144,14 @@ def main(argv=None) -> int:
 
     r = P.Renderer(sc.n)
     r.load_buffer(sc.ply_buffer(), sc.n)
-    r.set_config(tw, th, args.t_min)
     out = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
+    if args.tile == "auto":
+        r.set_config(8, 8, args.t_min)
+        tw, th, tuned_ms = r.autotune(cam, out, frames=10)
+        log(f"autotuned tile {tw}x{th} ({tuned_ms:.3f} ms/frame)")
+    else:
+        tw, th = (int(x) for x in args.tile.split("x"))
+        r.set_config(tw, th, args.t_min)
     for _ in range(3):  # grows the pair storage if the first frame overflowed it
         r.render(cam, out)
         status, visible, pairs = r.frame_info()
@@ -171,17 +243,30 @@ def main(argv=None) -> int:
         "hbm_frac_projection_sort": round(ps_gbs / HBM_PEAK_GBS, 4),
     }
 
-    # ---- CPU baseline: the oracle's full frame on the host cores ----
+    # host copy of the splat storage for the CPU baseline, then free the renderer
+    host_storage = None
     if not args.no_cpu_baseline and args.cpu_frames > 0:
+        st = r.storage()
+        host_storage = (st.position.cpu().numpy(), st.cov3d.cpu().numpy(), st.opacity.cpu().numpy(),
+                        st.sh.cpu().numpy().view(np.uint16))
+        del st
+    r.close()
+    torch.cuda.empty_cache()
+
+    # ---- fwd+bwd views/s: plain (one view per launch set) vs batched ----
+    if args.fwd_bwd_steps > 0:
+        try:
+            result["fwd_bwd"] = bench_fwd_bwd(args.fwd_bwd_steps, args.batch_scale)
+            log("fwd+bwd", result["fwd_bwd"])
+        except Exception as e:  # reported, never hides the frame figure
+            result["fwd_bwd"] = {"error": repr(e)}
+
+    # ---- CPU baseline: the oracle's full frame on the host cores ----
+    if host_storage is not None:
         try:
             import oracle
 
-            st = r.storage()
-            pos = st.position.cpu().numpy()
-            cov = st.cov3d.cpu().numpy()
-            opa = st.opacity.cpu().numpy()
-            sh = st.sh.cpu().numpy().view(np.uint16)
-            del st
+            pos, cov, opa, sh = host_storage
             oracle.lib()
             times = []
             for _ in range(args.cpu_frames):
@@ -198,7 +283,6 @@ def main(argv=None) -> int:
         except Exception as e:  # the baseline never blocks the GPU figure
             result["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(result), flush=True)
-    r.close()
     return 0
 
 

@@ -61,7 +61,8 @@ class RasterConfig(C.Structure):
         ("tile_w", C.c_uint32),
         ("tile_h", C.c_uint32),
         ("t_min", C.c_float),
-        ("reserved", C.c_uint32 * 5),
+        ("variant", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 

@@ -83,16 +83,39 @@ hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d
   if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
   if ((e = launch_bin_count(inst, order, d_count, max_inst, rs, b.rec, b.rect, b.cnt, s)) != hipSuccess) return e;
   if ((e = scan_exclusive_launch(b.scan, b.cnt, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess) return e;
-  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, d_count, max_inst, rs, max_pairs, b.keys, b.vals, b.status, s)) !=
-      hipSuccess)
+  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, nullptr, d_count, max_inst, rs, max_pairs, b.keys, b.vals, b.status,
+                           s)) != hipSuccess)
     return e;
-  if (marks) hipEventRecord(marks[0], s);
+  if (marks) (void)hipEventRecord(marks[0], s);
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   if ((e = sort_kv_launch(b.sort, b.pairs, b.keys, b.vals, (uint32_t)bits_for(tiles), b.sort_storage, s)) !=
       hipSuccess)
     return e;
   if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
-  if (marks) hipEventRecord(marks[1], s);
+  if (marks) (void)hipEventRecord(marks[1], s);
+  return launch_rasterize(b.rec, b.vals, b.ranges, rs, out, s);
+}
+
+// Fused-path draw: raster records and (rect, count) words were written per
+// slot by the projection; order[k] is the slot drawn k-th.  Pair values and
+// the raster's record indices are slots.
+hipError_t run_draw_slots(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
+                          const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
+                          hipEvent_t* marks) {
+  hipError_t e;
+  if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
+  if ((e = launch_bin_gather(rc, order, d_count, max_inst, b.rect, b.cnt, s)) != hipSuccess) return e;
+  if ((e = scan_exclusive_launch(b.scan, b.cnt, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess) return e;
+  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, order, d_count, max_inst, rs, max_pairs, b.keys, b.vals, b.status,
+                           s)) != hipSuccess)
+    return e;
+  if (marks) (void)hipEventRecord(marks[0], s);
+  const uint32_t tiles = rs.tiles_x * rs.tiles_y;
+  if ((e = sort_kv_launch(b.sort, b.pairs, b.keys, b.vals, (uint32_t)bits_for(tiles), b.sort_storage, s)) !=
+      hipSuccess)
+    return e;
+  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
+  if (marks) (void)hipEventRecord(marks[1], s);
   return launch_rasterize(b.rec, b.vals, b.ranges, rs, out, s);
 }
 

@@ -55,7 +55,7 @@ struct vkgs_renderer {
 
   // frame buffers sized by n
   uint32_t frame_n = 0;
-  DevBuf fc, misc, counts, inst, key, val, ids, sort_storage;
+  DevBuf fc, misc, counts, rc, key, val, ids, sort_storage;
   // binning sized by (n, pairs, tiles)
   uint64_t max_pairs = 0;
   DevBuf rec, rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges;
@@ -112,7 +112,7 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   if ((e = r->fc.ensure(sizeof(FrameConst))) != hipSuccess) return e;
   if ((e = r->misc.ensure(kMiscWords * 4)) != hipSuccess) return e;
   if ((e = r->counts.ensure(compact_storage_bytes(cap))) != hipSuccess) return e;
-  if ((e = r->inst.ensure((size_t)cap * 48)) != hipSuccess) return e;
+  if ((e = r->rc.ensure((size_t)cap * 16)) != hipSuccess) return e;
   if ((e = r->key.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->val.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->ids.ensure((size_t)cap * 4)) != hipSuccess) return e;
@@ -161,8 +161,8 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
     return e;
   if (timed) hipEventRecord(r->ev[1], s);
   if ((e = launch_fused_write(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
-                              r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->inst.as<float4>(),
-                              r->key.as<uint32_t>(), r->val.as<uint32_t>(), r->ids.as<uint32_t>(),
+                              r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(),
+                              r->rc.as<uint4>(), r->key.as<uint32_t>(), r->val.as<uint32_t>(), r->ids.as<uint32_t>(),
                               r->counts.as<uint32_t>(), s)) != hipSuccess)
     return e;
   if (timed) hipEventRecord(r->ev[2], s);
@@ -172,8 +172,9 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
     return e;
   if (timed) hipEventRecord(r->ev[3], s);
   hipEvent_t marks[2] = {r->ev[4], r->ev[5]};
-  if ((e = run_draw(r->inst.as<float4>(), r->val.as<uint32_t>(), misc + kMiscVisible, n ? n : 1, r->rs, r->max_pairs,
-                    draw_buffers(r), reinterpret_cast<float4*>(out), s, timed ? marks : nullptr)) != hipSuccess)
+  if ((e = run_draw_slots(r->rc.as<uint4>(), r->val.as<uint32_t>(), misc + kMiscVisible, n ? n : 1, r->rs,
+                          r->max_pairs, draw_buffers(r), reinterpret_cast<float4*>(out), s,
+                          timed ? marks : nullptr)) != hipSuccess)
     return e;
   if (timed) hipEventRecord(r->ev[6], s);
   return hipSuccess;
@@ -210,7 +211,7 @@ int vkgs_renderer_destroy(vkgs_renderer* r) {
   (void)hipDeviceSynchronize();
   drop_graph(r);
   DevBuf* bufs[] = {&r->position, &r->cov3d, &r->opacity, &r->sh,   &r->fc,   &r->misc,  &r->counts,
-                    &r->inst,     &r->key,   &r->val,     &r->ids,  &r->sort_storage, &r->rec, &r->rect, &r->cnt,
+                    &r->rc,       &r->key,   &r->val,     &r->ids,  &r->sort_storage, &r->rec, &r->rect, &r->cnt,
                     &r->offs,     &r->scan_storage, &r->pkeys, &r->pvals, &r->psort, &r->ranges, &r->sorted_ids,
                     &r->list_ids};
   for (DevBuf* b : bufs) b->release();
@@ -372,9 +373,9 @@ int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const 
   const uint64_t cap = r->max_pairs ? r->max_pairs : 1;
   VKGS_HIP_CHECK(r->list_ids.ensure((size_t)cap * 4));
   uint32_t* misc = r->misc.as<uint32_t>();
-  // pairs hold draw-order positions k; k -> slot (depth-sorted val) -> splat id
-  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val.as<uint32_t>(), r->pvals.as<uint32_t>(),
-                                   misc + kMiscPairs, (uint32_t)cap, r->list_ids.as<uint32_t>(), nullptr));
+  // pairs hold slots; slot -> splat id
+  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->pvals.as<uint32_t>(), nullptr, misc + kMiscPairs,
+                                   (uint32_t)cap, r->list_ids.as<uint32_t>(), nullptr));
   VKGS_HIP_CHECK(hipDeviceSynchronize());
   *d_ranges = r->ranges.as<uint32_t>();
   *d_list_ids = r->list_ids.as<uint32_t>();

@@ -33,13 +33,15 @@ hipError_t launch_projection(const FrameConst& fc, const float* pos, const float
                              const uint16_t* sh, const int32_t* inverse, uint32_t n, const uint32_t* d_count,
                              float4* inst, uint32_t* draw_indirect, hipStream_t s);
 
-// Fused cull + project + compaction (frame constants read from device memory),
-// in two launches: visible counts per workgroup + their scan (-> *d_count),
-// then instance, key = bits(1 - z), val = slot, ids[slot] = splat id.
+// Fused cull + project + compaction (frame constants read from device memory,
+// including the screen and tile shape), in two launches: visible counts per
+// workgroup + their scan (-> *d_count), then per visible splat at its slot:
+// raster record rec[3 slot..], (tile rect x, tile rect y, tile count, 0) in
+// rc[slot], key = bits(1 - z), val = slot, ids[slot] = splat id.
 hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t n, uint32_t* d_count,
                               uint32_t* counts, hipStream_t s);
 hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
-                              const uint16_t* sh, uint32_t n, float4* inst, uint32_t* key, uint32_t* val,
+                              const uint16_t* sh, uint32_t n, float4* rec, uint4* rc, uint32_t* key, uint32_t* val,
                               uint32_t* ids, const uint32_t* counts, hipStream_t s);
 
 // Copies a FrameConst passed by value into device memory (stream-ordered).
@@ -51,6 +53,7 @@ struct RasterShape {
   uint32_t width, height, tile_w, tile_h, tiles_x, tiles_y;
   uint32_t tile_shift_x, tile_shift_y;
   float t_min;
+  uint32_t variant;  // 0: scalar-load waves, 1: LDS batches
 };
 
 bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config* cfg, RasterShape* rs);
@@ -59,10 +62,15 @@ bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config
 // instance order[k] (order == nullptr: k), its tile rectangle and tile count.
 hipError_t launch_bin_count(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
                             const RasterShape& rs, float4* rec, TileRect* rect, uint32_t* cnt, hipStream_t s);
-// (tile, k) pairs at offs[k]; pairs beyond max_pairs set *status.
-hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* d_count,
-                           uint32_t max_n, const RasterShape& rs, uint64_t max_pairs, uint32_t* keys, uint32_t* vals,
-                           uint32_t* status, hipStream_t s);
+// Depth-ordered tile rectangles and counts from the per-slot words of the
+// fused projection: rect[k], cnt[k] of slot order[k].
+hipError_t launch_bin_gather(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
+                             TileRect* rect, uint32_t* cnt, hipStream_t s);
+// (tile, k) pairs at offs[k]; pairs beyond max_pairs set *status.  With
+// `order` the pair value is order[k] instead of k.
+hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* order,
+                           const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
+                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s);
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s);
 hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
@@ -91,6 +99,12 @@ struct DrawBuffers {
 hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
                     const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
                     hipEvent_t* marks);
+
+// Same after the fused projection: b.rec holds per-slot records, rc the
+// per-slot (rect, count) words, order[k] the slot drawn k-th.
+hipError_t run_draw_slots(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
+                          const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
+                          hipEvent_t* marks);
 
 // out[k] = src[idx[k]] (idx2 == nullptr) or src[idx[idx2[k]]] for k < *d_count.
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* idx2, const uint32_t* d_count,

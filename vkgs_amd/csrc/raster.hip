@@ -66,9 +66,22 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const float4* __restrict
   }
 }
 
+__global__ __launch_bounds__(256) void bin_gather_kernel(const uint4* __restrict__ rc,
+                                                         const uint32_t* __restrict__ order,
+                                                         const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                         TileRect* __restrict__ rect, uint32_t* __restrict__ cnt) {
+  const uint32_t n = min(*d_count, max_n);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint4 v = rc[order[k]];
+    rect[k] = TileRect{v.x, v.y};
+    cnt[k] = v.z;
+  }
+}
+
 __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restrict__ rect,
                                                        const uint32_t* __restrict__ cnt,
                                                        const uint32_t* __restrict__ offs,
+                                                       const uint32_t* __restrict__ order,
                                                        const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                        uint32_t tiles_x, uint64_t max_pairs,
                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
@@ -79,6 +92,7 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
     if (c == 0) continue;
     const uint64_t o = offs[k];
     const TileRect r = rect[k];
+    const uint32_t v = order ? order[k] : k;
     if (o + c > max_pairs) atomicOr(status, 1u);
     const uint32_t tx0 = r.x & 0xffffu, tx1 = r.x >> 16, ty0 = r.y & 0xffffu, ty1 = r.y >> 16;
     uint64_t j = o;
@@ -86,7 +100,7 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
       for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j) {
         if (j < max_pairs) {
           keys[j] = ty * tiles_x + tx;
-          vals[j] = k;
+          vals[j] = v;
         }
       }
   }
@@ -196,6 +210,68 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
   if (inside) out[(size_t)py * rs.width + px] = make_float4(cr, cg, cb, aa + T);
 }
 
+// Scalar-broadcast raster: every wave walks its tile's list on its own.  The
+// list entry and its 48-byte record have wave-uniform addresses, so they are
+// fetched with scalar loads into SGPRs once per wave (no LDS traffic, no
+// barrier); the next record is fetched while the current one is composited.
+// Waves of one tile share records through the scalar cache.
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void rasterize_wave_kernel(const float4* __restrict__ rec,
+                                                                const uint32_t* __restrict__ vals,
+                                                                const uint32_t* __restrict__ ranges, RasterShape rs,
+                                                                float4* __restrict__ out) {
+  constexpr int kRowsPerWave = kWave / TW;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t tx = tile % rs.tiles_x, ty = tile / rs.tiles_x;
+  const int t = threadIdx.x;
+  const int px = (int)tx * TW + t % TW, py = (int)ty * TH + t / TW;
+  const bool inside = px < (int)rs.width && py < (int)rs.height;
+  const int col0 = __builtin_amdgcn_readfirstlane((int)tx * TW), col1 = col0 + TW - 1;
+  const int row0 = __builtin_amdgcn_readfirstlane((int)ty * TH + (t / kWave) * kRowsPerWave);
+  const int row1 = row0 + kRowsPerWave - 1;
+  const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
+  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+  const float t_min = rs.t_min;
+
+  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
+  bool done = !inside;
+  if (end > start && __ballot(!done) != 0) {
+    uint32_t j = end - 1;
+    uint32_t k = __builtin_amdgcn_readfirstlane(vals[j]);
+    float4 a = rec[3 * (size_t)k], bb = rec[3 * (size_t)k + 1], c = rec[3 * (size_t)k + 2];
+    for (;;) {
+      const bool more = j > start;
+      const uint32_t kn = more ? __builtin_amdgcn_readfirstlane(vals[j - 1]) : k;
+      const float4 na = rec[3 * (size_t)kn], nb = rec[3 * (size_t)kn + 1], nc = rec[3 * (size_t)kn + 2];
+      const uint32_t bx = __float_as_uint(bb.w), by = __float_as_uint(c.w);
+      const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
+      const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
+      if (!(by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1)) {
+        const float dx = fx - a.x, dy = fy - a.y;
+        const float ux = fmaf(a.z, dx, a.w * dy);
+        const float uy = fmaf(bb.x, dx, bb.y * dy);
+        const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && !done;
+        const float q = fmaf(ux, ux, uy * uy);
+        const float alpha = in ? bb.z * __expf(-0.5f * q) : 0.f;
+        const float ta = T * alpha;
+        cr = fmaf(c.x, ta, cr);
+        cg = fmaf(c.y, ta, cg);
+        cb = fmaf(c.z, ta, cb);
+        aa = fmaf(alpha, ta, aa);
+        T = T - ta;
+        done = done || T < t_min;
+        if (__ballot(!done) == 0) break;
+      }
+      if (!more) break;
+      --j;
+      a = na;
+      bb = nb;
+      c = nc;
+    }
+  }
+  if (inside) out[(size_t)py * rs.width + px] = make_float4(cr, cg, cb, aa + T);
+}
+
 __global__ void gather_u32_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx,
                                   const uint32_t* __restrict__ idx2, const uint32_t* __restrict__ d_count,
                                   uint32_t max_n, uint32_t* __restrict__ out) {
@@ -232,6 +308,8 @@ bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config
                         (tw == 32 && th == 4) || (tw == 64 && th == 4);
   if (!ok_shape || width == 0 || height == 0 || width > 32768 || height > 32768) return false;
   if (!(cfg->t_min >= 0.f && cfg->t_min < 1.f)) return false;
+  if (cfg->variant > 1) return false;
+  rs->variant = cfg->variant;
   rs->width = width;
   rs->height = height;
   rs->tile_w = tw;
@@ -251,12 +329,19 @@ hipError_t launch_bin_count(const float4* inst, const uint32_t* order, const uin
   return hipGetLastError();
 }
 
-hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* d_count,
-                           uint32_t max_n, const RasterShape& rs, uint64_t max_pairs, uint32_t* keys, uint32_t* vals,
-                           uint32_t* status, hipStream_t s) {
+hipError_t launch_bin_gather(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
+                             TileRect* rect, uint32_t* cnt, hipStream_t s) {
   if (max_n == 0) return hipSuccess;
-  bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rect, cnt, offs, d_count, max_n, rs.tiles_x, max_pairs,
-                                                              keys, vals, status);
+  bin_gather_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rc, order, d_count, max_n, rect, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* order,
+                           const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
+                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s) {
+  if (max_n == 0) return hipSuccess;
+  bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rect, cnt, offs, order, d_count, max_n, rs.tiles_x,
+                                                              max_pairs, keys, vals, status);
   return hipGetLastError();
 }
 
@@ -272,10 +357,13 @@ hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint3
                             float4* out, hipStream_t s) {
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tw = rs.tile_w, th = rs.tile_h;
-#define VKGS_RASTER_CASE(W_, H_)                                                     \
-  if (tw == W_ && th == H_) {                                                        \
-    rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out);  \
-    return hipGetLastError();                                                        \
+#define VKGS_RASTER_CASE(W_, H_)                                                          \
+  if (tw == W_ && th == H_) {                                                             \
+    if (rs.variant == 1)                                                                  \
+      rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out);     \
+    else                                                                                  \
+      rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out); \
+    return hipGetLastError();                                                             \
   }
   VKGS_RASTER_CASE(16, 16)
   VKGS_RASTER_CASE(8, 8)
