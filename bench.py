@@ -43,7 +43,21 @@ def parse_args(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="timed fwd+bwd steps of the batch workload (0 = skip)")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scales every scene size of the batch workload")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (a known-size kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
     return ap.parse_args(argv)
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(HERE, "profiles", f"traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def batch_workload(scale: float = 1.0, seed: int = 5):
@@ -186,6 +200,17 @@ def main(argv=None) -> int:
     r.set_timing(False)
     stage_ms = {k: round(median(v), 5) for k, v in stages.items()}
     log("stage ms", stage_ms)
+    r_variant = 0
+
+    if args.calibrate_copy > 0:  # known-size kernel for scaling rocprofv3 --pmc byte counters
+        nbytes = args.calibrate_copy << 20
+        src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {nbytes} bytes done")
+        del src, dst
 
     # ---- algorithmic work of the frame ----
     ranges, _ = r.tile_lists()
@@ -209,17 +234,22 @@ def main(argv=None) -> int:
             + stage_ms["tile_sort"]) * 1e-3
     ps_gbs = (b_proj + b_sort + b_bin) / t_ps / 1e9
 
+    raster_name = "rasterize_kernel" if r_variant == 0 else "rasterize_wave_kernel"
     if t_raster >= t_proj:
-        roof = {"kernel": "rasterize_kernel", "bound": "valu-fp32", "achieved": round(raster_tflops, 3),
+        # vector fp32 work (no stage is a dense contraction, MFMA is unused)
+        roof = {"kernel": raster_name, "bound": "valu", "achieved": round(raster_tflops, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(raster_tflops / FP32_PEAK_TFLOPS, 4),
-                "traffic": None, "avg_ms": stage_ms["raster"],
+                "traffic": load_traffic(raster_name, wl.name), "avg_ms": stage_ms["raster"],
                 "work": f"{RASTER_FLOPS_PER_EVAL} flop x {p_eval:.4g} (pixel, splat) evaluations"}
     else:
         roof = {"kernel": "fused_write_kernel", "bound": "hbm", "achieved": round(proj_gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4),
+                "traffic": load_traffic("fused_write_kernel", wl.name),
                 "avg_ms": stage_ms["project"], "work": f"12N + 172V = {b_proj:.4g} B"}
     roof_proj = {"kernel": "fused_write_kernel", "bound": "hbm", "achieved": round(proj_gbs, 1),
-                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4)}
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4),
+                 "traffic": load_traffic("fused_write_kernel", wl.name), "avg_ms": stage_ms["project"],
+                 "work": f"12N + 172V = {b_proj:.4g} B"}
 
     result = {
         "metric": METRIC,

@@ -159,8 +159,8 @@ int vkgs_projection(const vkgs_camera* camera, const float model[16], const floa
 typedef struct vkgs_raster_config {
   uint32_t tile_w, tile_h; /* 8x8, 16x16, 32x8, 16x8, 8x16, 16x4, 32x4, 64x4 */
   float t_min;             /* stop a pixel once transmittance < t_min (0 = composite every splat) */
-  uint32_t variant;        /* 0: each wave fetches records with scalar loads; 1: workgroup stages
-                              batches of records in LDS.  Both give the same image. */
+  uint32_t variant;        /* 0: the workgroup stages batches of records in LDS; 1: each wave
+                              fetches records with scalar loads.  Both give the same image. */
   uint32_t reserved[4];
 } vkgs_raster_config;
 

@@ -17,8 +17,10 @@ struct DrawPlan {
   RasterShape rs{};
   SortPlan sort{};
   ScanPlan scan{};
+  SortPlan tile_sort{};
   size_t off_hdr = 0, off_rec = 0, off_rect = 0, off_cnt = 0, off_offs = 0, off_scan = 0, off_keys = 0,
-         off_vals = 0, off_sort = 0, off_ranges = 0, bytes = 0;
+         off_vals = 0, off_sort = 0, off_ranges = 0, off_tile_keys = 0, off_tile_order = 0, off_tile_sort = 0,
+         bytes = 0;
 };
 
 // Header words at the start of the draw storage.
@@ -30,6 +32,7 @@ enum : uint32_t {
   kHdrMagic = 4,
   kHdrOffVals = 5,    // byte offset of the tile-sorted instance list (2 words)
   kHdrOffRanges = 7,  // byte offset of the per-tile ranges (2 words)
+  kHdrTiles = 9,      // tile count (read by the tile-schedule sort)
   kHdrWords = 64
 };
 constexpr uint32_t kDrawMagic = 0x564b4753u;  // "VKGS"
@@ -62,7 +65,15 @@ bool draw_plan(uint32_t max_inst, uint32_t width, uint32_t height, uint64_t max_
   p->off_sort = off;
   off = align_up(off + p->sort.bytes, 256);
   p->off_ranges = off;
-  off = align_up(off + (size_t)p->rs.tiles_x * p->rs.tiles_y * 8, 256);
+  const uint32_t tiles = p->rs.tiles_x * p->rs.tiles_y;
+  off = align_up(off + (size_t)tiles * 8, 256);
+  p->tile_sort = sort_plan(tiles);
+  p->off_tile_keys = off;
+  off = align_up(off + (size_t)tiles * 4, 256);
+  p->off_tile_order = off;
+  off = align_up(off + (size_t)tiles * 4, 256);
+  p->off_tile_sort = off;
+  off = align_up(off + p->tile_sort.bytes, 256);
   p->bytes = off;
   return true;
 }
@@ -74,6 +85,21 @@ int bits_for(uint32_t v) {  // bits needed for values in [0, v)
 }
 
 }  // namespace
+
+// Raster with the longest-list-first tile schedule when its buffers are given.
+static hipError_t raster_scheduled(const DrawBuffers& b, const RasterShape& rs, uint32_t tiles, float4* out,
+                                   hipStream_t s) {
+  if (b.tile_order) {
+    Words16 w = {};
+    w.w[0] = tiles;
+    hipError_t e = set_words_launch(b.tile_count, w, 1, s);
+    if (e != hipSuccess) return e;
+    e = launch_tile_order(b.ranges, tiles, b.tile_keys, b.tile_order, b.tile_count, b.tile_sort,
+                          b.tile_sort_storage, s);
+    if (e != hipSuccess) return e;
+  }
+  return launch_rasterize(b.rec, b.vals, b.ranges, rs, b.tile_order, out, s);
+}
 
 // Shared by vkgs_draw and the renderer: bin, sort by tile, ranges, raster.
 hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
@@ -93,7 +119,7 @@ hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d
     return e;
   if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
   if (marks) (void)hipEventRecord(marks[1], s);
-  return launch_rasterize(b.rec, b.vals, b.ranges, rs, out, s);
+  return raster_scheduled(b, rs, tiles, out, s);
 }
 
 // Fused-path draw: raster records and (rect, count) words were written per
@@ -116,7 +142,7 @@ hipError_t run_draw_slots(const uint4* rc, const uint32_t* order, const uint32_t
     return e;
   if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
   if (marks) (void)hipEventRecord(marks[1], s);
-  return launch_rasterize(b.rec, b.vals, b.ranges, rs, out, s);
+  return raster_scheduled(b, rs, tiles, out, s);
 }
 
 }  // namespace vkgs
@@ -224,6 +250,11 @@ int vkgs_draw(const float* d_instances, const uint32_t* d_draw_indirect, uint32_
   db.ranges = reinterpret_cast<uint32_t*>(b + p.off_ranges);
   db.status = hdr + kHdrStatus;
   db.pairs = hdr + kHdrPairs;
+  db.tile_keys = reinterpret_cast<uint32_t*>(b + p.off_tile_keys);
+  db.tile_order = reinterpret_cast<uint32_t*>(b + p.off_tile_order);
+  db.tile_count = hdr + kHdrTiles;
+  db.tile_sort = p.tile_sort;
+  db.tile_sort_storage = b + p.off_tile_sort;
   VKGS_HIP_CHECK(run_draw(reinterpret_cast<const float4*>(d_instances), nullptr, d_draw_indirect + 8, max_instances,
                           p.rs, p.max_pairs, db, reinterpret_cast<float4*>(d_rgba), s, nullptr));
   return VKGS_OK;

@@ -19,7 +19,7 @@ using namespace vkgs;
 
 namespace {
 
-enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscWords = 64 };
+enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscTiles = 3, kMiscWords = 64 };
 constexpr int kEvents = 7;
 constexpr int kStages = 8;
 
@@ -58,7 +58,8 @@ struct vkgs_renderer {
   DevBuf fc, misc, counts, rc, key, val, ids, sort_storage;
   // binning sized by (n, pairs, tiles)
   uint64_t max_pairs = 0;
-  DevBuf rec, rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges;
+  DevBuf rec, rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges, tile_keys, tile_order, tile_sort;
+  SortPlan tile_plan{};
   DevBuf sorted_ids, list_ids;
   SortPlan depth_plan{}, pair_plan{};
   ScanPlan scan{};
@@ -126,6 +127,10 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   if ((e = r->pvals.ensure((size_t)r->max_pairs * 4)) != hipSuccess) return e;
   if ((e = r->psort.ensure(r->pair_plan.bytes)) != hipSuccess) return e;
   if ((e = r->ranges.ensure((size_t)tiles * 8)) != hipSuccess) return e;
+  r->tile_plan = sort_plan(tiles);
+  if ((e = r->tile_keys.ensure((size_t)tiles * 4)) != hipSuccess) return e;
+  if ((e = r->tile_order.ensure((size_t)tiles * 4)) != hipSuccess) return e;
+  if ((e = r->tile_sort.ensure(r->tile_plan.bytes)) != hipSuccess) return e;
   r->frame_n = n;
   return hipSuccess;
 }
@@ -146,6 +151,11 @@ DrawBuffers draw_buffers(vkgs_renderer* r) {
   b.ranges = r->ranges.as<uint32_t>();
   b.status = misc + kMiscStatus;
   b.pairs = misc + kMiscPairs;
+  b.tile_keys = r->tile_keys.as<uint32_t>();
+  b.tile_order = r->tile_order.as<uint32_t>();
+  b.tile_count = misc + kMiscTiles;
+  b.tile_sort = r->tile_plan;
+  b.tile_sort_storage = r->tile_sort.p;
   return b;
 }
 
@@ -212,7 +222,7 @@ int vkgs_renderer_destroy(vkgs_renderer* r) {
   drop_graph(r);
   DevBuf* bufs[] = {&r->position, &r->cov3d, &r->opacity, &r->sh,   &r->fc,   &r->misc,  &r->counts,
                     &r->rc,       &r->key,   &r->val,     &r->ids,  &r->sort_storage, &r->rec, &r->rect, &r->cnt,
-                    &r->offs,     &r->scan_storage, &r->pkeys, &r->pvals, &r->psort, &r->ranges, &r->sorted_ids,
+                    &r->offs,     &r->scan_storage, &r->pkeys, &r->pvals, &r->psort, &r->ranges, &r->tile_keys, &r->tile_order, &r->tile_sort, &r->sorted_ids,
                     &r->list_ids};
   for (DevBuf* b : bufs) b->release();
   if (r->capture_stream) hipStreamDestroy(r->capture_stream);

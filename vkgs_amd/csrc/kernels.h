@@ -53,7 +53,7 @@ struct RasterShape {
   uint32_t width, height, tile_w, tile_h, tiles_x, tiles_y;
   uint32_t tile_shift_x, tile_shift_y;
   float t_min;
-  uint32_t variant;  // 0: scalar-load waves, 1: LDS batches
+  uint32_t variant;  // 0: LDS batches, 1: scalar-load waves
 };
 
 bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config* cfg, RasterShape* rs);
@@ -73,8 +73,13 @@ hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint
                            uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s);
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s);
+// Tiles ordered longest list first (stable by tile id among equal lengths);
+// *d_tiles holds the tile count for the sort.
+hipError_t launch_tile_order(const uint32_t* ranges, uint32_t tiles, uint32_t* keys, uint32_t* order,
+                             const uint32_t* d_tiles, const SortPlan& plan, void* sort_storage, hipStream_t s);
+// tile_order (optional): workgroup i rasterizes tile tile_order[i].
 hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
-                            float4* out, hipStream_t s);
+                            const uint32_t* tile_order, float4* out, hipStream_t s);
 
 // Device buffers of one draw (bin, tile sort, ranges, raster).
 struct DrawBuffers {
@@ -91,6 +96,12 @@ struct DrawBuffers {
   uint32_t* ranges;  // [start, end) per tile
   uint32_t* status;  // overflow flag
   uint32_t* pairs;   // pair count
+  // longest-first tile schedule (optional: tile_order == nullptr disables it)
+  uint32_t* tile_keys;
+  uint32_t* tile_order;
+  uint32_t* tile_count;  // device word = number of tiles
+  SortPlan tile_sort;
+  void* tile_sort_storage;
 };
 
 // Bins instances order[k] (k < *d_count), sorts the pairs by tile, marks the

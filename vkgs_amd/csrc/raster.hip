@@ -121,16 +121,29 @@ __global__ void ranges_mark_kernel(const uint32_t* __restrict__ keys, const uint
   }
 }
 
+// Sort key of each tile for the longest-first schedule: 0xffff - list length
+// (clamped), value = tile id; an ascending stable sort puts long lists first.
+__global__ void tile_order_keys_kernel(const uint32_t* __restrict__ ranges, uint32_t tiles,
+                                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tiles) return;
+  const uint32_t len = ranges[2 * t + 1] - ranges[2 * t];
+  keys[t] = 0xffffu - (len < 0xffffu ? len : 0xffffu);
+  vals[t] = t;
+}
+
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restrict__ rec,
                                                            const uint32_t* __restrict__ vals,
                                                            const uint32_t* __restrict__ ranges, RasterShape rs,
+                                                           const uint32_t* __restrict__ tile_order,
                                                            float4* __restrict__ out) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
+  constexpr bool kOneWave = kN == kWave;  // the wave covers the tile: every binned record touches it
   __shared__ float4 s_rec[2][3][kN];
 
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
   const uint32_t tx = tile % rs.tiles_x, ty = tile / rs.tiles_x;
   const int t = threadIdx.x;
   const int lx = t % TW, ly = t / TW;
@@ -181,11 +194,15 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
         // LDS reads of the next record overlap this record's arithmetic
         const int en = e + 1 < cnt ? e + 1 : e;
         const float4 na = s0[en], nb = s1[en], nc = s2[en];
-        const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
-        const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
-        const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
-        const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
-        if (!(by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1)) {
+        bool touches = true;
+        if (!kOneWave) {
+          const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
+          const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
+          const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
+          const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
+          touches = !(by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1);
+        }
+        if (touches) {
           const float dx = fx - a.x, dy = fy - a.y;
           const float ux = fmaf(a.z, dx, a.w * dy);
           const float uy = fmaf(bb.x, dx, bb.y * dy);
@@ -353,17 +370,26 @@ hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t
   return hipGetLastError();
 }
 
+hipError_t launch_tile_order(const uint32_t* ranges, uint32_t tiles, uint32_t* keys, uint32_t* order,
+                              const uint32_t* d_tiles, const SortPlan& plan, void* sort_storage, hipStream_t s) {
+  if (tiles == 0) return hipSuccess;
+  tile_order_keys_kernel<<<ceil_div(tiles, 256), 256, 0, s>>>(ranges, tiles, keys, order);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return sort_kv_launch(plan, d_tiles, keys, order, 16u, sort_storage, s);
+}
+
 hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
-                            float4* out, hipStream_t s) {
+                            const uint32_t* tile_order, float4* out, hipStream_t s) {
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tw = rs.tile_w, th = rs.tile_h;
-#define VKGS_RASTER_CASE(W_, H_)                                                          \
-  if (tw == W_ && th == H_) {                                                             \
-    if (rs.variant == 1)                                                                  \
-      rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out);     \
-    else                                                                                  \
-      rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out); \
-    return hipGetLastError();                                                             \
+#define VKGS_RASTER_CASE(W_, H_)                                                                   \
+  if (tw == W_ && th == H_) {                                                                      \
+    if (rs.variant == 0)                                                                           \
+      rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, tile_order, out);  \
+    else                                                                                           \
+      rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out);         \
+    return hipGetLastError();                                                                      \
   }
   VKGS_RASTER_CASE(16, 16)
   VKGS_RASTER_CASE(8, 8)
