@@ -43,6 +43,8 @@ def parse_args(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="timed fwd+bwd steps of the batch workload (0 = skip)")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scales every scene size of the batch workload")
+    ap.add_argument("--fwd-bwd-mode", choices=("both", "plain", "batched"), default="both",
+                    help="which fwd+bwd forms to time (one form alone is for profiling it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a known-size kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
@@ -89,7 +91,7 @@ def batch_workload(scale: float = 1.0, seed: int = 5):
     return scenes, views
 
 
-def bench_fwd_bwd(steps: int, scale: float):
+def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
     """views/s of fwd+bwd (L1 loss) for the batch workload: one launch set per view
     (plain) and one launch set for all views (batched)."""
     import torch
@@ -110,6 +112,8 @@ def bench_fwd_bwd(steps: int, scale: float):
 
     out = {}
     for name, fn in (("plain", plain_step), ("batched", batched_step)):
+        if mode not in ("both", name):
+            continue
         fn()  # warm-up: sizes every buffer
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -119,7 +123,8 @@ def bench_fwd_bwd(steps: int, scale: float):
         dt = (time.perf_counter() - t0) / steps
         out[f"{name}_views_per_s"] = round(len(views) / dt, 2)
         out[f"{name}_ms_per_view"] = round(dt * 1e3 / len(views), 3)
-    out["batched_over_plain"] = round(out["batched_views_per_s"] / out["plain_views_per_s"], 3)
+    if mode == "both":
+        out["batched_over_plain"] = round(out["batched_views_per_s"] / out["plain_views_per_s"], 3)
     out["views"] = len(views)
     out["splats"] = [s.n for s in scenes]
     return out
@@ -286,7 +291,7 @@ def main(argv=None) -> int:
     # ---- fwd+bwd views/s: plain (one view per launch set) vs batched ----
     if args.fwd_bwd_steps > 0:
         try:
-            result["fwd_bwd"] = bench_fwd_bwd(args.fwd_bwd_steps, args.batch_scale)
+            result["fwd_bwd"] = bench_fwd_bwd(args.fwd_bwd_steps, args.batch_scale, args.fwd_bwd_mode)
             log("fwd+bwd", result["fwd_bwd"])
         except Exception as e:  # reported, never hides the frame figure
             result["fwd_bwd"] = {"error": repr(e)}

@@ -244,13 +244,14 @@ int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[8]);
  * which re-walks each tile list back to front from the stored per-pixel
  * transmittance and processed-entry count.
  * ------------------------------------------------------------------------- */
+/* sh and grad_sh must be 16-byte aligned (VKGS_ERR_INVALID_ARG otherwise). */
 typedef struct vkgs_diff_scene {
   const float* means;
   const float* log_scales;
   const float* quats;
   const float* opacity_logits;
   const float* sh;
-  float* grad_means; /* gradients are accumulated (+=) by vkgs_diff_backward */
+  float* grad_means; /* every gradient element is written (not accumulated) by vkgs_diff_backward */
   float* grad_log_scales;
   float* grad_quats;
   float* grad_opacity_logits;

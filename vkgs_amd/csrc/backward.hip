@@ -2,16 +2,19 @@
 //
 // Forward: per (view, splat) item the activation of parse_ply.comp:106-156
 // (fp32 SH, no fp16 rounding) and the projection of projection.comp:80-179,
-// ordered compaction, depth sort, binning into global tiles, tile sort, and a
-// raster that also stores, per pixel, the transmittance after the last
-// processed splat and how many list entries it processed.
+// ordered compaction with the raster record written at the item's slot, depth
+// sort, binning into global tiles, tile sort, and a raster that also stores,
+// per pixel, the transmittance after the last processed splat and how many
+// list entries it processed.
 //
 // Backward: the raster backward re-walks each tile list back to front from
 // that stored state, recomputing alpha per entry and the transmittance before
-// it as T_i = T_{i+1} / (1 - alpha_i); per entry it accumulates 9 gradient
-// words (colour 3, opacity 1, conic 3, centre 2) in LDS with ds_add_f32 and
-// flushes them with one global float atomic per word per tile.  The
-// projection backward then chains them to the raw parameters of each splat.
+// it as T_i = T_{i+1} / (1 - alpha_i); per entry the 9 gradient words (colour
+// 3, opacity 1, conic 3, centre 2) are summed over the wave's pixels with DPP
+// and flushed with one global float atomic per word per tile into the slot's
+// words.  The projection backward then runs one lane per scene splat, chains
+// the words of every view that saw the splat to its raw parameters and writes
+// each gradient once.
 //
 // alpha is clamped to kAlphaMax so the division above stays bounded; this is
 // the only departure of this path from the reference's blend.
@@ -19,6 +22,7 @@
 
 #include "common.h"
 #include "diff.h"
+#include "emit.h"
 #include "kernels.h"
 
 namespace vkgs {
@@ -172,13 +176,26 @@ __device__ void diff_project(const FrameConst& fc, const DiffScene& sc, uint32_t
   out[2] = make_float4(color[0], color[1], color[2], opacity);
 }
 
+// Largest i < n with bases[i] <= x (bases ascending, x < bases[n]); empty
+// ranges (equal bases) resolve to the last of them, the one holding x.
+__device__ __forceinline__ uint32_t range_of(const uint32_t* __restrict__ bases, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (bases[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 // --------------------------------------------------------------- forward ---
 
 __global__ __launch_bounds__(kCompactBlock) void diff_count_kernel(DiffLaunch L, uint32_t* __restrict__ counts) {
-  __shared__ uint32_t s_w[kCompactBlock / kWave];
+  __shared__ uint32_t s_w[kMaskWords];
   const uint32_t b = blockIdx.x;
-  const uint32_t v = L.block_view[b];
-  const DiffView& view = L.views[v];
+  const DiffView& view = L.views[range_of(L.view_blocks, L.n_views, b)];
   const DiffScene& sc = L.scenes[view.scene];
   const uint32_t id = (b - view.block_base) * kCompactBlock + threadIdx.x;
   bool vis = false;
@@ -192,19 +209,21 @@ __global__ __launch_bounds__(kCompactBlock) void diff_count_kernel(DiffLaunch L,
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t s = 0;
-    for (uint32_t i = 0; i < kCompactBlock / kWave; ++i) s += s_w[i];
+    for (uint32_t i = 0; i < kMaskWords; ++i) s += s_w[i];
     counts[b] = s;
   }
 }
 
+// Projection of the visible items of one block straight into raster records
+// at their slots (as the plain fused path does).
 __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L, const uint32_t* __restrict__ offsets,
-                                                                   float4* __restrict__ inst, uint32_t* __restrict__ key,
+                                                                   float4* __restrict__ rec, uint4* __restrict__ rc,
+                                                                   uint32_t* __restrict__ key,
                                                                    uint32_t* __restrict__ val,
-                                                                   uint32_t* __restrict__ ids,
-                                                                   uint32_t* __restrict__ vid) {
-  __shared__ uint32_t s_w[kCompactBlock / kWave];
+                                                                   uint64_t* __restrict__ masks) {
+  __shared__ uint32_t s_w[kMaskWords];
   const uint32_t b = blockIdx.x;
-  const uint32_t v = L.block_view[b];
+  const uint32_t v = range_of(L.view_blocks, L.n_views, b);
   const DiffView& view = L.views[v];
   const DiffScene& sc = L.scenes[view.scene];
   const uint32_t id = (b - view.block_base) * kCompactBlock + threadIdx.x;
@@ -215,7 +234,10 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
                        sc.means[3 * (size_t)id + 2], &omz);
   const uint64_t m = __ballot(vis);
   const uint32_t w = threadIdx.x / kWave;
-  if (__lane_id() == 0) s_w[w] = (uint32_t)__popcll(m);
+  if (__lane_id() == 0) {
+    s_w[w] = (uint32_t)__popcll(m);
+    masks[(size_t)b * kMaskWords + w] = m;
+  }
   __syncthreads();
   uint32_t slot = offsets[b];
   for (uint32_t i = 0; i < w; ++i) slot += s_w[i];
@@ -223,51 +245,41 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
   if (vis) {
     float4 o[3];
     diff_project(view.fc, sc, id, o);
-    inst[3 * (size_t)slot + 0] = o[0];
-    inst[3 * (size_t)slot + 1] = o[1];
-    inst[3 * (size_t)slot + 2] = o[2];
+    const float f[12] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w, o[2].x, o[2].y, o[2].z, o[2].w};
+    Splat2D sp = {};
+    uint4 r = make_uint4(0u, 0u, 0u, v);
+    if (setup_splat(f, view.fc.width, view.fc.height, &sp)) {
+      const uint32_t tx0 = (uint32_t)sp.x0 >> L.tile_shift_x, tx1 = (uint32_t)sp.x1 >> L.tile_shift_x;
+      const uint32_t ty0 = (uint32_t)sp.y0 >> L.tile_shift_y, ty1 = (uint32_t)sp.y1 >> L.tile_shift_y;
+      r.x = tx0 | (tx1 << 16);
+      r.y = ty0 | (ty1 << 16);
+      r.z = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    } else {  // empty pixel box: every raster tile test rejects it
+      sp.x0 = 1;
+      sp.x1 = 0;
+      sp.y0 = 1;
+      sp.y1 = 0;
+    }
+    rec[3 * (size_t)slot + 0] = make_float4(sp.cx, sp.cy, sp.i00, sp.i01);
+    rec[3 * (size_t)slot + 1] = make_float4(sp.i10, sp.i11, sp.o, __uint_as_float(pack16(sp.x0, sp.x1)));
+    rec[3 * (size_t)slot + 2] = make_float4(sp.r, sp.g, sp.b, __uint_as_float(pack16(sp.y0, sp.y1)));
+    rc[slot] = r;
     key[slot] = __float_as_uint(omz);
     val[slot] = slot;
-    ids[slot] = id;
-    vid[slot] = v;
   }
 }
 
-__global__ __launch_bounds__(256) void diff_bin_count_kernel(DiffLaunch L, const float4* __restrict__ inst,
-                                                             const uint32_t* __restrict__ order,
-                                                             const uint32_t* __restrict__ vid,
-                                                             const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                             float4* __restrict__ rec, TileRect* __restrict__ rect,
-                                                             uint32_t* __restrict__ cnt,
-                                                             uint32_t* __restrict__ kview) {
+__global__ __launch_bounds__(256) void diff_bin_gather_kernel(const uint4* __restrict__ rc,
+                                                              const uint32_t* __restrict__ order,
+                                                              const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                              TileRect* __restrict__ rect, uint32_t* __restrict__ cnt,
+                                                              uint32_t* __restrict__ kview) {
   const uint32_t n = min(*d_count, max_n);
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const uint32_t slot = order[k];
-    const uint32_t v = vid[slot];
-    const FrameConst& fc = L.views[v].fc;
-    const float4 a = inst[3 * (size_t)slot], b = inst[3 * (size_t)slot + 1], c = inst[3 * (size_t)slot + 2];
-    const float f[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
-    Splat2D s;
-    uint32_t count = 0;
-    TileRect r = {0u, 0u};
-    if (setup_splat(f, fc.width, fc.height, &s)) {
-      const uint32_t tx0 = (uint32_t)s.x0 >> L.tile_shift_x, tx1 = (uint32_t)s.x1 >> L.tile_shift_x;
-      const uint32_t ty0 = (uint32_t)s.y0 >> L.tile_shift_y, ty1 = (uint32_t)s.y1 >> L.tile_shift_y;
-      r.x = tx0 | (tx1 << 16);
-      r.y = ty0 | (ty1 << 16);
-      count = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-    } else {
-      s.x0 = 1;
-      s.x1 = 0;
-      s.y0 = 1;
-      s.y1 = 0;
-    }
-    rec[3 * (size_t)k + 0] = make_float4(s.cx, s.cy, s.i00, s.i01);
-    rec[3 * (size_t)k + 1] = make_float4(s.i10, s.i11, s.o, __uint_as_float(pack16(s.x0, s.x1)));
-    rec[3 * (size_t)k + 2] = make_float4(s.r, s.g, s.b, __uint_as_float(pack16(s.y0, s.y1)));
-    rect[k] = r;
-    cnt[k] = count;
-    kview[k] = v;
+    const uint4 u = rc[order[k]];
+    rect[k] = TileRect{u.x, u.y};
+    cnt[k] = u.z;
+    kview[k] = u.w;
   }
 }
 
@@ -275,27 +287,33 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
                                                             const uint32_t* __restrict__ cnt,
                                                             const uint32_t* __restrict__ offs,
                                                             const uint32_t* __restrict__ kview,
+                                                            const uint32_t* __restrict__ order,
                                                             const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                             uint64_t max_pairs, uint32_t* __restrict__ keys,
                                                             uint32_t* __restrict__ vals,
                                                             uint32_t* __restrict__ status) {
   const uint32_t n = min(*d_count, max_n);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const uint32_t c = cnt[k];
-    if (c == 0) continue;
-    const uint64_t o = offs[k];
-    const TileRect r = rect[k];
-    const DiffView& view = L.views[kview[k]];
-    if (o + c > max_pairs) atomicOr(status, 1u);
-    const uint32_t tx0 = r.x & 0xffffu, tx1 = r.x >> 16, ty0 = r.y & 0xffffu, ty1 = r.y >> 16;
-    uint64_t j = o;
-    for (uint32_t ty = ty0; ty <= ty1; ++ty)
-      for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j) {
-        if (j < max_pairs) {
-          keys[j] = view.tile_base + ty * view.fc.tiles_x + tx;
-          vals[j] = k;
-        }
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // wave-uniform loop: emit_pairs_wave needs every lane
+  for (uint32_t base = wave * kWave; base < n; base += stride) {
+    const uint32_t k = base + __lane_id();
+    uint32_t c = 0, slot = 0, key_base = 0, tiles_x = 0;
+    uint64_t o = 0;
+    TileRect r = {0u, 0u};
+    if (k < n) {
+      c = cnt[k];
+      if (c != 0) {
+        o = offs[k];
+        r = rect[k];
+        slot = order[k];
+        const DiffView& view = L.views[kview[k]];
+        key_base = view.tile_base;
+        tiles_x = view.fc.tiles_x;
+        if (o + c > max_pairs) atomicOr(status, 1u);
       }
+    }
+    emit_pairs_wave(c, o, r, slot, key_base, tiles_x, max_pairs, keys, vals);
   }
 }
 
@@ -304,13 +322,14 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                              const uint32_t* __restrict__ vals,
-                                                             const uint32_t* __restrict__ ranges, int keep_state) {
+                                                             const uint32_t* __restrict__ ranges,
+                                                             const uint32_t* __restrict__ tile_order, int keep_state) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
   __shared__ float4 s_rec[2][3][kN];
 
-  const uint32_t gtile = blockIdx.x;
-  const DiffView& view = L.views[L.tile_view[gtile]];
+  const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
+  const DiffView& view = L.views[range_of(L.view_tiles, L.n_views, gtile)];
   const uint32_t tile = gtile - view.tile_base;
   const uint32_t W = view.fc.width, H = view.fc.height;
   const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
@@ -392,20 +411,56 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
 
 // -------------------------------------------------------------- backward ---
 
+__global__ __launch_bounds__(256) void diff_zero_kernel(const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                        float* __restrict__ g) {
+  const size_t n = 9 * (size_t)min(*d_count, max_n);
+  const size_t n4 = n / 4;
+  float4* g4 = reinterpret_cast<float4*>(g);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += stride)
+    g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const size_t i = 4 * n4 + blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) g[i] = 0.f;
+}
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_add(float x) {
+  // lanes of rows outside kRowMask read 0 (the `old` operand)
+  return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, kRowMask, 0xf, false));
+}
+
+// Sum of x over the 64 lanes of a wave; every lane must be active.  The sum
+// lands in lane 63 (other lanes hold partial sums).
+__device__ __forceinline__ float wave_sum_last(float x) {
+  x = dpp_add<0xb1, 0xf>(x);   // quad_perm [1,0,3,2]: pairs
+  x = dpp_add<0x4e, 0xf>(x);   // quad_perm [2,3,0,1]: quads
+  x = dpp_add<0x124, 0xf>(x);  // row_ror:4
+  x = dpp_add<0x128, 0xf>(x);  // row_ror:8: every lane holds its row of 16
+  x = dpp_add<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  x = dpp_add<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3: lane 63 holds the total
+  return x;
+}
+
+// Re-walks one tile back to front from the stored per-pixel state.  Per list
+// entry the 9 gradient words are summed over the wave's pixels with DPP (no
+// LDS atomics on a shared address), kept in LDS per batch and flushed with
+// one global float atomic per nonzero word.
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                                       const uint32_t* __restrict__ vals,
                                                                       const uint32_t* __restrict__ ranges,
+                                                                      const uint32_t* __restrict__ tile_order,
                                                                       float* __restrict__ g9) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
+  constexpr bool kOneWave = kN == kWave;
   __shared__ float4 s_rec[3][kN];
   __shared__ float s_g[9][kN];
   __shared__ uint32_t s_k[kN];
   __shared__ uint32_t s_max;
 
-  const uint32_t gtile = blockIdx.x;
-  const DiffView& view = L.views[L.tile_view[gtile]];
+  const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
+  const DiffView& view = L.views[range_of(L.view_tiles, L.n_views, gtile)];
   const uint32_t tile = gtile - view.tile_base;
   const uint32_t W = view.fc.width, H = view.fc.height;
   const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
@@ -417,6 +472,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   const int row1 = row0 + kRowsPerWave - 1;
   const uint32_t end = ranges[2 * gtile + 1];
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+  const bool last_lane = __lane_id() == kWave - 1;
 
   uint32_t last = 0;
   float T = 1.f;
@@ -464,15 +520,18 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const float ux = fmaf(a.z, dx, a.w * dy);
       const float uy = fmaf(bb.x, dx, bb.y * dy);
       const bool in = i < last && fabsf(ux) <= 3.f && fabsf(uy) <= 3.f;
-      if (!in) continue;
+      if (__ballot(in) == 0) continue;
+      // every lane takes part from here (DPP reads all lanes); lanes with
+      // !in contribute zeros and keep their state
       const float q = fmaf(ux, ux, uy * uy);
       const float G = __expf(-0.5f * q);
       const float raw = bb.z * G;
-      const float alpha = fminf(raw, kAlphaMax);
+      const float alpha = in ? fminf(raw, kAlphaMax) : 0.f;
       const float om = 1.f - alpha;
-      T = T / om;  // transmittance in front of entry i
+      T = in ? T * __builtin_amdgcn_rcpf(om) : T;  // transmittance in front of entry i
       const float ta = T * alpha;
-      const float dalpha = T * (g.x * (c.x - rc) + g.y * (c.y - rg) + g.z * (c.z - rb) + g.w * (2.f * alpha - ra));
+      const float dalpha =
+          in ? T * (g.x * (c.x - rc) + g.y * (c.y - rg) + g.z * (c.z - rb) + g.w * (2.f * alpha - ra)) : 0.f;
       rc = alpha * c.x + om * rc;
       rg = alpha * c.y + om * rg;
       rb = alpha * c.z + om * rb;
@@ -480,15 +539,38 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const bool clamped = raw > kAlphaMax;
       const float dop = clamped ? 0.f : dalpha * G;
       const float dq = clamped ? 0.f : -0.5f * dalpha * alpha;
-      atomicAdd(&s_g[0][e], g.x * ta);
-      atomicAdd(&s_g[1][e], g.y * ta);
-      atomicAdd(&s_g[2][e], g.z * ta);
-      atomicAdd(&s_g[3][e], dop);
-      atomicAdd(&s_g[4][e], dq * dx * dx);
-      atomicAdd(&s_g[5][e], dq * 2.f * dx * dy);
-      atomicAdd(&s_g[6][e], dq * dy * dy);
-      atomicAdd(&s_g[7][e], -2.f * dq * (a.z * ux + bb.x * uy));
-      atomicAdd(&s_g[8][e], -2.f * dq * (a.w * ux + bb.y * uy));
+      const float s0 = wave_sum_last(g.x * ta);
+      const float s1 = wave_sum_last(g.y * ta);
+      const float s2 = wave_sum_last(g.z * ta);
+      const float s3 = wave_sum_last(dop);
+      const float s4 = wave_sum_last(dq * dx * dx);
+      const float s5 = wave_sum_last(dq * 2.f * dx * dy);
+      const float s6 = wave_sum_last(dq * dy * dy);
+      const float s7 = wave_sum_last(-2.f * dq * (a.z * ux + bb.x * uy));
+      const float s8 = wave_sum_last(-2.f * dq * (a.w * ux + bb.y * uy));
+      if (last_lane) {
+        if (kOneWave) {
+          s_g[0][e] = s0;
+          s_g[1][e] = s1;
+          s_g[2][e] = s2;
+          s_g[3][e] = s3;
+          s_g[4][e] = s4;
+          s_g[5][e] = s5;
+          s_g[6][e] = s6;
+          s_g[7][e] = s7;
+          s_g[8][e] = s8;
+        } else {
+          atomicAdd(&s_g[0][e], s0);
+          atomicAdd(&s_g[1][e], s1);
+          atomicAdd(&s_g[2][e], s2);
+          atomicAdd(&s_g[3][e], s3);
+          atomicAdd(&s_g[4][e], s4);
+          atomicAdd(&s_g[5][e], s5);
+          atomicAdd(&s_g[6][e], s6);
+          atomicAdd(&s_g[7][e], s7);
+          atomicAdd(&s_g[8][e], s8);
+        }
+      }
     }
     __syncthreads();
     if (t < cnt) {
@@ -503,8 +585,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   }
 }
 
-// Chains the 9 per-entry gradient words of draw-order position k to the raw
-// parameters of its splat.  Row-major 3x3 helpers below.
+// Row-major 3x3 helpers.
 __device__ __forceinline__ void mm3(const float* A, const float* B, float* C) {
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
@@ -518,219 +599,270 @@ __device__ __forceinline__ void mm3_nt(const float* A, const float* B, float* C)
     for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[3 * c] + A[3 * r + 1] * B[3 * c + 1] + A[3 * r + 2] * B[3 * c + 2];
 }
 
-__global__ __launch_bounds__(256) void diff_project_backward_kernel(DiffLaunch L, const uint32_t* __restrict__ order,
-                                                                    const uint32_t* __restrict__ ids,
-                                                                    const uint32_t* __restrict__ vid,
-                                                                    const float* __restrict__ g9,
-                                                                    const uint32_t* __restrict__ d_count,
-                                                                    uint32_t max_n) {
-  const uint32_t n = min(*d_count, max_n);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const float* gk = g9 + 9 * (size_t)k;
-    float gv[9];
-    bool any = false;
-    for (int j = 0; j < 9; ++j) {
-      gv[j] = gk[j];
-      any = any || gv[j] != 0.f;
-    }
-    if (!any) continue;
-    const uint32_t slot = order[k];
-    const uint32_t id = ids[slot];
-    const DiffView& view = L.views[vid[slot]];
-    const DiffScene& sc = L.scenes[view.scene];
-    const FrameConst& fc = view.fc;
+// View-independent state of one splat for the projection backward.
+struct SplatState {
+  float m[3];    // mean
+  float q[4];    // normalized (x, y, z, w)
+  float qlen;    // |q| before normalization
+  float s[3];    // scales
+  float R[9];    // rotation, row-major
+  float Sig[9];  // R diag(s^2) R^T
+  float sh[48];  // channel-major SH
+};
 
-    // ---- recompute the forward (row-major) ----
-    float q[4];
-    const float qlen = load_quat(sc, id, q);
-    float R[9];
-    quat_rot(q, R);
-    float s[3];
-    for (int i = 0; i < 3; ++i) s[i] = expf(sc.log_scales[3 * (size_t)id + i]);
+// Chains the 9 gradient words gv of one view (colour 3, opacity 1, conic 3,
+// centre 2) through that view's projection and adds the result to the
+// view-summed dL/dSigma (GSig), dL/dmean (gm) and dL/dSH (gsh).
+__device__ __forceinline__ void chain_view(const FrameConst& fc, const SplatState& P, const float gv[9],
+                                           float GSig[9], float gm[3], float gsh[48]) {
+  // ---- recompute the forward (row-major) ----
+  float V3[9], M3[9], Wm[9];  // W = V3 * M3 from the column-major matrices
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      V3[3 * r + c] = fc.V[4 * c + r];
+      M3[3 * r + c] = fc.M[4 * c + r];
+    }
+  mm3(V3, M3, Wm);
+  float p4[4] = {P.m[0], P.m[1], P.m[2], 1.f}, t4[4], pv[4];
+  m4_mulv(fc.M, p4, t4);
+  m4_mulv(fc.V, t4, pv);
+  const float x = pv[0], y = pv[1], z = pv[2];
+  float WS[9], Sp[9];
+  mm3(Wm, P.Sig, WS);
+  mm3_nt(WS, Wm, Sp);
+  const float iz = 1.f / z, iz2 = iz * iz;
+  const float J2[6] = {-iz, 0.f, x * iz2, 0.f, -iz, y * iz2};
+  float JS[6];  // J2 Sp (2x3)
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c) JS[3 * r + c] = J2[3 * r] * Sp[c] + J2[3 * r + 1] * Sp[3 + c] + J2[3 * r + 2] * Sp[6 + c];
+  float S2[4];  // J2 Sp J2^T
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c)
+      S2[2 * r + c] = JS[3 * r] * J2[3 * c] + JS[3 * r + 1] * J2[3 * c + 1] + JS[3 * r + 2] * J2[3 * c + 2];
+  const float Pm[4] = {fc.P[0], fc.P[4], fc.P[1], fc.P[5]};  // mat2(projection), row-major
+  float PS[4], cov[4];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) PS[2 * r + c] = Pm[2 * r] * S2[c] + Pm[2 * r + 1] * S2[2 + c];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) cov[2 * r + c] = PS[2 * r] * Pm[c] + PS[2 * r + 1] * Pm[2 + c];
+  cov[0] += fc.inv_w2;
+  cov[3] += fc.inv_h2;
+  const float hw = 0.5f * (float)fc.width, hh = 0.5f * (float)fc.height;
+  const float cp00 = cov[0] * hw * hw, cp01 = cov[1] * hw * hh, cp10 = cov[2] * hh * hw, cp11 = cov[3] * hh * hh;
+  const float det = cp00 * cp11 - cp01 * cp10;
+  const float Q00 = cp11 / det, Q01 = -cp01 / det, Q10 = -cp10 / det, Q11 = cp00 / det;
+
+  // ---- conic -> cov2d (pixels) -> cov2d (NDC) -> S2 ----
+  const float GQ00 = gv[4], GQ01 = 0.5f * gv[5], GQ11 = gv[6];
+  // G_px = -Q GQ Q
+  const float A00 = Q00 * GQ00 + Q01 * GQ01, A01 = Q00 * GQ01 + Q01 * GQ11;
+  const float A10 = Q10 * GQ00 + Q11 * GQ01, A11 = Q10 * GQ01 + Q11 * GQ11;
+  const float Gp00 = -(A00 * Q00 + A01 * Q10), Gp01 = -(A00 * Q01 + A01 * Q11);
+  const float Gp10 = -(A10 * Q00 + A11 * Q10), Gp11 = -(A10 * Q01 + A11 * Q11);
+  const float Gn[4] = {Gp00 * hw * hw, Gp01 * hw * hh, Gp10 * hh * hw, Gp11 * hh * hh};
+  // cov = Pm S2 Pm  ->  dS2 = Pm^T Gn Pm^T
+  float T1[4], GS2[4];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) T1[2 * r + c] = Pm[r] * Gn[c] + Pm[2 + r] * Gn[2 + c];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) GS2[2 * r + c] = T1[2 * r] * Pm[c * 2] + T1[2 * r + 1] * Pm[c * 2 + 1];
+  // S2 = J2 Sp J2^T -> dSp = J2^T GS2 J2, dJ2 = (GS2 + GS2^T) J2 Sp
+  float GSp[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      float acc = 0.f;
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc += J2[3 * i + r] * GS2[2 * i + j] * J2[3 * j + c];
+      GSp[3 * r + c] = acc;
+    }
+  float GJ2[6];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c) {
+      float acc = 0.f;
+      for (int i = 0; i < 2; ++i) acc += (GS2[2 * r + i] + GS2[2 * i + r]) * JS[3 * i + c];
+      GJ2[3 * r + c] = acc;
+    }
+  // Sp = W Sig W^T -> dSig = W^T GSp W
+  float TT[9], GSv[9];
+  mm3_tn(Wm, GSp, TT);
+  mm3(TT, Wm, GSv);
+  for (int i = 0; i < 9; ++i) GSig[i] += GSv[i];
+
+  // ---- view-space position: J2 and the projected centre ----
+  float gpv[3];
+  gpv[0] = GJ2[2] * iz2;
+  gpv[1] = GJ2[5] * iz2;
+  gpv[2] = (GJ2[0] + GJ2[4]) * iz2 - 2.f * (x * GJ2[2] + y * GJ2[5]) * iz2 * iz;
+  {
+    float clip[4];
+    float pv4[4] = {x, y, z, 1.f};
+    m4_mulv(fc.P, pv4, clip);
+    const float iw = 1.f / clip[3];
+    const float ndx = clip[0] * iw, ndy = clip[1] * iw;
+    const float gx = gv[7] * hw, gy = gv[8] * hh;  // d/d ndc
+    for (int j = 0; j < 3; ++j) {
+      gpv[j] += gx * (fc.P[4 * j + 0] - ndx * fc.P[4 * j + 3]) * iw;
+      gpv[j] += gy * (fc.P[4 * j + 1] - ndy * fc.P[4 * j + 3]) * iw;
+    }
+  }
+  // mean <- W^T gpv
+  for (int j = 0; j < 3; ++j) gm[j] += Wm[j] * gpv[0] + Wm[3 + j] * gpv[1] + Wm[6 + j] * gpv[2];
+
+  // ---- colour: SH coefficients and view direction ----
+  const float d0 = P.m[0] - fc.eye_model[0], d1 = P.m[1] - fc.eye_model[1], d2 = P.m[2] - fc.eye_model[2];
+  const float dl = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+  const float dx = d0 / dl, dy = d1 / dl, dz = d2 / dl;
+  float basis[16];
+  sh_basis(dx, dy, dz, basis);
+  float gcol[3];
+  for (int ch = 0; ch < 3; ++ch) {
+    float acc = 0.f;
+    for (int j = 0; j < 16; ++j) acc += basis[j] * P.sh[ch * 16 + j];
+    gcol[ch] = (acc + 0.5f < 0.f) ? 0.f : gv[ch];
+  }
+  float gb[16];
+  for (int j = 0; j < 16; ++j) {
+    gb[j] = gcol[0] * P.sh[j] + gcol[1] * P.sh[16 + j] + gcol[2] * P.sh[32 + j];
+    for (int ch = 0; ch < 3; ++ch) gsh[ch * 16 + j] += gcol[ch] * basis[j];
+  }
+  {
+    const float C1 = 0.4886025119029199f, C20 = 1.0925484305920792f, C21 = 0.31539156525252005f,
+                C22 = 0.5462742152960396f, C30 = 0.5900435899266435f, C31 = 2.890611442640554f,
+                C32 = 0.4570457994644658f, C33 = 0.3731763325901154f, C34 = 1.445305721320277f;
+    const float X = dx, Y = dy, Z = dz, XX = X * X, YY = Y * Y, ZZ = Z * Z;
+    float gdx = 0.f, gdy = 0.f, gdz = 0.f;
+    gdy += gb[1] * -C1;
+    gdz += gb[2] * C1;
+    gdx += gb[3] * -C1;
+    gdx += gb[4] * C20 * Y;
+    gdy += gb[4] * C20 * X;
+    gdy += gb[5] * -C20 * Z;
+    gdz += gb[5] * -C20 * Y;
+    gdx += gb[6] * -2.f * C21 * X;
+    gdy += gb[6] * -2.f * C21 * Y;
+    gdz += gb[6] * 4.f * C21 * Z;
+    gdx += gb[7] * -C20 * Z;
+    gdz += gb[7] * -C20 * X;
+    gdx += gb[8] * 2.f * C22 * X;
+    gdy += gb[8] * -2.f * C22 * Y;
+    gdx += gb[9] * -6.f * C30 * X * Y;
+    gdy += gb[9] * -3.f * C30 * (XX - YY);
+    gdx += gb[10] * C31 * Y * Z;
+    gdy += gb[10] * C31 * X * Z;
+    gdz += gb[10] * C31 * X * Y;
+    gdx += gb[11] * 2.f * C32 * X * Y;
+    gdy += gb[11] * -C32 * (4.f * ZZ - XX - 3.f * YY);
+    gdz += gb[11] * -8.f * C32 * Y * Z;
+    gdx += gb[12] * -6.f * C33 * X * Z;
+    gdy += gb[12] * -6.f * C33 * Y * Z;
+    gdz += gb[12] * C33 * (6.f * ZZ - 3.f * XX - 3.f * YY);
+    gdx += gb[13] * -C32 * (4.f * ZZ - 3.f * XX - YY);
+    gdy += gb[13] * 2.f * C32 * X * Y;
+    gdz += gb[13] * -8.f * C32 * X * Z;
+    gdx += gb[14] * 2.f * C34 * X * Z;
+    gdy += gb[14] * -2.f * C34 * Y * Z;
+    gdz += gb[14] * C34 * (XX - YY);
+    gdx += gb[15] * -3.f * C30 * (XX - YY);
+    gdy += gb[15] * 6.f * C30 * X * Y;
+    const float dd = gdx * X + gdy * Y + gdz * Z;
+    gm[0] += (gdx - X * dd) / dl;
+    gm[1] += (gdy - Y * dd) / dl;
+    gm[2] += (gdz - Z * dd) / dl;
+  }
+}
+
+// One lane per scene splat: visits every view of the scene, reads the view's
+// 9 gradient words at the splat's slot (found from the forward's visibility
+// ballots and block offsets) and chains them; then writes all 59 parameter
+// gradients of the splat once (zeros for a splat no view touched).
+__global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(DiffLaunch L,
+                                                                              const uint32_t* __restrict__ offsets,
+                                                                              const uint64_t* __restrict__ masks,
+                                                                              const float* __restrict__ g9) {
+  const uint32_t sb = blockIdx.x;
+  const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
+  const uint32_t lb = sb - sc.block_base;
+  const uint32_t id = lb * kCompactBlock + threadIdx.x;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint32_t lane = __lane_id();
+  const bool valid = id < sc.n;
+
+  SplatState P;
+  float op = 0.5f;
+  if (valid) {
+    P.qlen = load_quat(sc, id, P.q);
+    for (int i = 0; i < 3; ++i) {
+      P.m[i] = sc.means[3 * (size_t)id + i];
+      P.s[i] = expf(sc.log_scales[3 * (size_t)id + i]);
+    }
+    const float4* sh4 = reinterpret_cast<const float4*>(sc.sh + 48 * (size_t)id);
+    for (int j = 0; j < 12; ++j) {
+      const float4 u = sh4[j];
+      P.sh[4 * j] = u.x;
+      P.sh[4 * j + 1] = u.y;
+      P.sh[4 * j + 2] = u.z;
+      P.sh[4 * j + 3] = u.w;
+    }
+    op = 1.f / (1.f + expf(-sc.opacity[id]));
+  } else {
+    for (int i = 0; i < 3; ++i) {
+      P.m[i] = 0.f;
+      P.s[i] = 1.f;
+    }
+    P.q[0] = P.q[1] = P.q[2] = 0.f;
+    P.q[3] = 1.f;
+    P.qlen = 1.f;
+    for (int j = 0; j < 48; ++j) P.sh[j] = 0.f;
+  }
+  quat_rot(P.q, P.R);
+  {
     float RS2[9];
     for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) RS2[3 * r + c] = R[3 * r + c] * s[c] * s[c];
-    float Sig[9];
-    mm3_nt(RS2, R, Sig);
-    // W = V3 * M3 (row-major from the column-major matrices)
-    float V3[9], M3[9], Wm[9];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) {
-        V3[3 * r + c] = fc.V[4 * c + r];
-        M3[3 * r + c] = fc.M[4 * c + r];
-      }
-    mm3(V3, M3, Wm);
-    const float mx = sc.means[3 * (size_t)id], my = sc.means[3 * (size_t)id + 1], mz = sc.means[3 * (size_t)id + 2];
-    float p4[4] = {mx, my, mz, 1.f}, t4[4], pv[4];
-    m4_mulv(fc.M, p4, t4);
-    m4_mulv(fc.V, t4, pv);
-    const float x = pv[0], y = pv[1], z = pv[2];
-    float WS[9], Sp[9];
-    mm3(Wm, Sig, WS);
-    mm3_nt(WS, Wm, Sp);
-    const float iz = 1.f / z, iz2 = iz * iz;
-    const float J2[6] = {-iz, 0.f, x * iz2, 0.f, -iz, y * iz2};
-    float JS[6];  // J2 Sp (2x3)
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 3; ++c) JS[3 * r + c] = J2[3 * r] * Sp[c] + J2[3 * r + 1] * Sp[3 + c] + J2[3 * r + 2] * Sp[6 + c];
-    float S2[4];  // J2 Sp J2^T
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c)
-        S2[2 * r + c] = JS[3 * r] * J2[3 * c] + JS[3 * r + 1] * J2[3 * c + 1] + JS[3 * r + 2] * J2[3 * c + 2];
-    const float Pm[4] = {fc.P[0], fc.P[4], fc.P[1], fc.P[5]};  // mat2(projection), row-major
-    float PS[4], cov[4];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c) PS[2 * r + c] = Pm[2 * r] * S2[c] + Pm[2 * r + 1] * S2[2 + c];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c) cov[2 * r + c] = PS[2 * r] * Pm[c] + PS[2 * r + 1] * Pm[2 + c];
-    cov[0] += fc.inv_w2;
-    cov[3] += fc.inv_h2;
-    const float hw = 0.5f * (float)fc.width, hh = 0.5f * (float)fc.height;
-    const float cp00 = cov[0] * hw * hw, cp01 = cov[1] * hw * hh, cp10 = cov[2] * hh * hw, cp11 = cov[3] * hh * hh;
-    const float det = cp00 * cp11 - cp01 * cp10;
-    const float Q00 = cp11 / det, Q01 = -cp01 / det, Q10 = -cp10 / det, Q11 = cp00 / det;
+      for (int c = 0; c < 3; ++c) RS2[3 * r + c] = P.R[3 * r + c] * P.s[c] * P.s[c];
+    mm3_nt(RS2, P.R, P.Sig);
+  }
 
-    // ---- conic -> cov2d (pixels) -> cov2d (NDC) -> S2 ----
-    const float GQ00 = gv[4], GQ01 = 0.5f * gv[5], GQ11 = gv[6];
-    // G_px = -Q GQ Q
-    const float A00 = Q00 * GQ00 + Q01 * GQ01, A01 = Q00 * GQ01 + Q01 * GQ11;
-    const float A10 = Q10 * GQ00 + Q11 * GQ01, A11 = Q10 * GQ01 + Q11 * GQ11;
-    const float Gp00 = -(A00 * Q00 + A01 * Q10), Gp01 = -(A00 * Q01 + A01 * Q11);
-    const float Gp10 = -(A10 * Q00 + A11 * Q10), Gp11 = -(A10 * Q01 + A11 * Q11);
-    const float Gn[4] = {Gp00 * hw * hw, Gp01 * hw * hh, Gp10 * hh * hw, Gp11 * hh * hh};
-    // cov = Pm S2 Pm  ->  dS2 = Pm^T Gn Pm^T
-    float T1[4], GS2[4];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c) T1[2 * r + c] = Pm[r] * Gn[c] + Pm[2 + r] * Gn[2 + c];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c) GS2[2 * r + c] = T1[2 * r] * Pm[c * 2] + T1[2 * r + 1] * Pm[c * 2 + 1];
-    // S2 = J2 Sp J2^T -> dSp = J2^T GS2 J2, dJ2 = (GS2 + GS2^T) J2 Sp
-    float GSp[9];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) {
-        float acc = 0.f;
-        for (int i = 0; i < 2; ++i)
-          for (int j = 0; j < 2; ++j) acc += J2[3 * i + r] * GS2[2 * i + j] * J2[3 * j + c];
-        GSp[3 * r + c] = acc;
-      }
-    float GJ2[6];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 3; ++c) {
-        float acc = 0.f;
-        for (int i = 0; i < 2; ++i) acc += (GS2[2 * r + i] + GS2[2 * i + r]) * JS[3 * i + c];
-        GJ2[3 * r + c] = acc;
-      }
-    // Sp = W Sig W^T -> dSig = W^T GSp W
-    float TT[9], GSig[9];
-    mm3_tn(Wm, GSp, TT);
-    mm3(TT, Wm, GSig);
-
-    // ---- view-space position: J2 and the projected centre ----
-    float gpv[3];
-    gpv[0] = GJ2[2] * iz2;
-    gpv[1] = GJ2[5] * iz2;
-    gpv[2] = (GJ2[0] + GJ2[4]) * iz2 - 2.f * (x * GJ2[2] + y * GJ2[5]) * iz2 * iz;
-    {
-      float clip[4];
-      float pv4[4] = {x, y, z, 1.f};
-      m4_mulv(fc.P, pv4, clip);
-      const float iw = 1.f / clip[3];
-      const float ndx = clip[0] * iw, ndy = clip[1] * iw;
-      const float gx = gv[7] * hw, gy = gv[8] * hh;  // d/d ndc
-      for (int j = 0; j < 3; ++j) {
-        gpv[j] += gx * (fc.P[4 * j + 0] - ndx * fc.P[4 * j + 3]) * iw;
-        gpv[j] += gy * (fc.P[4 * j + 1] - ndy * fc.P[4 * j + 3]) * iw;
-      }
+  float GSig[9] = {}, gm[3] = {}, gsh[48] = {};
+  float gop = 0.f;
+  bool any = false;
+  for (uint32_t vi = sc.view_begin; vi < sc.view_end; ++vi) {
+    const DiffView& view = L.views[L.scene_views[vi]];
+    const uint32_t vb = view.block_base + lb;
+    const uint64_t* mb = masks + (size_t)vb * kMaskWords;
+    const uint64_t m = mb[w];
+    if (m == 0) continue;
+    if (!((m >> lane) & 1ull)) continue;
+    uint32_t slot = offsets[vb] + mbcnt(m);
+    for (uint32_t j = 0; j < w; ++j) slot += (uint32_t)__popcll(mb[j]);
+    const float* gk = g9 + 9 * (size_t)slot;
+    float gv[9];
+    bool nz = false;
+    for (int j = 0; j < 9; ++j) {
+      gv[j] = gk[j];
+      nz = nz || gv[j] != 0.f;
     }
-    // mean <- W^T gpv
-    float gm[3];
-    for (int j = 0; j < 3; ++j) gm[j] = Wm[j] * gpv[0] + Wm[3 + j] * gpv[1] + Wm[6 + j] * gpv[2];
+    if (!nz) continue;
+    any = true;
+    gop += gv[3];
+    chain_view(view.fc, P, gv, GSig, gm, gsh);
+  }
+  if (!valid) return;
 
-    // ---- colour: SH coefficients and view direction ----
-    const float d0 = mx - fc.eye_model[0], d1 = my - fc.eye_model[1], d2 = mz - fc.eye_model[2];
-    const float dl = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
-    const float dx = d0 / dl, dy = d1 / dl, dz = d2 / dl;
-    float basis[16];
-    sh_basis(dx, dy, dz, basis);
-    const float* shp = sc.sh + 48 * (size_t)id;
-    float gcol[3];
-    for (int ch = 0; ch < 3; ++ch) {
-      float acc = 0.f;
-      for (int j = 0; j < 16; ++j) acc += basis[j] * shp[ch * 16 + j];
-      gcol[ch] = (acc + 0.5f < 0.f) ? 0.f : gv[ch];
-    }
-    float* gsh = sc.g_sh + 48 * (size_t)id;
-    float gb[16];
-    for (int j = 0; j < 16; ++j) {
-      gb[j] = gcol[0] * shp[j] + gcol[1] * shp[16 + j] + gcol[2] * shp[32 + j];
-      for (int ch = 0; ch < 3; ++ch)
-        if (gcol[ch] != 0.f) unsafeAtomicAdd(&gsh[ch * 16 + j], gcol[ch] * basis[j]);
-    }
-    {
-      const float C1 = 0.4886025119029199f, C20 = 1.0925484305920792f, C21 = 0.31539156525252005f,
-                  C22 = 0.5462742152960396f, C30 = 0.5900435899266435f, C31 = 2.890611442640554f,
-                  C32 = 0.4570457994644658f, C33 = 0.3731763325901154f, C34 = 1.445305721320277f;
-      const float X = dx, Y = dy, Z = dz, XX = X * X, YY = Y * Y, ZZ = Z * Z;
-      float gdx = 0.f, gdy = 0.f, gdz = 0.f;
-      gdy += gb[1] * -C1;
-      gdz += gb[2] * C1;
-      gdx += gb[3] * -C1;
-      gdx += gb[4] * C20 * Y;
-      gdy += gb[4] * C20 * X;
-      gdy += gb[5] * -C20 * Z;
-      gdz += gb[5] * -C20 * Y;
-      gdx += gb[6] * -2.f * C21 * X;
-      gdy += gb[6] * -2.f * C21 * Y;
-      gdz += gb[6] * 4.f * C21 * Z;
-      gdx += gb[7] * -C20 * Z;
-      gdz += gb[7] * -C20 * X;
-      gdx += gb[8] * 2.f * C22 * X;
-      gdy += gb[8] * -2.f * C22 * Y;
-      gdx += gb[9] * -6.f * C30 * X * Y;
-      gdy += gb[9] * -3.f * C30 * (XX - YY);
-      gdx += gb[10] * C31 * Y * Z;
-      gdy += gb[10] * C31 * X * Z;
-      gdz += gb[10] * C31 * X * Y;
-      gdx += gb[11] * 2.f * C32 * X * Y;
-      gdy += gb[11] * -C32 * (4.f * ZZ - XX - 3.f * YY);
-      gdz += gb[11] * -8.f * C32 * Y * Z;
-      gdx += gb[12] * -6.f * C33 * X * Z;
-      gdy += gb[12] * -6.f * C33 * Y * Z;
-      gdz += gb[12] * C33 * (6.f * ZZ - 3.f * XX - 3.f * YY);
-      gdx += gb[13] * -C32 * (4.f * ZZ - 3.f * XX - YY);
-      gdy += gb[13] * 2.f * C32 * X * Y;
-      gdz += gb[13] * -8.f * C32 * X * Z;
-      gdx += gb[14] * 2.f * C34 * X * Z;
-      gdy += gb[14] * -2.f * C34 * Y * Z;
-      gdz += gb[14] * C34 * (XX - YY);
-      gdx += gb[15] * -3.f * C30 * (XX - YY);
-      gdy += gb[15] * 6.f * C30 * X * Y;
-      const float dd = gdx * X + gdy * Y + gdz * Z;
-      gm[0] += (gdx - X * dd) / dl;
-      gm[1] += (gdy - Y * dd) / dl;
-      gm[2] += (gdz - Z * dd) / dl;
-    }
-    unsafeAtomicAdd(&sc.g_means[3 * (size_t)id + 0], gm[0]);
-    unsafeAtomicAdd(&sc.g_means[3 * (size_t)id + 1], gm[1]);
-    unsafeAtomicAdd(&sc.g_means[3 * (size_t)id + 2], gm[2]);
-
-    // ---- opacity ----
-    const float op = 1.f / (1.f + expf(-sc.opacity[id]));
-    unsafeAtomicAdd(&sc.g_opacity[id], gv[3] * op * (1.f - op));
-
+  float gls[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f};  // gq in PLY order (w, x, y, z)
+  float gopl = 0.f;
+  if (any) {
+    gopl = gop * op * (1.f - op);
     // ---- Sig = R diag(s^2) R^T -> scales and quaternion ----
-    float GR[9];  // 2 GSig R diag(s^2)  (GSig symmetric)
-    float GSR[9];
-    float Gs[9];
+    float Gs[9], GSR[9], GR[9];
     for (int i = 0; i < 9; ++i) Gs[i] = 0.5f * (GSig[i] + GSig[3 * (i % 3) + i / 3]);
-    mm3(Gs, R, GSR);
+    mm3(Gs, P.R, GSR);
     for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) GR[3 * r + c] = 2.f * GSR[3 * r + c] * s[c] * s[c];
+      for (int c = 0; c < 3; ++c) GR[3 * r + c] = 2.f * GSR[3 * r + c] * P.s[c] * P.s[c];
     for (int c = 0; c < 3; ++c) {
-      const float rgr = R[c] * GSR[c] + R[3 + c] * GSR[3 + c] + R[6 + c] * GSR[6 + c];  // (R^T Gs R)_cc
-      unsafeAtomicAdd(&sc.g_log_scales[3 * (size_t)id + c], 2.f * rgr * s[c] * s[c]);
+      const float rgr = P.R[c] * GSR[c] + P.R[3 + c] * GSR[3 + c] + P.R[6 + c] * GSR[6 + c];  // (R^T Gs R)_cc
+      gls[c] = 2.f * rgr * P.s[c] * P.s[c];
     }
-    const float qx = q[0], qy = q[1], qz = q[2], qw = q[3];
+    const float qx = P.q[0], qy = P.q[1], qz = P.q[2], qw = P.q[3];
     const float gqx = 2.f * (qy * (GR[1] + GR[3]) + qz * (GR[2] + GR[6]) - 2.f * qx * (GR[4] + GR[8]) +
                              qw * (GR[7] - GR[5]));
     const float gqy = 2.f * (qx * (GR[1] + GR[3]) + qz * (GR[5] + GR[7]) - 2.f * qy * (GR[0] + GR[8]) +
@@ -739,12 +871,19 @@ __global__ __launch_bounds__(256) void diff_project_backward_kernel(DiffLaunch L
                              qw * (GR[3] - GR[1]));
     const float gqw = 2.f * (qx * (GR[7] - GR[5]) + qy * (GR[2] - GR[6]) + qz * (GR[3] - GR[1]));
     const float dot = gqx * qx + gqy * qy + gqz * qz + gqw * qw;
-    float* gq = sc.g_quats + 4 * (size_t)id;  // PLY order (w, x, y, z)
-    unsafeAtomicAdd(&gq[0], (gqw - qw * dot) / qlen);
-    unsafeAtomicAdd(&gq[1], (gqx - qx * dot) / qlen);
-    unsafeAtomicAdd(&gq[2], (gqy - qy * dot) / qlen);
-    unsafeAtomicAdd(&gq[3], (gqz - qz * dot) / qlen);
+    gq[0] = (gqw - qw * dot) / P.qlen;
+    gq[1] = (gqx - qx * dot) / P.qlen;
+    gq[2] = (gqy - qy * dot) / P.qlen;
+    gq[3] = (gqz - qz * dot) / P.qlen;
   }
+  for (int i = 0; i < 3; ++i) {
+    sc.g_means[3 * (size_t)id + i] = gm[i];
+    sc.g_log_scales[3 * (size_t)id + i] = gls[i];
+  }
+  for (int i = 0; i < 4; ++i) sc.g_quats[4 * (size_t)id + i] = gq[i];
+  sc.g_opacity[id] = gopl;
+  float4* gsh4 = reinterpret_cast<float4*>(sc.g_sh + 48 * (size_t)id);
+  for (int j = 0; j < 12; ++j) gsh4[j] = make_float4(gsh[4 * j], gsh[4 * j + 1], gsh[4 * j + 2], gsh[4 * j + 3]);
 }
 
 uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
@@ -763,27 +902,26 @@ hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint32_t* d_
   return scan_exclusive_launch(p, counts, counts, nullptr, d_visible, partials, s);
 }
 
-hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* counts, float4* inst, uint32_t* key, uint32_t* val,
-                             uint32_t* ids, uint32_t* vid, hipStream_t s) {
+hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, float4* rec, uint4* rc, uint32_t* key,
+                             uint32_t* val, uint64_t* masks, hipStream_t s) {
   if (L.n_blocks == 0) return hipSuccess;
-  diff_write_kernel<<<L.n_blocks, kCompactBlock, 0, s>>>(L, counts, inst, key, val, ids, vid);
+  diff_write_kernel<<<L.n_blocks, kCompactBlock, 0, s>>>(L, offsets, rec, rc, key, val, masks);
   return hipGetLastError();
 }
 
-hipError_t launch_diff_bin_count(const DiffLaunch& L, const float4* inst, const uint32_t* order, const uint32_t* vid,
-                                 const uint32_t* d_count, uint32_t max_n, float4* rec, TileRect* rect, uint32_t* cnt,
-                                 uint32_t* kview, hipStream_t s) {
+hipError_t launch_diff_bin_gather(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
+                                  TileRect* rect, uint32_t* cnt, uint32_t* kview, hipStream_t s) {
   if (max_n == 0) return hipSuccess;
-  diff_bin_count_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, inst, order, vid, d_count, max_n, rec, rect,
-                                                                   cnt, kview);
+  diff_bin_gather_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rc, order, d_count, max_n, rect, cnt, kview);
   return hipGetLastError();
 }
 
 hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const uint32_t* cnt, const uint32_t* offs,
-                                const uint32_t* kview, const uint32_t* d_count, uint32_t max_n, uint64_t max_pairs,
-                                uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s) {
+                                const uint32_t* kview, const uint32_t* order, const uint32_t* d_count,
+                                uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* status,
+                                hipStream_t s) {
   if (max_n == 0) return hipSuccess;
-  diff_bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, rect, cnt, offs, kview, d_count, max_n,
+  diff_bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, rect, cnt, offs, kview, order, d_count, max_n,
                                                                   max_pairs, keys, vals, status);
   return hipGetLastError();
 }
@@ -799,12 +937,12 @@ hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const
   MACRO(64, 4)
 
 hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
-                              bool keep_state, hipStream_t s) {
+                              const uint32_t* tile_order, bool keep_state, hipStream_t s) {
   if (L.n_tiles == 0) return hipSuccess;
-#define VKGS_CASE(W_, H_)                                                                             \
-  if (L.tile_w == W_ && L.tile_h == H_) {                                                             \
-    diff_raster_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, keep_state ? 1 : 0); \
-    return hipGetLastError();                                                                         \
+#define VKGS_CASE(W_, H_)                                                                                         \
+  if (L.tile_w == W_ && L.tile_h == H_) {                                                                         \
+    diff_raster_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, tile_order, keep_state ? 1 : 0); \
+    return hipGetLastError();                                                                                     \
   }
   VKGS_DIFF_CASES(VKGS_CASE)
 #undef VKGS_CASE
@@ -812,11 +950,13 @@ hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint
 }
 
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
-                                       const uint32_t* ranges, float* g9, hipStream_t s) {
-  if (L.n_tiles == 0) return hipSuccess;
+                                       const uint32_t* ranges, const uint32_t* tile_order, const uint32_t* d_count,
+                                       uint32_t max_n, float* g9, hipStream_t s) {
+  if (max_n > 0) diff_zero_kernel<<<grid_for((uint64_t)max_n * 9 / 4 + 1, 256, 8192), 256, 0, s>>>(d_count, max_n, g9);
+  if (L.n_tiles == 0) return hipGetLastError();
 #define VKGS_CASE(W_, H_)                                                                          \
   if (L.tile_w == W_ && L.tile_h == H_) {                                                          \
-    diff_raster_backward_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, g9);   \
+    diff_raster_backward_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, tile_order, g9); \
     return hipGetLastError();                                                                      \
   }
   VKGS_DIFF_CASES(VKGS_CASE)
@@ -824,11 +964,10 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* order, const uint32_t* ids,
-                                        const uint32_t* vid, const float* g9, const uint32_t* d_count, uint32_t max_n,
-                                        hipStream_t s) {
-  if (max_n == 0) return hipSuccess;
-  diff_project_backward_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, order, ids, vid, g9, d_count, max_n);
+hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
+                                        const float* g9, hipStream_t s) {
+  if (L.n_scene_blocks == 0) return hipSuccess;
+  diff_project_backward_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, g9);
   return hipGetLastError();
 }
 

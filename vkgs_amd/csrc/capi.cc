@@ -87,8 +87,10 @@ int bits_for(uint32_t v) {  // bits needed for values in [0, v)
 }  // namespace
 
 // Raster with the longest-list-first tile schedule when its buffers are given.
+// Longest-list-first tile order (when b.tile_order is set), then the raster;
+// `mark` (optional) is recorded between the two.
 static hipError_t raster_scheduled(const DrawBuffers& b, const RasterShape& rs, uint32_t tiles, float4* out,
-                                   hipStream_t s) {
+                                   hipStream_t s, hipEvent_t mark) {
   if (b.tile_order) {
     Words16 w = {};
     w.w[0] = tiles;
@@ -98,6 +100,7 @@ static hipError_t raster_scheduled(const DrawBuffers& b, const RasterShape& rs, 
                           b.tile_sort_storage, s);
     if (e != hipSuccess) return e;
   }
+  if (mark) (void)hipEventRecord(mark, s);
   return launch_rasterize(b.rec, b.vals, b.ranges, rs, b.tile_order, out, s);
 }
 
@@ -118,8 +121,7 @@ hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d
       hipSuccess)
     return e;
   if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
-  if (marks) (void)hipEventRecord(marks[1], s);
-  return raster_scheduled(b, rs, tiles, out, s);
+  return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }
 
 // Fused-path draw: raster records and (rect, count) words were written per
@@ -141,8 +143,7 @@ hipError_t run_draw_slots(const uint4* rc, const uint32_t* order, const uint32_t
       hipSuccess)
     return e;
   if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
-  if (marks) (void)hipEventRecord(marks[1], s);
-  return raster_scheduled(b, rs, tiles, out, s);
+  return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }
 
 }  // namespace vkgs

@@ -48,22 +48,25 @@ int bits_for(uint32_t v) {
 }  // namespace
 
 struct vkgs_diff_ctx {
-  // device tables
-  Buf views, scenes, block_view, tile_view;
+  // device tables (one packed upload: views, scenes, range bases, scene views)
+  Buf tables;
   // per-item buffers
   uint32_t cap_items = 0;
-  Buf counts, inst, key, val, ids, vid, kview, g9, misc, rec, rect, cnt, offs, scan_storage, depth_sort;
+  Buf counts, rec, rc, key, val, masks, kview, g9, misc, rect, cnt, offs, scan_storage, depth_sort;
   // pairs
   uint64_t max_pairs = 0;
   Buf pkeys, pvals, pair_sort, ranges;
+  // longest-list-first tile schedule
+  Buf tile_keys, tile_order, tile_sort;
   // per-pixel state
   Buf final_t, last;
   // host mirrors of the last forward
   std::vector<DiffView> h_views;
   std::vector<DiffScene> h_scenes;
-  std::vector<uint32_t> h_block_view, h_tile_view;
+  std::vector<uint32_t> h_view_blocks, h_view_tiles, h_scene_blocks, h_scene_views;
+  std::vector<unsigned char> h_tables;
   DiffLaunch launch{};
-  SortPlan depth_plan{}, pair_plan{};
+  SortPlan depth_plan{}, pair_plan{}, tile_plan{};
   ScanPlan scan{};
   bool have_forward = false;
   bool grow_pairs = false;
@@ -71,17 +74,43 @@ struct vkgs_diff_ctx {
 
 namespace {
 
-enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2 };
+enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscTiles = 3 };
 
-hipError_t upload(Buf& b, const void* src, size_t bytes, hipStream_t s) {
-  hipError_t e = b.ensure(bytes);
-  if (e != hipSuccess) return e;
-  // the host vectors live in the context until the next forward; the copy is
-  // made synchronous with respect to the host so they can be reused at once
-  e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
-  if (e != hipSuccess) return e;
-  return hipStreamSynchronize(s);
+template <typename T>
+size_t pack(std::vector<unsigned char>& out, const std::vector<T>& v) {
+  const size_t off = align_up(out.size(), 256);
+  out.resize(off + sizeof(T) * v.size());
+  if (!v.empty()) memcpy(out.data() + off, v.data(), sizeof(T) * v.size());
+  return off;
 }
+
+// Packs the host tables into one buffer, uploads it (synchronously with
+// respect to the host, so the host vectors can change at once) and points the
+// launch's table pointers into it.
+hipError_t upload_tables(vkgs_diff_ctx* c, hipStream_t s) {
+  std::vector<unsigned char>& h = c->h_tables;
+  h.clear();
+  const size_t o_views = pack(h, c->h_views), o_scenes = pack(h, c->h_scenes);
+  const size_t o_vb = pack(h, c->h_view_blocks), o_vt = pack(h, c->h_view_tiles);
+  const size_t o_sb = pack(h, c->h_scene_blocks), o_sv = pack(h, c->h_scene_views);
+  hipError_t e = c->tables.ensure(h.size());
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(c->tables.p, h.data(), h.size(), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  unsigned char* base = c->tables.as<unsigned char>();
+  DiffLaunch& L = c->launch;
+  L.views = reinterpret_cast<const DiffView*>(base + o_views);
+  L.scenes = reinterpret_cast<const DiffScene*>(base + o_scenes);
+  L.view_blocks = reinterpret_cast<const uint32_t*>(base + o_vb);
+  L.view_tiles = reinterpret_cast<const uint32_t*>(base + o_vt);
+  L.scene_blocks = reinterpret_cast<const uint32_t*>(base + o_sb);
+  L.scene_views = reinterpret_cast<const uint32_t*>(base + o_sv);
+  return hipSuccess;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 }  // namespace
 
@@ -98,11 +127,10 @@ int vkgs_diff_create(vkgs_diff_ctx** out) {
 int vkgs_diff_destroy(vkgs_diff_ctx* c) {
   if (!c) return VKGS_ERR_INVALID_ARG;
   (void)hipDeviceSynchronize();
-  Buf* bufs[] = {&c->views,   &c->scenes, &c->block_view, &c->tile_view, &c->counts,       &c->inst,
-                 &c->key,     &c->val,    &c->ids,        &c->vid,       &c->kview,        &c->g9,
-                 &c->misc,    &c->rec,    &c->rect,       &c->cnt,       &c->offs,         &c->scan_storage,
-                 &c->depth_sort, &c->pkeys, &c->pvals,    &c->pair_sort, &c->ranges,       &c->final_t,
-                 &c->last};
+  Buf* bufs[] = {&c->tables, &c->counts, &c->rec,    &c->rc,        &c->key,    &c->val,
+                 &c->masks,  &c->kview,  &c->g9,     &c->misc,      &c->rect,   &c->cnt,
+                 &c->offs,   &c->scan_storage, &c->depth_sort, &c->pkeys, &c->pvals, &c->pair_sort,
+                 &c->ranges, &c->final_t, &c->last, &c->tile_keys, &c->tile_order, &c->tile_sort};
   for (Buf* b : bufs) b->release();
   delete c;
   return VKGS_OK;
@@ -119,9 +147,12 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   }
   // ---- host tables ----
   c->h_scenes.assign(n_scenes, DiffScene{});
+  c->h_scene_blocks.assign(n_scenes + 1, 0u);
+  uint64_t sblocks = 0;
   for (uint32_t i = 0; i < n_scenes; ++i) {
     const vkgs_diff_scene& a = scenes[i];
     if (a.n > 0 && (!a.means || !a.log_scales || !a.quats || !a.opacity_logits || !a.sh)) return VKGS_ERR_INVALID_ARG;
+    if (a.n > 0 && !aligned16(a.sh)) return VKGS_ERR_INVALID_ARG;
     if (a.n > (1u << 28)) return VKGS_ERR_CAPACITY;
     DiffScene& d = c->h_scenes[i];
     d.means = a.means;
@@ -135,12 +166,16 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
     d.g_opacity = a.grad_opacity_logits;
     d.g_sh = a.grad_sh;
     d.n = a.n;
+    d.block_base = (uint32_t)sblocks;
+    c->h_scene_blocks[i] = (uint32_t)sblocks;
+    sblocks += ceil_div(a.n, kCompactBlock);
   }
+  c->h_scene_blocks[n_scenes] = (uint32_t)sblocks;
   RasterShape rs0;
   if (!make_raster_shape(16, 16, cfg, &rs0)) return VKGS_ERR_INVALID_ARG;
   c->h_views.assign(n_views, DiffView{});
-  c->h_block_view.clear();
-  c->h_tile_view.clear();
+  c->h_view_blocks.assign(n_views + 1, 0u);
+  c->h_view_tiles.assign(n_views + 1, 0u);
   uint64_t blocks = 0, tiles = 0, pixels = 0;
   for (uint32_t v = 0; v < n_views; ++v) {
     const vkgs_diff_view& a = views[v];
@@ -158,15 +193,23 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
     d.tile_base = (uint32_t)tiles;
     d.out = reinterpret_cast<float4*>(a.rgba);
     d.grad_out = reinterpret_cast<const float4*>(a.grad_rgba);
-    const uint32_t nb = ceil_div(c->h_scenes[a.scene].n, kCompactBlock);
-    const uint32_t nt = rs.tiles_x * rs.tiles_y;
-    for (uint32_t i = 0; i < nb; ++i) c->h_block_view.push_back(v);
-    for (uint32_t i = 0; i < nt; ++i) c->h_tile_view.push_back(v);
-    blocks += nb;
-    tiles += nt;
+    c->h_view_blocks[v] = (uint32_t)blocks;
+    c->h_view_tiles[v] = (uint32_t)tiles;
+    blocks += ceil_div(c->h_scenes[a.scene].n, kCompactBlock);
+    tiles += (uint64_t)rs.tiles_x * rs.tiles_y;
     pixels += (uint64_t)rs.width * rs.height;
+    if (blocks * kCompactBlock > (1ull << 30) || tiles > (1u << 24)) return VKGS_ERR_CAPACITY;
   }
-  if (blocks * kCompactBlock > (1ull << 30) || tiles > (1u << 24)) return VKGS_ERR_CAPACITY;
+  c->h_view_blocks[n_views] = (uint32_t)blocks;
+  c->h_view_tiles[n_views] = (uint32_t)tiles;
+  // views grouped by scene, in view order within a scene
+  c->h_scene_views.clear();
+  for (uint32_t i = 0; i < n_scenes; ++i) {
+    c->h_scenes[i].view_begin = (uint32_t)c->h_scene_views.size();
+    for (uint32_t v = 0; v < n_views; ++v)
+      if (views[v].scene == i) c->h_scene_views.push_back(v);
+    c->h_scenes[i].view_end = (uint32_t)c->h_scene_views.size();
+  }
   const uint32_t items = (uint32_t)(blocks * kCompactBlock);
   const uint32_t cap = items ? items : 1;
 
@@ -181,18 +224,18 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   c->depth_plan = sort_plan(cap);
   c->pair_plan = sort_plan((uint32_t)c->max_pairs);
   c->scan = scan_plan(cap);
+  c->tile_plan = sort_plan((uint32_t)(tiles ? tiles : 1));
 #define ENSURE(buf, bytes) \
   if (e == hipSuccess) e = c->buf.ensure(bytes)
   ENSURE(counts, compact_storage_bytes(cap));
-  ENSURE(inst, (size_t)cap * 48);
+  ENSURE(rec, (size_t)cap * 48);
+  ENSURE(rc, (size_t)cap * 16);
   ENSURE(key, (size_t)cap * 4);
   ENSURE(val, (size_t)cap * 4);
-  ENSURE(ids, (size_t)cap * 4);
-  ENSURE(vid, (size_t)cap * 4);
+  ENSURE(masks, (size_t)(blocks ? blocks : 1) * kMaskWords * 8);
   ENSURE(kview, (size_t)cap * 4);
   ENSURE(g9, (size_t)cap * 36);
   ENSURE(misc, 256);
-  ENSURE(rec, (size_t)cap * 48);
   ENSURE(rect, (size_t)cap * sizeof(TileRect));
   ENSURE(cnt, (size_t)cap * 4);
   ENSURE(offs, (size_t)cap * 4);
@@ -202,6 +245,9 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   ENSURE(pvals, (size_t)c->max_pairs * 4);
   ENSURE(pair_sort, c->pair_plan.bytes);
   ENSURE(ranges, (size_t)(tiles ? tiles : 1) * 8);
+  ENSURE(tile_keys, (size_t)(tiles ? tiles : 1) * 4);
+  ENSURE(tile_order, (size_t)(tiles ? tiles : 1) * 4);
+  ENSURE(tile_sort, c->tile_plan.bytes);
   ENSURE(final_t, (size_t)(pixels ? pixels : 1) * 4);
   ENSURE(last, (size_t)(pixels ? pixels : 1) * 4);
 #undef ENSURE
@@ -213,19 +259,14 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
     d.last = c->last.as<uint32_t>() + pix;
     pix += (uint64_t)d.fc.width * d.fc.height;
   }
-  VKGS_HIP_CHECK(upload(c->views, c->h_views.data(), sizeof(DiffView) * n_views, s));
-  VKGS_HIP_CHECK(upload(c->scenes, c->h_scenes.data(), sizeof(DiffScene) * n_scenes, s));
-  VKGS_HIP_CHECK(upload(c->block_view, c->h_block_view.data(), 4 * (c->h_block_view.size() + 1), s));
-  VKGS_HIP_CHECK(upload(c->tile_view, c->h_tile_view.data(), 4 * (c->h_tile_view.size() + 1), s));
+  VKGS_HIP_CHECK(upload_tables(c, s));
 
   DiffLaunch& L = c->launch;
-  L.views = c->views.as<DiffView>();
-  L.scenes = c->scenes.as<DiffScene>();
-  L.block_view = c->block_view.as<uint32_t>();
-  L.tile_view = c->tile_view.as<uint32_t>();
   L.n_views = n_views;
+  L.n_scenes = n_scenes;
   L.n_blocks = (uint32_t)blocks;
   L.n_tiles = (uint32_t)tiles;
+  L.n_scene_blocks = (uint32_t)sblocks;
   L.tile_w = rs0.tile_w;
   L.tile_h = rs0.tile_h;
   L.tile_shift_x = rs0.tile_shift_x;
@@ -235,25 +276,29 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
 
   // ---- forward ----
   uint32_t* misc = c->misc.as<uint32_t>();
+  uint32_t* order = c->val.as<uint32_t>();
   VKGS_HIP_CHECK(fill_u32_launch(misc, 0u, 4, s));
   VKGS_HIP_CHECK(launch_diff_count(L, c->counts.as<uint32_t>(), misc + kMiscVisible, s));
-  VKGS_HIP_CHECK(launch_diff_write(L, c->counts.as<uint32_t>(), c->inst.as<float4>(), c->key.as<uint32_t>(),
-                                   c->val.as<uint32_t>(), c->ids.as<uint32_t>(), c->vid.as<uint32_t>(), s));
-  VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), c->val.as<uint32_t>(), 30u,
+  VKGS_HIP_CHECK(launch_diff_write(L, c->counts.as<uint32_t>(), c->rec.as<float4>(), c->rc.as<uint4>(),
+                                   c->key.as<uint32_t>(), order, c->masks.as<uint64_t>(), s));
+  VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, 30u,
                                 c->depth_sort.p, s));
-  VKGS_HIP_CHECK(launch_diff_bin_count(L, c->inst.as<float4>(), c->val.as<uint32_t>(), c->vid.as<uint32_t>(),
-                                       misc + kMiscVisible, cap, c->rec.as<float4>(), c->rect.as<TileRect>(),
-                                       c->cnt.as<uint32_t>(), c->kview.as<uint32_t>(), s));
+  VKGS_HIP_CHECK(launch_diff_bin_gather(c->rc.as<uint4>(), order, misc + kMiscVisible, cap, c->rect.as<TileRect>(),
+                                        c->cnt.as<uint32_t>(), c->kview.as<uint32_t>(), s));
   VKGS_HIP_CHECK(scan_exclusive_launch(c->scan, c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(), misc + kMiscVisible,
                                        misc + kMiscPairs, c->scan_storage.p, s));
   VKGS_HIP_CHECK(launch_diff_bin_emit(L, c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),
-                                      c->kview.as<uint32_t>(), misc + kMiscVisible, cap, c->max_pairs,
+                                      c->kview.as<uint32_t>(), order, misc + kMiscVisible, cap, c->max_pairs,
                                       c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(), misc + kMiscStatus, s));
   VKGS_HIP_CHECK(sort_kv_launch(c->pair_plan, misc + kMiscPairs, c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(),
                                 (uint32_t)bits_for((uint32_t)tiles), c->pair_sort.p, s));
   VKGS_HIP_CHECK(launch_ranges(c->pkeys.as<uint32_t>(), misc + kMiscPairs, c->max_pairs, (uint32_t)tiles,
                                c->ranges.as<uint32_t>(), s));
-  VKGS_HIP_CHECK(launch_diff_raster(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(), c->ranges.as<uint32_t>(), true, s));
+  VKGS_HIP_CHECK(fill_u32_launch(misc + kMiscTiles, (uint32_t)tiles, 1, s));
+  VKGS_HIP_CHECK(launch_tile_order(c->ranges.as<uint32_t>(), (uint32_t)tiles, c->tile_keys.as<uint32_t>(),
+                                   c->tile_order.as<uint32_t>(), misc + kMiscTiles, c->tile_plan, c->tile_sort.p, s));
+  VKGS_HIP_CHECK(launch_diff_raster(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(), c->ranges.as<uint32_t>(),
+                                    c->tile_order.as<uint32_t>(), true, s));
   c->have_forward = true;
   return VKGS_OK;
 }
@@ -273,21 +318,21 @@ int vkgs_diff_backward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t
     if (a.n != d.n) return VKGS_ERR_INVALID_ARG;
     if (d.n > 0 && (!a.grad_means || !a.grad_log_scales || !a.grad_quats || !a.grad_opacity_logits || !a.grad_sh))
       return VKGS_ERR_INVALID_ARG;
+    if (d.n > 0 && !aligned16(a.grad_sh)) return VKGS_ERR_INVALID_ARG;
     d.g_means = a.grad_means;
     d.g_log_scales = a.grad_log_scales;
     d.g_quats = a.grad_quats;
     d.g_opacity = a.grad_opacity_logits;
     d.g_sh = a.grad_sh;
   }
-  VKGS_HIP_CHECK(upload(c->views, c->h_views.data(), sizeof(DiffView) * n_views, s));
-  VKGS_HIP_CHECK(upload(c->scenes, c->h_scenes.data(), sizeof(DiffScene) * n_scenes, s));
+  VKGS_HIP_CHECK(upload_tables(c, s));
   const DiffLaunch& L = c->launch;
   uint32_t* misc = c->misc.as<uint32_t>();
-  VKGS_HIP_CHECK(fill_u32_launch(c->g9.as<uint32_t>(), 0u, (size_t)c->cap_items * 9, s));
   VKGS_HIP_CHECK(launch_diff_raster_backward(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(), c->ranges.as<uint32_t>(),
+                                             c->tile_order.as<uint32_t>(), misc + kMiscVisible, c->cap_items,
                                              c->g9.as<float>(), s));
-  VKGS_HIP_CHECK(launch_diff_project_backward(L, c->val.as<uint32_t>(), c->ids.as<uint32_t>(), c->vid.as<uint32_t>(),
-                                              c->g9.as<float>(), misc + kMiscVisible, c->cap_items, s));
+  VKGS_HIP_CHECK(launch_diff_project_backward(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(),
+                                              c->g9.as<float>(), s));
   return VKGS_OK;
 }
 

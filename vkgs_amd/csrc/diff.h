@@ -7,9 +7,14 @@
 // follows projection.comp:80-179 and the raster the reference's blend.
 //
 // Every (view, splat) pair is one item; items are grouped per view in blocks
-// of kCompactBlock so a block belongs to one view.  Tiles of all views share
-// one global id space (view tiles are contiguous), so sorting (tile, k) pairs
-// by global tile id puts the view id in the high key bits.
+// of kCompactBlock so a block belongs to one view.  Visible items get slots in
+// (view, splat id) order; the raster record of a slot is written by the
+// projection.  Tiles of all views share one global id space (view tiles are
+// contiguous), so sorting (tile, slot) pairs by global tile id puts the view id
+// in the high key bits.  The projection backward runs per scene splat and
+// visits the views of its scene, finding each view's slot from the forward's
+// per-wave visibility masks, so every gradient is written once, without
+// atomics.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,6 +24,8 @@
 #include "kernels.h"
 
 namespace vkgs {
+
+constexpr uint32_t kMaskWords = kCompactBlock / kWave;  // visibility ballots per compaction block
 
 struct DiffScene {
   const float* means;
@@ -32,7 +39,9 @@ struct DiffScene {
   float* g_opacity;
   float* g_sh;
   uint32_t n;
-  uint32_t pad[3];
+  uint32_t block_base;  // first projection-backward block of this scene
+  uint32_t view_begin;  // views of this scene: scene_views[view_begin .. view_end)
+  uint32_t view_end;
 };
 
 struct DiffView {
@@ -50,40 +59,39 @@ struct DiffView {
 struct DiffLaunch {
   const DiffView* views;
   const DiffScene* scenes;
-  const uint32_t* block_view;  // compaction block -> view
-  const uint32_t* tile_view;   // global tile -> view
-  uint32_t n_views, n_blocks, n_tiles, tile_w, tile_h, tile_shift_x, tile_shift_y;
+  const uint32_t* view_blocks;  // n_views + 1 ascending compaction-block bases
+  const uint32_t* view_tiles;   // n_views + 1 ascending global-tile bases
+  const uint32_t* scene_blocks; // n_scenes + 1 ascending projection-backward block bases
+  const uint32_t* scene_views;  // view indices grouped by scene
+  uint32_t n_views, n_scenes, n_blocks, n_tiles, n_scene_blocks;
+  uint32_t tile_w, tile_h, tile_shift_x, tile_shift_y;
   float t_min;
 };
 
-// Per-frame device buffers of the batched path.
-struct DiffBuffers {
-  uint32_t* counts;     // compact_storage_bytes(n_blocks * kCompactBlock)
-  float4* inst;         // 3 per slot
-  uint32_t* key;        // depth key per slot, sorted in place
-  uint32_t* val;        // slot, sorted with the keys -> draw order
-  uint32_t* ids;        // splat id per slot
-  uint32_t* vid;        // view per slot
-  uint32_t* kview;      // view per draw-order position
-  float* g9;            // 9 gradient words per draw-order position
-  uint32_t* misc;       // [0] visible, [1] pairs, [2] overflow
-};
-
 hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint32_t* d_visible, hipStream_t s);
-hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* counts, float4* inst, uint32_t* key, uint32_t* val,
-                             uint32_t* ids, uint32_t* vid, hipStream_t s);
-hipError_t launch_diff_bin_count(const DiffLaunch& L, const float4* inst, const uint32_t* order, const uint32_t* vid,
-                                 const uint32_t* d_count, uint32_t max_n, float4* rec, TileRect* rect, uint32_t* cnt,
-                                 uint32_t* kview, hipStream_t s);
+// Per visible item at its slot: raster record rec[3 slot..], (tile rect x,
+// tile rect y, tile count, view) in rc[slot], key = bits(1 - z), val = slot;
+// masks[kMaskWords b + w] = visibility ballot of wave w of block b.
+hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, float4* rec, uint4* rc, uint32_t* key,
+                             uint32_t* val, uint64_t* masks, hipStream_t s);
+// rect[k], cnt[k], kview[k] of slot order[k].
+hipError_t launch_diff_bin_gather(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
+                                  TileRect* rect, uint32_t* cnt, uint32_t* kview, hipStream_t s);
+// (global tile, slot) pairs at offs[k]; pairs beyond max_pairs set *status.
 hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const uint32_t* cnt, const uint32_t* offs,
-                                const uint32_t* kview, const uint32_t* d_count, uint32_t max_n, uint64_t max_pairs,
-                                uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s);
+                                const uint32_t* kview, const uint32_t* order, const uint32_t* d_count,
+                                uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* status,
+                                hipStream_t s);
+// tile_order (optional): workgroup i rasterizes global tile tile_order[i].
 hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
-                              bool keep_state, hipStream_t s);
+                              const uint32_t* tile_order, bool keep_state, hipStream_t s);
+// Zeroes g9 (9 words per slot < *d_count), then accumulates the per-slot
+// gradient words of the blend.
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
-                                       const uint32_t* ranges, float* g9, hipStream_t s);
-hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* order, const uint32_t* ids,
-                                        const uint32_t* vid, const float* g9, const uint32_t* d_count, uint32_t max_n,
-                                        hipStream_t s);
+                                       const uint32_t* ranges, const uint32_t* tile_order, const uint32_t* d_count,
+                                       uint32_t max_n, float* g9, hipStream_t s);
+// Writes (overwrites) every parameter gradient of every scene.
+hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
+                                        const float* g9, hipStream_t s);
 
 }  // namespace vkgs

@@ -1,0 +1,62 @@
+// Pair emission of the binning stage, shared by the plain and the batched
+// paths (device code only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace vkgs {
+
+// Splats covering more tiles than this are emitted by a whole wave.
+constexpr uint32_t kEmitSerial = 16;
+
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)lane);
+}
+
+// Emits the pairs of one splat per lane: c tiles of rectangle r (tile x in
+// r.x's low/high 16 bits, tile y in r.y's) as keys key_base + ty * tiles_x + tx
+// with value val, at o, o + 1, ... in row-major tile order; pairs at or beyond
+// max_pairs are dropped (the caller flags the overflow).  Lanes with up to
+// kEmitSerial tiles write their own pairs; each larger splat is then written by
+// all 64 lanes together, so one near-camera splat does not serialize a wave.
+// Every lane of the wave must call this (c = 0 for lanes without a splat).
+__device__ __forceinline__ void emit_pairs_wave(uint32_t c, uint64_t o, TileRect r, uint32_t val, uint32_t key_base,
+                                                uint32_t tiles_x, uint64_t max_pairs, uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+  const bool big = c > kEmitSerial;
+  if (c != 0 && !big) {
+    const uint32_t tx0 = r.x & 0xffffu, tx1 = r.x >> 16, ty0 = r.y & 0xffffu, ty1 = r.y >> 16;
+    uint64_t j = o;
+    for (uint32_t ty = ty0; ty <= ty1; ++ty)
+      for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j) {
+        if (j < max_pairs) {
+          keys[j] = key_base + ty * tiles_x + tx;
+          vals[j] = val;
+        }
+      }
+  }
+  uint64_t pending = __ballot(big);
+  const uint32_t lane = __lane_id();
+  while (pending) {
+    const uint32_t src = (uint32_t)__builtin_ctzll(pending);
+    pending &= pending - 1;
+    const uint32_t bc = lane_u32(c, src), bval = lane_u32(val, src), bx = lane_u32(r.x, src);
+    const uint32_t by = lane_u32(r.y, src), btx = lane_u32(tiles_x, src), bkey = lane_u32(key_base, src);
+    const uint64_t bo = ((uint64_t)lane_u32((uint32_t)(o >> 32), src) << 32) | lane_u32((uint32_t)o, src);
+    const uint32_t tx0 = bx & 0xffffu, w = (bx >> 16) - tx0 + 1, ty0 = by & 0xffffu;
+    const uint32_t first = bkey + ty0 * btx + tx0;
+    for (uint32_t i = lane; i < bc; i += kWave) {
+      const uint32_t row = i / w, col = i - row * w;
+      const uint64_t j = bo + i;
+      if (j < max_pairs) {
+        keys[j] = first + row * btx + col;
+        vals[j] = bval;
+      }
+    }
+  }
+}
+
+}  // namespace vkgs

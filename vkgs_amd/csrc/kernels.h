@@ -106,7 +106,8 @@ struct DrawBuffers {
 
 // Bins instances order[k] (k < *d_count), sorts the pairs by tile, marks the
 // tile ranges and rasterizes; marks[0..1] (optional) are recorded after the
-// binning and after the tile sort.
+// binning and right before the raster kernel (after the tile sort, the ranges
+// and the tile schedule).
 hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d_count, uint32_t max_inst,
                     const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
                     hipEvent_t* marks);

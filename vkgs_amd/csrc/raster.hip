@@ -26,6 +26,7 @@
 #pragma clang fp contract(off)
 
 #include "common.h"
+#include "emit.h"
 #include "kernels.h"
 
 namespace vkgs {
@@ -87,22 +88,24 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                        uint32_t* __restrict__ status) {
   const uint32_t n = min(*d_count, max_n);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const uint32_t c = cnt[k];
-    if (c == 0) continue;
-    const uint64_t o = offs[k];
-    const TileRect r = rect[k];
-    const uint32_t v = order ? order[k] : k;
-    if (o + c > max_pairs) atomicOr(status, 1u);
-    const uint32_t tx0 = r.x & 0xffffu, tx1 = r.x >> 16, ty0 = r.y & 0xffffu, ty1 = r.y >> 16;
-    uint64_t j = o;
-    for (uint32_t ty = ty0; ty <= ty1; ++ty)
-      for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j) {
-        if (j < max_pairs) {
-          keys[j] = ty * tiles_x + tx;
-          vals[j] = v;
-        }
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // wave-uniform loop: emit_pairs_wave needs every lane
+  for (uint32_t base = wave * kWave; base < n; base += stride) {
+    const uint32_t k = base + __lane_id();
+    uint32_t c = 0, v = 0;
+    uint64_t o = 0;
+    TileRect r = {0u, 0u};
+    if (k < n) {
+      c = cnt[k];
+      if (c != 0) {
+        o = offs[k];
+        r = rect[k];
+        v = order ? order[k] : k;
+        if (o + c > max_pairs) atomicOr(status, 1u);
       }
+    }
+    emit_pairs_wave(c, o, r, v, 0u, tiles_x, max_pairs, keys, vals);
   }
 }
 

@@ -85,6 +85,11 @@ PARAM_NAMES = ("means", "log_scales", "quats", "opacity_logits", "sh")
 IDENTITY = np.eye(4, dtype=np.float32).ravel()
 
 
+def _aligned(t: torch.Tensor) -> torch.Tensor:
+    """t itself if its data is 16-byte aligned (the SH rows are read as float4), else an aligned copy."""
+    return t if t.data_ptr() % 16 == 0 else t.clone()
+
+
 def _stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
@@ -125,7 +130,8 @@ class _RenderBatch(torch.autograd.Function):
     @staticmethod
     def forward(ctx, meta, *params):
         views, n_scenes, cfg = meta
-        groups = [tuple(p.detach().contiguous().float() for p in params[5 * i:5 * i + 5]) for i in range(n_scenes)]
+        groups = [tuple(_aligned(p.detach().contiguous().float()) for p in params[5 * i:5 * i + 5])
+                  for i in range(n_scenes)]
         dev = groups[0][0].device
         outs = [torch.empty((cam.screen_size[1], cam.screen_size[0], 4), dtype=torch.float32, device=dev)
                 for _, cam, _ in views]
@@ -157,7 +163,8 @@ class _RenderBatch(torch.autograd.Function):
                 g = torch.zeros((cam.screen_size[1], cam.screen_size[0], 4), dtype=torch.float32,
                                 device=groups[0][0].device)
             gouts.append(g.contiguous().float())
-        grads = [tuple(torch.zeros_like(t) for t in g) for g in groups]
+        # the backward writes every gradient element, so no zero fill
+        grads = [tuple(torch.empty_like(t) for t in g) for g in groups]
         outs_dummy = gouts  # rgba pointers are not used by the backward
         sarr = _scene_structs(groups, grads)
         varr = _view_structs(views, outs_dummy, gouts)
