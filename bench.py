@@ -232,12 +232,17 @@ def main(argv=None) -> int:
     t_raster = stage_ms["raster"] * 1e-3
     proj_gbs = b_proj / t_proj / 1e9
     raster_tflops = f_raster / t_raster / 1e12
-    # reference-shaped sort traffic: 4 passes x (read + write) of 8-B pairs over V, 2 passes over K
-    b_sort = 64.0 * V + 32.0 * K
-    b_bin = 12.0 * K + 8.0 * K + 8.0 * ranges.shape[0]
+    T = ranges.shape[0]
     t_ps = (stage_ms["count"] + stage_ms["project"] + stage_ms["depth_sort"] + stage_ms["bin"]
             + stage_ms["tile_sort"]) * 1e-3
-    ps_gbs = (b_proj + b_sort + b_bin) / t_ps / 1e9
+    # algorithmic bytes of a (tile | depth) key design, fixed by the workload alone: 12-B pairs emitted,
+    # P = ceil((ceil(log2 T) + 32) / 8) radix passes each reading and writing every pair, ranges
+    passes = -(-(int(math.ceil(math.log2(max(T, 2)))) + 32) // 8)
+    b_emit, b_sort, b_ranges = 12.0 * K, 24.0 * K * passes, 8.0 * K + 8.0 * T
+    ps_gbs = (b_proj + b_emit + b_sort + b_ranges) / t_ps / 1e9
+    # bytes this design moves by its own algorithm (depth sort over V, tile sort over K, 8-B pairs)
+    own_bytes = b_proj + 64.0 * V + 32.0 * K + 20.0 * K + 8.0 * T
+    own_gbs = own_bytes / t_ps / 1e9
 
     raster_name = "rasterize_kernel" if r_variant == 0 else "rasterize_wave_kernel"
     if t_raster >= t_proj:
@@ -276,6 +281,10 @@ def main(argv=None) -> int:
         "roofline": roof,
         "roofline_projection": roof_proj,
         "hbm_frac_projection_sort": round(ps_gbs / HBM_PEAK_GBS, 4),
+        "projection_sort": {"ms": round(t_ps * 1e3, 4), "bytes": round(b_proj + b_emit + b_sort + b_ranges),
+                            "passes": passes, "gbs": round(ps_gbs, 1),
+                            "own_bytes": round(own_bytes), "own_gbs": round(own_gbs, 1),
+                            "work": "12N + 172V + 12K + 24K*P + 8K + 8T (P radix passes of a (tile|depth) key)"},
     }
 
     # host copy of the splat storage for the CPU baseline, then free the renderer

@@ -117,3 +117,24 @@ def test_early_termination_gradients_are_finite():
     img.sum().backward()
     for k in NAMES:
         assert torch.isfinite(p[k].grad).all(), k
+
+
+def test_backward_writes_zeros_for_untouched_splats_and_viewless_scenes():
+    """The backward writes every gradient element: splats behind the camera of
+    the only view, and every splat of a scene that no view renders, get exact
+    zeros (the gradient tensors are not pre-zeroed)."""
+    sa, cam = small_scene(1000, 36, 64, 48)
+    sb, _ = small_scene(700, 37, 64, 48)
+    pa, pb = D.scene_tensors(sa), D.scene_tensors(sb)
+    # a third of scene a far behind the camera: culled, so untouched
+    eye = np.asarray(S.camera_arrays(cam)[2][:3], np.float64)
+    behind = np.arange(0, sa.n, 3)
+    with torch.no_grad():
+        pa["means"][behind] = torch.tensor(2.0 * eye, dtype=torch.float32, device="cuda")
+    outs = D.render_batch([pa, pb], [(0, cam)], t_min=0.0)  # scene b: no view
+    outs[0].sum().backward()
+    for k in NAMES:
+        assert torch.isfinite(pa[k].grad).all(), k
+        assert torch.count_nonzero(pa[k].grad[behind]) == 0, k
+        assert torch.count_nonzero(pb[k].grad) == 0, k
+    assert torch.count_nonzero(pa["means"].grad) > 0

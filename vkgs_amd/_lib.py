@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libvkgs_hip.so")
+# VKGS_HIP_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("VKGS_HIP_LIB") or os.path.join(_HERE, "lib", "libvkgs_hip.so")
 
 VKGS_OK = 0
 STATUS_NAMES = {

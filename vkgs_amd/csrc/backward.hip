@@ -10,9 +10,9 @@
 // Backward: the raster backward re-walks each tile list back to front from
 // that stored state, recomputing alpha per entry and the transmittance before
 // it as T_i = T_{i+1} / (1 - alpha_i); per entry the 9 gradient words (colour
-// 3, opacity 1, conic 3, centre 2) are summed over the wave's pixels with DPP
-// and flushed with one global float atomic per word per tile into the slot's
-// words.  The projection backward then runs one lane per scene splat, chains
+// 3, opacity 1, conic 3, centre 2) are summed over the tile's pixels (DPP row
+// sums, then LDS atomics) and flushed with one global float atomic per word per
+// tile into the slot's words.  The projection backward then runs one lane per scene splat, chains
 // the words of every view that saw the splat to its raw parameters and writes
 // each gradient once.
 //
@@ -22,7 +22,7 @@
 
 #include "common.h"
 #include "diff.h"
-#include "emit.h"
+#include "device.h"
 #include "kernels.h"
 
 namespace vkgs {
@@ -108,11 +108,34 @@ __device__ __forceinline__ void sh_basis(float x, float y, float z, float b[16])
   b[15] = (-C30 * x) * (xx - 3.f * yy);
 }
 
-// projection.comp:80-179 on raw parameters; fills the 48-byte instance.
-__device__ void diff_project(const FrameConst& fc, const DiffScene& sc, uint32_t id, float4 out[3]) {
+// View-independent inputs of the projection of one splat: mean, activated
+// cov3d (parse_ply.comp:106-138, column-major), fp32 SH and opacity.
+struct SplatInputs {
+  float p[3];
   float S[9];
-  activate_cov(sc, id, S);
-  float p4[4] = {sc.means[3 * (size_t)id], sc.means[3 * (size_t)id + 1], sc.means[3 * (size_t)id + 2], 1.f};
+  float sh[48];
+  float opacity;
+};
+
+__device__ __forceinline__ void load_inputs(const DiffScene& sc, uint32_t id, SplatInputs& in) {
+  activate_cov(sc, id, in.S);
+  for (int i = 0; i < 3; ++i) in.p[i] = sc.means[3 * (size_t)id + i];
+  const float4* sh4 = reinterpret_cast<const float4*>(sc.sh + 48 * (size_t)id);
+  for (int j = 0; j < 12; ++j) {
+    const float4 u = sh4[j];
+    in.sh[4 * j] = u.x;
+    in.sh[4 * j + 1] = u.y;
+    in.sh[4 * j + 2] = u.z;
+    in.sh[4 * j + 3] = u.w;
+  }
+  in.opacity = 1.f / (1.f + expf(-sc.opacity[id]));
+}
+
+// projection.comp:80-179 for one view; fills the 48-byte instance.
+__device__ void diff_project(const FrameConst& fc, const SplatInputs& in, float4 out[3]) {
+  float S[9];
+  for (int i = 0; i < 9; ++i) S[i] = in.S[i];
+  float p4[4] = {in.p[0], in.p[1], in.p[2], 1.f};
   const float d0 = p4[0] - fc.eye_model[0], d1 = p4[1] - fc.eye_model[1], d2 = p4[2] - fc.eye_model[2];
   const float dl = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
   const float dx = d0 / dl, dy = d1 / dl, dz = d2 / dl;
@@ -155,25 +178,23 @@ __device__ void diff_project(const FrameConst& fc, const DiffScene& sc, uint32_t
   const float nx = t4[0] / t4[3], ny = t4[1] / t4[3], nz = t4[2] / t4[3];
   float basis[16];
   sh_basis(dx, dy, dz, basis);
-  const float* shp = sc.sh + 48 * (size_t)id;
   float color[3];
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
     float acc = 0.f;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      float dot = basis[4 * j + 0] * shp[ch * 16 + 4 * j + 0];
-      dot = dot + basis[4 * j + 1] * shp[ch * 16 + 4 * j + 1];
-      dot = dot + basis[4 * j + 2] * shp[ch * 16 + 4 * j + 2];
-      dot = dot + basis[4 * j + 3] * shp[ch * 16 + 4 * j + 3];
+      float dot = basis[4 * j + 0] * in.sh[ch * 16 + 4 * j + 0];
+      dot = dot + basis[4 * j + 1] * in.sh[ch * 16 + 4 * j + 1];
+      dot = dot + basis[4 * j + 2] * in.sh[ch * 16 + 4 * j + 2];
+      dot = dot + basis[4 * j + 3] * in.sh[ch * 16 + 4 * j + 3];
       acc = (j == 0) ? dot : acc + dot;
     }
     color[ch] = glsl_max(acc + 0.5f, 0.f);
   }
-  const float opacity = 1.f / (1.f + expf(-sc.opacity[id]));
   out[0] = make_float4(nx, ny, nz, 0.f);
   out[1] = make_float4(s0 * cos_t, s0 * sin_t, (-s1) * sin_t, s1 * cos_t);
-  out[2] = make_float4(color[0], color[1], color[2], opacity);
+  out[2] = make_float4(color[0], color[1], color[2], in.opacity);
 }
 
 // Largest i < n with bases[i] <= x (bases ascending, x < bases[n]); empty
@@ -192,59 +213,69 @@ __device__ __forceinline__ uint32_t range_of(const uint32_t* __restrict__ bases,
 
 // --------------------------------------------------------------- forward ---
 
-__global__ __launch_bounds__(kCompactBlock) void diff_count_kernel(DiffLaunch L, uint32_t* __restrict__ counts) {
-  __shared__ uint32_t s_w[kMaskWords];
-  const uint32_t b = blockIdx.x;
-  const DiffView& view = L.views[range_of(L.view_blocks, L.n_views, b)];
-  const DiffScene& sc = L.scenes[view.scene];
-  const uint32_t id = (b - view.block_base) * kCompactBlock + threadIdx.x;
-  bool vis = false;
-  if (id < sc.n) {
-    float omz;
-    vis = rank_visible(view.fc.PVM, sc.means[3 * (size_t)id], sc.means[3 * (size_t)id + 1],
-                       sc.means[3 * (size_t)id + 2], &omz);
+// Visibility of every (view, splat) item, one lane per scene splat visiting
+// the views of its scene: masks[w] = the ballot of wave w (the 64 splats of a
+// wave, for one view) and counts[w] = its popcount, where w = kMaskWords *
+// (view.block_base + block) + wave.  Wave counts are scanned into slot offsets.
+__global__ __launch_bounds__(kCompactBlock) void diff_count_kernel(DiffLaunch L, uint32_t* __restrict__ counts,
+                                                                   uint64_t* __restrict__ masks) {
+  const uint32_t sb = blockIdx.x;
+  const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
+  const uint32_t lb = sb - sc.block_base;
+  const uint32_t id = lb * kCompactBlock + threadIdx.x;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const bool valid = id < sc.n;
+  float x = 0.f, y = 0.f, z = 0.f;
+  if (valid) {
+    x = sc.means[3 * (size_t)id];
+    y = sc.means[3 * (size_t)id + 1];
+    z = sc.means[3 * (size_t)id + 2];
   }
-  const uint64_t m = __ballot(vis);
-  if (__lane_id() == 0) s_w[threadIdx.x / kWave] = (uint32_t)__popcll(m);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t s = 0;
-    for (uint32_t i = 0; i < kMaskWords; ++i) s += s_w[i];
-    counts[b] = s;
+  for (uint32_t vi = sc.view_begin; vi < sc.view_end; ++vi) {
+    const DiffView& view = L.views[L.scene_views[vi]];
+    float omz;
+    const bool vis = valid && rank_visible(view.fc.PVM, x, y, z, &omz);
+    const uint64_t m = __ballot(vis);
+    if (__lane_id() == 0) {
+      const size_t wi = (size_t)(view.block_base + lb) * kMaskWords + w;
+      masks[wi] = m;
+      counts[wi] = (uint32_t)__popcll(m);
+    }
   }
 }
 
-// Projection of the visible items of one block straight into raster records
-// at their slots (as the plain fused path does).
+// Projection of the visible items into raster records at their slots (as the
+// plain fused path does), one lane per scene splat: the activated covariance,
+// SH and opacity are formed once and reused for every view of the scene.
 __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L, const uint32_t* __restrict__ offsets,
+                                                                   const uint64_t* __restrict__ masks,
                                                                    float4* __restrict__ rec, uint4* __restrict__ rc,
                                                                    uint32_t* __restrict__ key,
-                                                                   uint32_t* __restrict__ val,
-                                                                   uint64_t* __restrict__ masks) {
-  __shared__ uint32_t s_w[kMaskWords];
-  const uint32_t b = blockIdx.x;
-  const uint32_t v = range_of(L.view_blocks, L.n_views, b);
-  const DiffView& view = L.views[v];
-  const DiffScene& sc = L.scenes[view.scene];
-  const uint32_t id = (b - view.block_base) * kCompactBlock + threadIdx.x;
-  bool vis = false;
-  float omz = 0.f;
-  if (id < sc.n)
-    vis = rank_visible(view.fc.PVM, sc.means[3 * (size_t)id], sc.means[3 * (size_t)id + 1],
-                       sc.means[3 * (size_t)id + 2], &omz);
-  const uint64_t m = __ballot(vis);
-  const uint32_t w = threadIdx.x / kWave;
-  if (__lane_id() == 0) {
-    s_w[w] = (uint32_t)__popcll(m);
-    masks[(size_t)b * kMaskWords + w] = m;
-  }
-  __syncthreads();
-  uint32_t slot = offsets[b];
-  for (uint32_t i = 0; i < w; ++i) slot += s_w[i];
-  slot += mbcnt(m);
-  if (vis) {
+                                                                   uint32_t* __restrict__ val) {
+  const uint32_t sb = blockIdx.x;
+  const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
+  const uint32_t lb = sb - sc.block_base;
+  const uint32_t id = lb * kCompactBlock + threadIdx.x;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint32_t lane = __lane_id();
+  // a wave with no visible item in any view skips the parameter loads
+  bool any = false;
+  for (uint32_t vi = sc.view_begin; vi < sc.view_end && !any; ++vi)
+    any = masks[(size_t)(L.views[L.scene_views[vi]].block_base + lb) * kMaskWords + w] != 0;
+  if (!any) return;
+  SplatInputs in;
+  if (id < sc.n) load_inputs(sc, id, in);
+  for (uint32_t vi = sc.view_begin; vi < sc.view_end; ++vi) {
+    const uint32_t v = L.scene_views[vi];
+    const DiffView& view = L.views[v];
+    const size_t wi = (size_t)(view.block_base + lb) * kMaskWords + w;
+    const uint64_t m = masks[wi];
+    if (!((m >> lane) & 1ull)) continue;
+    const uint32_t slot = offsets[wi] + mbcnt(m);
+    float omz;
+    (void)rank_visible(view.fc.PVM, in.p[0], in.p[1], in.p[2], &omz);
     float4 o[3];
-    diff_project(view.fc, sc, id, o);
+    diff_project(view.fc, in, o);
     const float f[12] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w, o[2].x, o[2].y, o[2].z, o[2].w};
     Splat2D sp = {};
     uint4 r = make_uint4(0u, 0u, 0u, v);
@@ -384,7 +415,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
         const float uy = fmaf(bb.x, dx, bb.y * dy);
         const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && !done;
         const float q = fmaf(ux, ux, uy * uy);
-        const float alpha = in ? fminf(bb.z * __expf(-0.5f * q), kAlphaMax) : 0.f;
+        const float alpha = in ? fminf(bb.z * gauss_falloff(q), kAlphaMax) : 0.f;
         const float ta = T * alpha;
         cr = fmaf(c.x, ta, cr);
         cg = fmaf(c.y, ta, cg);
@@ -423,28 +454,27 @@ __global__ __launch_bounds__(256) void diff_zero_kernel(const uint32_t* __restri
   if (i < n) g[i] = 0.f;
 }
 
-template <int kCtrl, int kRowMask>
+template <int kCtrl>
 __device__ __forceinline__ float dpp_add(float x) {
-  // lanes of rows outside kRowMask read 0 (the `old` operand)
-  return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, kRowMask, 0xf, false));
+  return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xf, 0xf, false));
 }
 
-// Sum of x over the 64 lanes of a wave; every lane must be active.  The sum
-// lands in lane 63 (other lanes hold partial sums).
-__device__ __forceinline__ float wave_sum_last(float x) {
-  x = dpp_add<0xb1, 0xf>(x);   // quad_perm [1,0,3,2]: pairs
-  x = dpp_add<0x4e, 0xf>(x);   // quad_perm [2,3,0,1]: quads
-  x = dpp_add<0x124, 0xf>(x);  // row_ror:4
-  x = dpp_add<0x128, 0xf>(x);  // row_ror:8: every lane holds its row of 16
-  x = dpp_add<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
-  x = dpp_add<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3: lane 63 holds the total
-  return x;
+// Sum of x over each row of 16 lanes (DPP row); every lane must be active
+// and ends holding its row's sum.
+__device__ __forceinline__ float row_sum16(float x) {
+  x = dpp_add<0xb1>(x);   // quad_perm [1,0,3,2]: pairs
+  x = dpp_add<0x4e>(x);   // quad_perm [2,3,0,1]: quads
+  x = dpp_add<0x124>(x);  // row_ror:4
+  return dpp_add<0x128>(x);  // row_ror:8
 }
 
 // Re-walks one tile back to front from the stored per-pixel state.  Per list
-// entry the 9 gradient words are summed over the wave's pixels with DPP (no
-// LDS atomics on a shared address), kept in LDS per batch and flushed with
-// one global float atomic per nonzero word.
+// entry 9 sums over the tile's pixels are formed: within each row of 16 lanes
+// with DPP, then one LDS float atomic per row (4 per wave).  The sums are
+// colour 3, opacity 1, dq*dx^2, dq*dx*dy, dq*dy^2, dq*dx, dq*dy (dq = dL/dq of
+// the exponent's quadratic form); the conic and centre words follow from the
+// last five and the entry's conic rows when the batch is flushed, with one
+// global float atomic per nonzero word.
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                                       const uint32_t* __restrict__ vals,
@@ -453,9 +483,8 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
                                                                       float* __restrict__ g9) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
-  constexpr bool kOneWave = kN == kWave;
   __shared__ float4 s_rec[3][kN];
-  __shared__ float s_g[9][kN];
+  __shared__ float s_g[9][4][kN];  // per word, one partial sum per DPP row of the wave(s)
   __shared__ uint32_t s_k[kN];
   __shared__ uint32_t s_max;
 
@@ -472,7 +501,8 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   const int row1 = row0 + kRowsPerWave - 1;
   const uint32_t end = ranges[2 * gtile + 1];
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-  const bool last_lane = __lane_id() == kWave - 1;
+  const bool row_last = (__lane_id() & 15u) == 15u;
+  const uint32_t row = __lane_id() >> 4;
 
   uint32_t last = 0;
   float T = 1.f;
@@ -503,7 +533,9 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       s_rec[1][t] = rec[3 * (size_t)k + 1];
       s_rec[2][t] = rec[3 * (size_t)k + 2];
 #pragma unroll
-      for (int j = 0; j < 9; ++j) s_g[j][t] = 0.f;
+      for (int j = 0; j < 9; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s_g[j][r][t] = 0.f;
     }
     __syncthreads();
     for (int e = 0; e < cnt; ++e) {
@@ -524,7 +556,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       // every lane takes part from here (DPP reads all lanes); lanes with
       // !in contribute zeros and keep their state
       const float q = fmaf(ux, ux, uy * uy);
-      const float G = __expf(-0.5f * q);
+      const float G = gauss_falloff(q);
       const float raw = bb.z * G;
       const float alpha = in ? fminf(raw, kAlphaMax) : 0.f;
       const float om = 1.f - alpha;
@@ -539,47 +571,54 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const bool clamped = raw > kAlphaMax;
       const float dop = clamped ? 0.f : dalpha * G;
       const float dq = clamped ? 0.f : -0.5f * dalpha * alpha;
-      const float s0 = wave_sum_last(g.x * ta);
-      const float s1 = wave_sum_last(g.y * ta);
-      const float s2 = wave_sum_last(g.z * ta);
-      const float s3 = wave_sum_last(dop);
-      const float s4 = wave_sum_last(dq * dx * dx);
-      const float s5 = wave_sum_last(dq * 2.f * dx * dy);
-      const float s6 = wave_sum_last(dq * dy * dy);
-      const float s7 = wave_sum_last(-2.f * dq * (a.z * ux + bb.x * uy));
-      const float s8 = wave_sum_last(-2.f * dq * (a.w * ux + bb.y * uy));
-      if (last_lane) {
-        if (kOneWave) {
-          s_g[0][e] = s0;
-          s_g[1][e] = s1;
-          s_g[2][e] = s2;
-          s_g[3][e] = s3;
-          s_g[4][e] = s4;
-          s_g[5][e] = s5;
-          s_g[6][e] = s6;
-          s_g[7][e] = s7;
-          s_g[8][e] = s8;
-        } else {
-          atomicAdd(&s_g[0][e], s0);
-          atomicAdd(&s_g[1][e], s1);
-          atomicAdd(&s_g[2][e], s2);
-          atomicAdd(&s_g[3][e], s3);
-          atomicAdd(&s_g[4][e], s4);
-          atomicAdd(&s_g[5][e], s5);
-          atomicAdd(&s_g[6][e], s6);
-          atomicAdd(&s_g[7][e], s7);
-          atomicAdd(&s_g[8][e], s8);
-        }
+      const float dqx = dq * dx, dqy = dq * dy;
+      const float s0 = row_sum16(g.x * ta);
+      const float s1 = row_sum16(g.y * ta);
+      const float s2 = row_sum16(g.z * ta);
+      const float s3 = row_sum16(dop);
+      const float s4 = row_sum16(dqx * dx);
+      const float s5 = row_sum16(dqx * dy);
+      const float s6 = row_sum16(dqy * dy);
+      const float s7 = row_sum16(dqx);
+      const float s8 = row_sum16(dqy);
+      if (row_last) {
+        // each row its own slot (rows of other waves of a larger tile share
+        // it, hence the atomics); the slot index varies over the writing
+        // lanes, so these stay plain ds_add_f32
+        atomicAdd(&s_g[0][row][e], s0);
+        atomicAdd(&s_g[1][row][e], s1);
+        atomicAdd(&s_g[2][row][e], s2);
+        atomicAdd(&s_g[3][row][e], s3);
+        atomicAdd(&s_g[4][row][e], s4);
+        atomicAdd(&s_g[5][row][e], s5);
+        atomicAdd(&s_g[6][row][e], s6);
+        atomicAdd(&s_g[7][row][e], s7);
+        atomicAdd(&s_g[8][row][e], s8);
       }
     }
     __syncthreads();
     if (t < cnt) {
       const uint32_t k = s_k[t];
+      // conic rows (a.z, a.w) = (i00, i01) and (bb.x, bb.y) = (i10, i11):
+      // sum dq (i00 ux + i10 uy) = P sum dq dx + Q sum dq dy, and likewise
+      const float4 a = s_rec[0][t], bb = s_rec[1][t];
+      const float P = a.z * a.z + bb.x * bb.x, Q = a.z * a.w + bb.x * bb.y, R = a.w * a.w + bb.y * bb.y;
+      float u[9];
 #pragma unroll
-      for (int j = 0; j < 9; ++j) {
-        const float v = s_g[j][t];
-        if (v != 0.f) unsafeAtomicAdd(&g9[9 * (size_t)k + j], v);
-      }
+      for (int j = 0; j < 9; ++j) u[j] = (s_g[j][0][t] + s_g[j][1][t]) + (s_g[j][2][t] + s_g[j][3][t]);
+      float v[9];
+      v[0] = u[0];
+      v[1] = u[1];
+      v[2] = u[2];
+      v[3] = u[3];
+      v[4] = u[4];
+      v[5] = 2.f * u[5];
+      v[6] = u[6];
+      v[7] = -2.f * (P * u[7] + Q * u[8]);
+      v[8] = -2.f * (Q * u[7] + R * u[8]);
+#pragma unroll
+      for (int j = 0; j < 9; ++j)
+        if (v[j] != 0.f) unsafeAtomicAdd(&g9[9 * (size_t)k + j], v[j]);
     }
     hi = lo;
   }
@@ -773,7 +812,7 @@ __device__ __forceinline__ void chain_view(const FrameConst& fc, const SplatStat
 
 // One lane per scene splat: visits every view of the scene, reads the view's
 // 9 gradient words at the splat's slot (found from the forward's visibility
-// ballots and block offsets) and chains them; then writes all 59 parameter
+// ballots and per-wave offsets) and chains them; then writes all 59 parameter
 // gradients of the splat once (zeros for a splat no view touched).
 __global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(DiffLaunch L,
                                                                               const uint32_t* __restrict__ offsets,
@@ -827,13 +866,11 @@ __global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(Di
   bool any = false;
   for (uint32_t vi = sc.view_begin; vi < sc.view_end; ++vi) {
     const DiffView& view = L.views[L.scene_views[vi]];
-    const uint32_t vb = view.block_base + lb;
-    const uint64_t* mb = masks + (size_t)vb * kMaskWords;
-    const uint64_t m = mb[w];
+    const size_t wi = (size_t)(view.block_base + lb) * kMaskWords + w;
+    const uint64_t m = masks[wi];
     if (m == 0) continue;
     if (!((m >> lane) & 1ull)) continue;
-    uint32_t slot = offsets[vb] + mbcnt(m);
-    for (uint32_t j = 0; j < w; ++j) slot += (uint32_t)__popcll(mb[j]);
+    const uint32_t slot = offsets[wi] + mbcnt(m);
     const float* gk = g9 + 9 * (size_t)slot;
     float gv[9];
     bool nz = false;
@@ -894,18 +931,20 @@ uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
 
 }  // namespace
 
-hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint32_t* d_visible, hipStream_t s) {
-  if (L.n_blocks == 0) return fill_u32_launch(d_visible, 0u, 1, s);
-  diff_count_kernel<<<L.n_blocks, kCompactBlock, 0, s>>>(L, counts);
-  const ScanPlan p = scan_plan(L.n_blocks);
-  void* partials = reinterpret_cast<char*>(counts) + align_up((size_t)L.n_blocks * 4, 256);
-  return scan_exclusive_launch(p, counts, counts, nullptr, d_visible, partials, s);
+hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint64_t* masks, uint32_t* d_visible,
+                             void* scan_storage, hipStream_t s) {
+  const uint32_t waves = L.n_blocks * kMaskWords;
+  if (waves == 0) return fill_u32_launch(d_visible, 0u, 1, s);
+  diff_count_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, counts, masks);
+  return scan_exclusive_launch(scan_plan(waves), counts, counts, nullptr, d_visible, scan_storage, s);
 }
 
-hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, float4* rec, uint4* rc, uint32_t* key,
-                             uint32_t* val, uint64_t* masks, hipStream_t s) {
+size_t diff_count_scan_bytes(uint32_t n_blocks) { return scan_plan(n_blocks * kMaskWords).bytes; }
+
+hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks, float4* rec,
+                             uint4* rc, uint32_t* key, uint32_t* val, hipStream_t s) {
   if (L.n_blocks == 0) return hipSuccess;
-  diff_write_kernel<<<L.n_blocks, kCompactBlock, 0, s>>>(L, offsets, rec, rc, key, val, masks);
+  diff_write_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, rec, rc, key, val);
   return hipGetLastError();
 }
 

@@ -1,5 +1,5 @@
-// Pair emission of the binning stage, shared by the plain and the batched
-// paths (device code only).
+// Device-only helpers shared by the kernels of the plain and the batched
+// paths: the Gaussian falloff of the raster and the pair emission of binning.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,6 +8,14 @@
 #include "common.h"
 
 namespace vkgs {
+
+// exp(-q / 2) as v_exp_f32 (exp2) of q * (-log2(e) / 2).  Halving the float
+// log2(e) is exact, so this equals exp2((-q / 2) * log2(e)) bit for bit while
+// saving a multiply.
+__device__ __forceinline__ float gauss_falloff(float q) {
+  constexpr float kNegHalfLog2e = -0.5f * 1.44269504088896340736f;
+  return __builtin_amdgcn_exp2f(q * kNegHalfLog2e);
+}
 
 // Splats covering more tiles than this are emitted by a whole wave.
 constexpr uint32_t kEmitSerial = 16;

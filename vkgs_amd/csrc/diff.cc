@@ -52,7 +52,7 @@ struct vkgs_diff_ctx {
   Buf tables;
   // per-item buffers
   uint32_t cap_items = 0;
-  Buf counts, rec, rc, key, val, masks, kview, g9, misc, rect, cnt, offs, scan_storage, depth_sort;
+  Buf counts, count_scan, rec, rc, key, val, masks, kview, g9, misc, rect, cnt, offs, scan_storage, depth_sort;
   // pairs
   uint64_t max_pairs = 0;
   Buf pkeys, pvals, pair_sort, ranges;
@@ -127,7 +127,7 @@ int vkgs_diff_create(vkgs_diff_ctx** out) {
 int vkgs_diff_destroy(vkgs_diff_ctx* c) {
   if (!c) return VKGS_ERR_INVALID_ARG;
   (void)hipDeviceSynchronize();
-  Buf* bufs[] = {&c->tables, &c->counts, &c->rec,    &c->rc,        &c->key,    &c->val,
+  Buf* bufs[] = {&c->tables, &c->counts, &c->count_scan, &c->rec,    &c->rc,        &c->key,    &c->val,
                  &c->masks,  &c->kview,  &c->g9,     &c->misc,      &c->rect,   &c->cnt,
                  &c->offs,   &c->scan_storage, &c->depth_sort, &c->pkeys, &c->pvals, &c->pair_sort,
                  &c->ranges, &c->final_t, &c->last, &c->tile_keys, &c->tile_order, &c->tile_sort};
@@ -227,7 +227,8 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   c->tile_plan = sort_plan((uint32_t)(tiles ? tiles : 1));
 #define ENSURE(buf, bytes) \
   if (e == hipSuccess) e = c->buf.ensure(bytes)
-  ENSURE(counts, compact_storage_bytes(cap));
+  ENSURE(counts, (size_t)(blocks ? blocks : 1) * kMaskWords * 4);
+  ENSURE(count_scan, diff_count_scan_bytes((uint32_t)(blocks ? blocks : 1)));
   ENSURE(rec, (size_t)cap * 48);
   ENSURE(rc, (size_t)cap * 16);
   ENSURE(key, (size_t)cap * 4);
@@ -278,9 +279,10 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   uint32_t* misc = c->misc.as<uint32_t>();
   uint32_t* order = c->val.as<uint32_t>();
   VKGS_HIP_CHECK(fill_u32_launch(misc, 0u, 4, s));
-  VKGS_HIP_CHECK(launch_diff_count(L, c->counts.as<uint32_t>(), misc + kMiscVisible, s));
-  VKGS_HIP_CHECK(launch_diff_write(L, c->counts.as<uint32_t>(), c->rec.as<float4>(), c->rc.as<uint4>(),
-                                   c->key.as<uint32_t>(), order, c->masks.as<uint64_t>(), s));
+  VKGS_HIP_CHECK(launch_diff_count(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), misc + kMiscVisible,
+                                   c->count_scan.p, s));
+  VKGS_HIP_CHECK(launch_diff_write(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), c->rec.as<float4>(),
+                                   c->rc.as<uint4>(), c->key.as<uint32_t>(), order, s));
   VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, 30u,
                                 c->depth_sort.p, s));
   VKGS_HIP_CHECK(launch_diff_bin_gather(c->rc.as<uint4>(), order, misc + kMiscVisible, cap, c->rect.as<TileRect>(),

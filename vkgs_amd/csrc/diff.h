@@ -7,9 +7,10 @@
 // follows projection.comp:80-179 and the raster the reference's blend.
 //
 // Every (view, splat) pair is one item; items are grouped per view in blocks
-// of kCompactBlock so a block belongs to one view.  Visible items get slots in
-// (view, splat id) order; the raster record of a slot is written by the
-// projection.  Tiles of all views share one global id space (view tiles are
+// of kCompactBlock splats (a block belongs to one view) and waves of 64.
+// Visible items get slots in (view, splat id) order; the raster record of a
+// slot is written by the projection, which runs one lane per scene splat and
+// visits the views of its scene.  Tiles of all views share one global id space (view tiles are
 // contiguous), so sorting (tile, slot) pairs by global tile id puts the view id
 // in the high key bits.  The projection backward runs per scene splat and
 // visits the views of its scene, finding each view's slot from the forward's
@@ -68,12 +69,18 @@ struct DiffLaunch {
   float t_min;
 };
 
-hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint32_t* d_visible, hipStream_t s);
-// Per visible item at its slot: raster record rec[3 slot..], (tile rect x,
-// tile rect y, tile count, view) in rc[slot], key = bits(1 - z), val = slot;
-// masks[kMaskWords b + w] = visibility ballot of wave w of block b.
-hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, float4* rec, uint4* rc, uint32_t* key,
-                             uint32_t* val, uint64_t* masks, hipStream_t s);
+// Visibility ballots and per-wave visible counts of every (view, splat) item
+// (masks and counts hold kMaskWords words per compaction block), then the
+// counts scanned in place into slot offsets; *d_visible = total.  scan_storage
+// is diff_count_scan_bytes(n_blocks) bytes.
+hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint64_t* masks, uint32_t* d_visible,
+                             void* scan_storage, hipStream_t s);
+size_t diff_count_scan_bytes(uint32_t n_blocks);
+// Per visible item at its slot offsets[w] + (visible lanes of wave w before
+// it): raster record rec[3 slot..], (tile rect x, tile rect y, tile count,
+// view) in rc[slot], key = bits(1 - z), val = slot.
+hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks, float4* rec,
+                             uint4* rc, uint32_t* key, uint32_t* val, hipStream_t s);
 // rect[k], cnt[k], kview[k] of slot order[k].
 hipError_t launch_diff_bin_gather(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
                                   TileRect* rect, uint32_t* cnt, uint32_t* kview, hipStream_t s);

@@ -26,7 +26,7 @@
 #pragma clang fp contract(off)
 
 #include "common.h"
-#include "emit.h"
+#include "device.h"
 #include "kernels.h"
 
 namespace vkgs {
@@ -171,14 +171,16 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
     p2 = rec[3 * (size_t)k + 2];
   }
 
-  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
-  bool done = !inside;
+  // T >= t_min is the pixel's "still compositing" state: T only decreases,
+  // and pixels outside the image start at T = 0 (their colour is never
+  // stored).
+  float T = inside ? 1.f : 0.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
   for (uint32_t b = 0; b < batches; ++b) {
     const int buf = b & 1;
     s_rec[buf][0][t] = p0;
     s_rec[buf][1][t] = p1;
     s_rec[buf][2][t] = p2;
-    if (__syncthreads_count(!done) == 0) break;
+    if (__syncthreads_count(T >= t_min) == 0) break;
     // gather the next batch while this one is composited
     const uint32_t next = (b + 1) * kN + t;
     if (next < items) {
@@ -187,16 +189,14 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
       p1 = rec[3 * (size_t)k + 1];
       p2 = rec[3 * (size_t)k + 2];
     }
-    const int cnt = (int)min((uint32_t)kN, items - b * kN);
-    if (__ballot(!done) != 0) {
+    const int cnt = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kN, items - b * kN));
+    if (__ballot(T >= t_min) != 0) {
       const float4* s0 = s_rec[buf][0];
       const float4* s1 = s_rec[buf][1];
       const float4* s2 = s_rec[buf][2];
-      float4 a = s0[0], bb = s1[0], c = s2[0];
+      bool alive = T >= t_min;
       for (int e = 0; e < cnt; ++e) {
-        // LDS reads of the next record overlap this record's arithmetic
-        const int en = e + 1 < cnt ? e + 1 : e;
-        const float4 na = s0[en], nb = s1[en], nc = s2[en];
+        const float4 a = s0[e], bb = s1[e], c = s2[e];
         bool touches = true;
         if (!kOneWave) {
           const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
@@ -209,21 +209,18 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
           const float dx = fx - a.x, dy = fy - a.y;
           const float ux = fmaf(a.z, dx, a.w * dy);
           const float uy = fmaf(bb.x, dx, bb.y * dy);
-          const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && !done;
+          const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && alive;
           const float q = fmaf(ux, ux, uy * uy);
-          const float alpha = in ? bb.z * __expf(-0.5f * q) : 0.f;
+          const float alpha = in ? bb.z * gauss_falloff(q) : 0.f;
           const float ta = T * alpha;
           cr = fmaf(c.x, ta, cr);
           cg = fmaf(c.y, ta, cg);
           cb = fmaf(c.z, ta, cb);
           aa = fmaf(alpha, ta, aa);
           T = T - ta;
-          done = done || T < t_min;
-          if (__ballot(!done) == 0) break;
+          alive = T >= t_min;
+          if (__ballot(alive) == 0) break;
         }
-        a = na;
-        bb = nb;
-        c = nc;
       }
     }
   }
@@ -272,7 +269,7 @@ __global__ __launch_bounds__(TW* TH) void rasterize_wave_kernel(const float4* __
         const float uy = fmaf(bb.x, dx, bb.y * dy);
         const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && !done;
         const float q = fmaf(ux, ux, uy * uy);
-        const float alpha = in ? bb.z * __expf(-0.5f * q) : 0.f;
+        const float alpha = in ? bb.z * gauss_falloff(q) : 0.f;
         const float ta = T * alpha;
         cr = fmaf(c.x, ta, cr);
         cg = fmaf(c.y, ta, cg);

@@ -3,9 +3,9 @@
 // Replaces the reference's external sorter (vrdxCmdSortKeyValueIndirect,
 // engine.cc:1218-1219; "reduce-then-scan", DETAILS.md:26-30).  Each pass of
 // up to 8 key bits runs three kernels:
-//   upsweep   : every block counts the digits of its contiguous chunk into a
-//               per-wave LDS histogram (one LDS atomic per distinct digit per
-//               64 keys, found with 8 wave ballots), then writes hist[d][block];
+//   upsweep   : every block counts the digits of its contiguous chunk with
+//               one LDS atomic per key into lane-interleaved copies of a
+//               per-wave histogram, then writes hist[d][block];
 //   scan      : one workgroup turns hist into exclusive global offsets;
 //   downsweep : every block walks its chunk in 2048-key sub-tiles, ranks keys
 //               stably (wave multisplit: ballots + LDS counters, then a
@@ -26,22 +26,32 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
 constexpr int kIpt = 8;
 constexpr int kTile = kThreads * kIpt;  // keys per sub-tile
-constexpr int kMaxBlocks = 1024;
+#ifndef VKGS_SORT_MAX_BLOCKS
+#define VKGS_SORT_MAX_BLOCKS 1024
+#endif
+constexpr int kMaxBlocks = VKGS_SORT_MAX_BLOCKS;
 constexpr int kScanThreads = 1024;
 
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Mask of the lanes whose `bits`-bit digit equals this lane's (valid lanes only).
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint32_t bits, bool valid) {
-  uint64_t m = __ballot(valid);
-  for (uint32_t b = 0; b < bits; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bb = __ballot(bit);
-    m &= bit ? bb : ~bb;
+// Mask of the lanes whose kBits-bit digit equals this lane's (valid lanes
+// only): per bit one ballot and m &= ~(ballot ^ this lane's bit broadcast).
+template <int kBits>
+__device__ __forceinline__ uint64_t match_digit_t(uint32_t d, bool valid) {
+  const uint64_t m0 = __ballot(valid);
+  uint32_t lo = (uint32_t)m0, hi = (uint32_t)(m0 >> 32);
+#pragma unroll
+  for (int b = 0; b < kBits; ++b) {
+    const uint32_t sel = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // bit b of d, all ones or zero
+    const uint64_t bb = __ballot(sel != 0u);
+    // m &= ~(bb ^ sel): v_bitop3 with m as the middle operand and the
+    // symmetric bb / sel on either side, truth table 0x84
+    lo = __builtin_amdgcn_bitop3_b32((uint32_t)bb, lo, sel, 0x84);
+    hi = __builtin_amdgcn_bitop3_b32((uint32_t)(bb >> 32), hi, sel, 0x84);
   }
-  return m;
+  return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
@@ -88,9 +98,14 @@ __global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restr
                                                          const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                          uint32_t shift, uint32_t bits,
                                                          uint32_t* __restrict__ hist) {
-  __shared__ uint32_t s_h[kWaves][256];
-  const uint32_t tid = threadIdx.x, w = tid / kWave;
-  for (uint32_t i = tid; i < kWaves * 256; i += kThreads) (&s_h[0][0])[i] = 0u;
+  // a histogram needs no ranking: one LDS atomic per key, into one of kCopies
+  // copies per wave chosen by lane, so lanes with equal digits collide at
+  // most kWave / kCopies ways
+  constexpr int kCopies = 4;
+  __shared__ uint32_t s_h[kWaves * kCopies][256];
+  const uint32_t tid = threadIdx.x;
+  uint32_t* h = s_h[(tid / kWave) * kCopies + (tid % kCopies)];
+  for (uint32_t i = tid; i < kWaves * kCopies * 256; i += kThreads) (&s_h[0][0])[i] = 0u;
   __syncthreads();
   const uint32_t n = min(*d_count, max_n);
   uint32_t begin, end;
@@ -104,18 +119,14 @@ __global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restr
       k[r] = i < end ? keys[i] : 0u;
     }
 #pragma unroll
-    for (int r = 0; r < kIpt; ++r) {
-      const bool valid = base + r * kThreads + tid < end;
-      const uint32_t d = (k[r] >> shift) & mask;
-      const uint64_t peers = match_digit(d, bits, valid);
-      if (valid && mbcnt(peers) == 0) atomicAdd(&s_h[w][d], (uint32_t)__popcll(peers));
-    }
+    for (int r = 0; r < kIpt; ++r)
+      if (base + r * kThreads + tid < end) atomicAdd(&h[(k[r] >> shift) & mask], 1u);
   }
   __syncthreads();
   if (tid <= mask) {
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < kWaves; ++i) s += s_h[i][tid];
+    for (int i = 0; i < kWaves * kCopies; ++i) s += s_h[i][tid];
     hist[tid * gridDim.x + blockIdx.x] = s;
   }
 }
@@ -147,13 +158,14 @@ __global__ __launch_bounds__(kThreads) void sort_scan_rows(uint32_t* __restrict_
   if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
 }
 
+template <int kBits>
 __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                           uint32_t shift, uint32_t bits,
-                                                           const uint32_t* __restrict__ hist,
+                                                           uint32_t shift, const uint32_t* __restrict__ hist,
                                                            const uint32_t* __restrict__ digit_total) {
+  constexpr uint32_t bits = kBits;
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_wcnt[kWaves][256];
   __shared__ uint32_t s_start[256];
@@ -188,17 +200,21 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
       k[r] = valid ? kin[i] : 0u;
       v[r] = valid ? vin[i] : 0u;
     }
-    // stable rank inside this wave's segment, 64 keys at a time
+    // stable rank inside this wave's segment, 64 keys at a time.  The wave
+    // owns its counter row, so no atomics: every lane reads its digit's count,
+    // then the first lane of each digit group writes count + group size.  LDS
+    // operations of one wave complete in issue order, so the reads of a row
+    // see the writes of the rows before it and none of its own.
 #pragma unroll
     for (int r = 0; r < kIpt; ++r) {
       const bool valid = seg + r * kWave + lane < end;
       const uint32_t d = (k[r] >> shift) & mask;
-      const uint64_t peers = match_digit(d, bits, valid);
+      const uint64_t peers = match_digit_t<kBits>(d, valid);
       const uint32_t below = mbcnt(peers);
-      uint32_t old = 0;
-      if (valid && below == 0) old = atomicAdd(&s_wcnt[w][d], (uint32_t)__popcll(peers));
-      const int leader = peers ? (int)__ffsll((unsigned long long)peers) - 1 : (int)lane;
-      old = __shfl(old, leader, kWave);
+      const uint32_t old = valid ? s_wcnt[w][d] : 0u;
+      __builtin_amdgcn_wave_barrier();
+      if (valid && below == 0) s_wcnt[w][d] = old + (uint32_t)__popcll(peers);
+      __builtin_amdgcn_wave_barrier();
       rank[r] = old + below;
     }
     __syncthreads();
@@ -364,8 +380,22 @@ hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_
     const uint32_t bits = key_bits - shift < 8 ? key_bits - shift : 8;
     sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
     sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total);
-    sort_downsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, plan.max_n, shift,
-                                                         bits, hist, digit_total);
+    switch (bits) {
+#define VKGS_DOWNSWEEP(B)                                                                                     \
+  case B:                                                                                                     \
+    sort_downsweep<B><<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, plan.max_n, \
+                                                            shift, hist, digit_total);                        \
+    break;
+      VKGS_DOWNSWEEP(1)
+      VKGS_DOWNSWEEP(2)
+      VKGS_DOWNSWEEP(3)
+      VKGS_DOWNSWEEP(4)
+      VKGS_DOWNSWEEP(5)
+      VKGS_DOWNSWEEP(6)
+      VKGS_DOWNSWEEP(7)
+      VKGS_DOWNSWEEP(8)
+#undef VKGS_DOWNSWEEP
+    }
     uint32_t* tk = src_k;
     uint32_t* tv = src_v;
     src_k = dst_k;
