@@ -30,6 +30,10 @@ namespace {
 
 constexpr float kAlphaMax = 0.999f;
 
+#ifndef VKGS_BWD_BATCH
+#define VKGS_BWD_BATCH 32  // list entries per batch of the one-wave raster backward
+#endif
+
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
@@ -314,6 +318,9 @@ __global__ __launch_bounds__(256) void diff_bin_gather_kernel(const uint4* __res
   }
 }
 
+// Pairs (global tile, pair index) with pslot[pair index] = slot: the sort
+// carries the pair's emission index, so the backward can map each sorted
+// position back to its splat.
 __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const TileRect* __restrict__ rect,
                                                             const uint32_t* __restrict__ cnt,
                                                             const uint32_t* __restrict__ offs,
@@ -321,7 +328,7 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
                                                             const uint32_t* __restrict__ order,
                                                             const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                             uint64_t max_pairs, uint32_t* __restrict__ keys,
-                                                            uint32_t* __restrict__ vals,
+                                                            uint32_t* __restrict__ vals, uint32_t* __restrict__ pslot,
                                                             uint32_t* __restrict__ status) {
   const uint32_t n = min(*d_count, max_n);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
         if (o + c > max_pairs) atomicOr(status, 1u);
       }
     }
-    emit_pairs_wave(c, o, r, slot, key_base, tiles_x, max_pairs, keys, vals);
+    emit_pairs_wave<true>(c, o, r, slot, key_base, tiles_x, max_pairs, keys, vals, pslot);
   }
 }
 
@@ -354,6 +361,7 @@ template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                              const uint32_t* __restrict__ vals,
                                                              const uint32_t* __restrict__ ranges,
+                                                             const uint32_t* __restrict__ pslot,
                                                              const uint32_t* __restrict__ tile_order, int keep_state) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
@@ -378,7 +386,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
 
   float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
   if ((uint32_t)t < items) {
-    const uint32_t k = vals[end - 1 - t];
+    const uint32_t k = pslot[vals[end - 1 - t]];
     p0 = rec[3 * (size_t)k];
     p1 = rec[3 * (size_t)k + 1];
     p2 = rec[3 * (size_t)k + 2];
@@ -394,7 +402,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
     if (__syncthreads_count(!done) == 0) break;
     const uint32_t next = (b + 1) * kN + t;
     if (next < items) {
-      const uint32_t k = vals[end - 1 - next];
+      const uint32_t k = pslot[vals[end - 1 - next]];
       p0 = rec[3 * (size_t)k];
       p1 = rec[3 * (size_t)k + 1];
       p2 = rec[3 * (size_t)k + 2];
@@ -442,18 +450,6 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
 
 // -------------------------------------------------------------- backward ---
 
-__global__ __launch_bounds__(256) void diff_zero_kernel(const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                        float* __restrict__ g) {
-  const size_t n = 9 * (size_t)min(*d_count, max_n);
-  const size_t n4 = n / 4;
-  float4* g4 = reinterpret_cast<float4*>(g);
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += stride)
-    g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  const size_t i = 4 * n4 + blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i < n) g[i] = 0.f;
-}
-
 template <int kCtrl>
 __device__ __forceinline__ float dpp_add(float x) {
   return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xf, 0xf, false));
@@ -479,13 +475,20 @@ template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                                       const uint32_t* __restrict__ vals,
                                                                       const uint32_t* __restrict__ ranges,
+                                                                      const uint32_t* __restrict__ pslot,
                                                                       const uint32_t* __restrict__ tile_order,
-                                                                      float* __restrict__ g9) {
+                                                                      float* __restrict__ gpart) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
-  __shared__ float4 s_rec[3][kN];
-  __shared__ float s_g[9][4][kN];  // per word, one partial sum per DPP row of the wave(s)
-  __shared__ uint32_t s_k[kN];
+  constexpr bool kOneWave = kN == kWave;  // the wave covers the tile: every listed record touches it
+  // list entries per batch: one per thread for multi-wave tiles; a one-wave
+  // tile takes fewer, so the LDS footprint (records + partial sums) admits
+  // more resident waves
+  constexpr int kB = kOneWave ? VKGS_BWD_BATCH : kN;
+  static_assert(kB <= kN, "batch larger than the workgroup");
+  __shared__ float4 s_rec[3][kB];
+  __shared__ float s_g[9][4][kB + 1];  // per word, one partial sum per DPP row; +1 spreads the rows over banks
+  __shared__ uint32_t s_j[kB];         // emission index of each batch entry
   __shared__ uint32_t s_max;
 
   const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
@@ -522,13 +525,14 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
 
   // traversal index i (0 = nearest) maps to list position end - 1 - i; walk i downwards
   for (uint32_t hi = max_last; hi > 0;) {
-    const uint32_t lo = hi > (uint32_t)kN ? hi - kN : 0u;
+    const uint32_t lo = hi > (uint32_t)kB ? hi - kB : 0u;
     const int cnt = (int)(hi - lo);
     __syncthreads();
     if (t < cnt) {
       const uint32_t i = hi - 1 - t;
-      const uint32_t k = vals[end - 1 - i];
-      s_k[t] = k;
+      const uint32_t j = vals[end - 1 - i];
+      const uint32_t k = pslot[j];
+      s_j[t] = j;
       s_rec[0][t] = rec[3 * (size_t)k];
       s_rec[1][t] = rec[3 * (size_t)k + 1];
       s_rec[2][t] = rec[3 * (size_t)k + 2];
@@ -543,11 +547,13 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const float4 a = s_rec[0][e];
       const float4 bb = s_rec[1][e];
       const float4 c = s_rec[2][e];
-      const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
-      const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
-      const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
-      const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
-      if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
+      if (!kOneWave) {
+        const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
+        const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
+        const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
+        const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
+        if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
+      }
       const float dx = fx - a.x, dy = fy - a.y;
       const float ux = fmaf(a.z, dx, a.w * dy);
       const float uy = fmaf(bb.x, dx, bb.y * dy);
@@ -598,7 +604,6 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
     }
     __syncthreads();
     if (t < cnt) {
-      const uint32_t k = s_k[t];
       // conic rows (a.z, a.w) = (i00, i01) and (bb.x, bb.y) = (i10, i11):
       // sum dq (i00 ux + i10 uy) = P sum dq dx + Q sum dq dy, and likewise
       const float4 a = s_rec[0][t], bb = s_rec[1][t];
@@ -606,21 +611,76 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       float u[9];
 #pragma unroll
       for (int j = 0; j < 9; ++j) u[j] = (s_g[j][0][t] + s_g[j][1][t]) + (s_g[j][2][t] + s_g[j][3][t]);
-      float v[9];
-      v[0] = u[0];
-      v[1] = u[1];
-      v[2] = u[2];
-      v[3] = u[3];
-      v[4] = u[4];
-      v[5] = 2.f * u[5];
-      v[6] = u[6];
-      v[7] = -2.f * (P * u[7] + Q * u[8]);
-      v[8] = -2.f * (Q * u[7] + R * u[8]);
-#pragma unroll
-      for (int j = 0; j < 9; ++j)
-        if (v[j] != 0.f) unsafeAtomicAdd(&g9[9 * (size_t)k + j], v[j]);
+      float* out = gpart + 9 * (size_t)s_j[t];  // at the pair's emission index
+      out[0] = u[0];
+      out[1] = u[1];
+      out[2] = u[2];
+      out[3] = u[3];
+      out[4] = u[4];
+      out[5] = 2.f * u[5];
+      out[6] = u[6];
+      out[7] = -2.f * (P * u[7] + Q * u[8]);
+      out[8] = -2.f * (Q * u[7] + R * u[8]);
     }
     hi = lo;
+  }
+}
+
+__global__ __launch_bounds__(256) void diff_fill_zero_kernel(float* __restrict__ g, uint64_t n) {
+  const uint64_t n4 = n / 4;
+  float4* g4 = reinterpret_cast<float4*>(g);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n4; i += stride)
+    g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint64_t i = 4 * n4 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i < n) g[i] = 0.f;
+}
+
+// g9[slot] = sum of the partials of the slot's pairs.  The pairs of a splat
+// are contiguous in emission order, so each 256-pair block loads its partials
+// coalesced, and the first thread of every run of equal slots sums the run
+// from LDS.  A run that reaches a block edge may continue in the next block:
+// those add atomically (g9 starts at zero), interior runs store.
+__global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* __restrict__ pslot,
+                                                               const float* __restrict__ gpart,
+                                                               const uint32_t* __restrict__ d_pairs,
+                                                               uint64_t max_pairs, float* __restrict__ g9) {
+  __shared__ float s_v[9][257];
+  __shared__ uint32_t s_slot[256];
+  const uint64_t n = min((uint64_t)*d_pairs, max_pairs);
+  const uint32_t t = threadIdx.x;
+  for (uint64_t base = blockIdx.x * 256ull; base < n; base += (uint64_t)gridDim.x * 256) {
+    const uint64_t j = base + t;
+    const bool valid = j < n;
+    uint32_t slot = 0xffffffffu;
+    if (valid) {
+      slot = pslot[j];
+      const float* src = gpart + 9 * j;
+#pragma unroll
+      for (int w = 0; w < 9; ++w) s_v[w][t] = src[w];
+    }
+    s_slot[t] = slot;
+    __syncthreads();
+    if (valid && (t == 0 || s_slot[t - 1] != slot)) {
+      float acc[9];
+#pragma unroll
+      for (int w = 0; w < 9; ++w) acc[w] = s_v[w][t];
+      uint32_t u = t + 1;
+      for (; u < 256 && s_slot[u] == slot; ++u)
+#pragma unroll
+        for (int w = 0; w < 9; ++w) acc[w] += s_v[w][u];
+      const bool at_edge = (t == 0 && j > 0 && pslot[j - 1] == slot) ||
+                           (u == 256 && base + 256 < n && pslot[base + 256] == slot);
+      float* dst = g9 + 9 * (size_t)slot;
+      if (at_edge) {
+#pragma unroll
+        for (int w = 0; w < 9; ++w) unsafeAtomicAdd(&dst[w], acc[w]);
+      } else {
+#pragma unroll
+        for (int w = 0; w < 9; ++w) dst[w] = acc[w];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -957,11 +1017,11 @@ hipError_t launch_diff_bin_gather(const uint4* rc, const uint32_t* order, const 
 
 hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const uint32_t* cnt, const uint32_t* offs,
                                 const uint32_t* kview, const uint32_t* order, const uint32_t* d_count,
-                                uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* status,
-                                hipStream_t s) {
+                                uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* pslot,
+                                uint32_t* status, hipStream_t s) {
   if (max_n == 0) return hipSuccess;
   diff_bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, rect, cnt, offs, kview, order, d_count, max_n,
-                                                                  max_pairs, keys, vals, status);
+                                                                  max_pairs, keys, vals, pslot, status);
   return hipGetLastError();
 }
 
@@ -976,12 +1036,13 @@ hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const
   MACRO(64, 4)
 
 hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
-                              const uint32_t* tile_order, bool keep_state, hipStream_t s) {
+                              const uint32_t* pslot, const uint32_t* tile_order, bool keep_state, hipStream_t s) {
   if (L.n_tiles == 0) return hipSuccess;
-#define VKGS_CASE(W_, H_)                                                                                         \
-  if (L.tile_w == W_ && L.tile_h == H_) {                                                                         \
-    diff_raster_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, tile_order, keep_state ? 1 : 0); \
-    return hipGetLastError();                                                                                     \
+#define VKGS_CASE(W_, H_)                                                                                  \
+  if (L.tile_w == W_ && L.tile_h == H_) {                                                                  \
+    diff_raster_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, pslot, tile_order,      \
+                                                              keep_state ? 1 : 0);                         \
+    return hipGetLastError();                                                                              \
   }
   VKGS_DIFF_CASES(VKGS_CASE)
 #undef VKGS_CASE
@@ -989,18 +1050,28 @@ hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint
 }
 
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
-                                       const uint32_t* ranges, const uint32_t* tile_order, const uint32_t* d_count,
-                                       uint32_t max_n, float* g9, hipStream_t s) {
-  if (max_n > 0) diff_zero_kernel<<<grid_for((uint64_t)max_n * 9 / 4 + 1, 256, 8192), 256, 0, s>>>(d_count, max_n, g9);
-  if (L.n_tiles == 0) return hipGetLastError();
-#define VKGS_CASE(W_, H_)                                                                          \
-  if (L.tile_w == W_ && L.tile_h == H_) {                                                          \
-    diff_raster_backward_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, tile_order, g9); \
-    return hipGetLastError();                                                                      \
+                                       const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
+                                       const uint32_t* d_pairs, uint64_t pairs, uint32_t max_slots, float* gpart,
+                                       float* g9, hipStream_t s) {
+  // partials of pairs no pixel reached stay zero; slots without pairs keep zero words
+  if (pairs > 0) diff_fill_zero_kernel<<<grid_for(pairs * 9 / 4 + 1, 256, 8192), 256, 0, s>>>(gpart, pairs * 9);
+  if (max_slots > 0)
+    diff_fill_zero_kernel<<<grid_for((uint64_t)max_slots * 9 / 4 + 1, 256, 8192), 256, 0, s>>>(g9, (uint64_t)max_slots * 9);
+  if (L.n_tiles > 0) {
+    bool launched = false;
+#define VKGS_CASE(W_, H_)                                                                                  \
+  if (!launched && L.tile_w == W_ && L.tile_h == H_) {                                                     \
+    diff_raster_backward_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, pslot,         \
+                                                                       tile_order, gpart);                 \
+    launched = true;                                                                                       \
   }
-  VKGS_DIFF_CASES(VKGS_CASE)
+    VKGS_DIFF_CASES(VKGS_CASE)
 #undef VKGS_CASE
-  return hipErrorInvalidValue;
+    if (!launched) return hipErrorInvalidValue;
+  }
+  if (pairs > 0)
+    diff_grad_reduce_kernel<<<grid_for(pairs, 256, 16384), 256, 0, s>>>(pslot, gpart, d_pairs, pairs, g9);
+  return hipGetLastError();
 }
 
 hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,

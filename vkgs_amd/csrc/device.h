@@ -26,14 +26,17 @@ __device__ __forceinline__ uint32_t lane_u32(uint32_t x, uint32_t lane) {
 
 // Emits the pairs of one splat per lane: c tiles of rectangle r (tile x in
 // r.x's low/high 16 bits, tile y in r.y's) as keys key_base + ty * tiles_x + tx
-// with value val, at o, o + 1, ... in row-major tile order; pairs at or beyond
-// max_pairs are dropped (the caller flags the overflow).  Lanes with up to
-// kEmitSerial tiles write their own pairs; each larger splat is then written by
-// all 64 lanes together, so one near-camera splat does not serialize a wave.
-// Every lane of the wave must call this (c = 0 for lanes without a splat).
+// at o, o + 1, ... in row-major tile order; pairs at or beyond max_pairs are
+// dropped (the caller flags the overflow).  The value of pair j is val, or,
+// with kIndexValues, j itself with val stored in slots[j] (so a later pass can
+// find where the sort moved each pair).  Lanes with up to kEmitSerial tiles
+// write their own pairs; each larger splat is then written by all 64 lanes
+// together, so one near-camera splat does not serialize a wave.  Every lane of
+// the wave must call this (c = 0 for lanes without a splat).
+template <bool kIndexValues = false>
 __device__ __forceinline__ void emit_pairs_wave(uint32_t c, uint64_t o, TileRect r, uint32_t val, uint32_t key_base,
                                                 uint32_t tiles_x, uint64_t max_pairs, uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
+                                                uint32_t* __restrict__ vals, uint32_t* __restrict__ slots = nullptr) {
   const bool big = c > kEmitSerial;
   if (c != 0 && !big) {
     const uint32_t tx0 = r.x & 0xffffu, tx1 = r.x >> 16, ty0 = r.y & 0xffffu, ty1 = r.y >> 16;
@@ -42,7 +45,12 @@ __device__ __forceinline__ void emit_pairs_wave(uint32_t c, uint64_t o, TileRect
       for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j) {
         if (j < max_pairs) {
           keys[j] = key_base + ty * tiles_x + tx;
-          vals[j] = val;
+          if (kIndexValues) {
+            vals[j] = (uint32_t)j;
+            slots[j] = val;
+          } else {
+            vals[j] = val;
+          }
         }
       }
   }
@@ -61,7 +69,12 @@ __device__ __forceinline__ void emit_pairs_wave(uint32_t c, uint64_t o, TileRect
       const uint64_t j = bo + i;
       if (j < max_pairs) {
         keys[j] = first + row * btx + col;
-        vals[j] = bval;
+        if (kIndexValues) {
+          vals[j] = (uint32_t)j;
+          slots[j] = bval;
+        } else {
+          vals[j] = bval;
+        }
       }
     }
   }

@@ -55,7 +55,7 @@ struct vkgs_diff_ctx {
   Buf counts, count_scan, rec, rc, key, val, masks, kview, g9, misc, rect, cnt, offs, scan_storage, depth_sort;
   // pairs
   uint64_t max_pairs = 0;
-  Buf pkeys, pvals, pair_sort, ranges;
+  Buf pkeys, pvals, pslot, gpart, pair_sort, ranges;
   // longest-list-first tile schedule
   Buf tile_keys, tile_order, tile_sort;
   // per-pixel state
@@ -130,7 +130,8 @@ int vkgs_diff_destroy(vkgs_diff_ctx* c) {
   Buf* bufs[] = {&c->tables, &c->counts, &c->count_scan, &c->rec,    &c->rc,        &c->key,    &c->val,
                  &c->masks,  &c->kview,  &c->g9,     &c->misc,      &c->rect,   &c->cnt,
                  &c->offs,   &c->scan_storage, &c->depth_sort, &c->pkeys, &c->pvals, &c->pair_sort,
-                 &c->ranges, &c->final_t, &c->last, &c->tile_keys, &c->tile_order, &c->tile_sort};
+                 &c->ranges, &c->final_t, &c->last, &c->tile_keys, &c->tile_order, &c->tile_sort,
+                 &c->pslot, &c->gpart};
   for (Buf* b : bufs) b->release();
   delete c;
   return VKGS_OK;
@@ -244,6 +245,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   ENSURE(depth_sort, c->depth_plan.bytes);
   ENSURE(pkeys, (size_t)c->max_pairs * 4);
   ENSURE(pvals, (size_t)c->max_pairs * 4);
+  ENSURE(pslot, (size_t)c->max_pairs * 4);
   ENSURE(pair_sort, c->pair_plan.bytes);
   ENSURE(ranges, (size_t)(tiles ? tiles : 1) * 8);
   ENSURE(tile_keys, (size_t)(tiles ? tiles : 1) * 4);
@@ -291,7 +293,8 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
                                        misc + kMiscPairs, c->scan_storage.p, s));
   VKGS_HIP_CHECK(launch_diff_bin_emit(L, c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),
                                       c->kview.as<uint32_t>(), order, misc + kMiscVisible, cap, c->max_pairs,
-                                      c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(), misc + kMiscStatus, s));
+                                      c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(), c->pslot.as<uint32_t>(),
+                                      misc + kMiscStatus, s));
   VKGS_HIP_CHECK(sort_kv_launch(c->pair_plan, misc + kMiscPairs, c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(),
                                 (uint32_t)bits_for((uint32_t)tiles), c->pair_sort.p, s));
   VKGS_HIP_CHECK(launch_ranges(c->pkeys.as<uint32_t>(), misc + kMiscPairs, c->max_pairs, (uint32_t)tiles,
@@ -300,7 +303,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   VKGS_HIP_CHECK(launch_tile_order(c->ranges.as<uint32_t>(), (uint32_t)tiles, c->tile_keys.as<uint32_t>(),
                                    c->tile_order.as<uint32_t>(), misc + kMiscTiles, c->tile_plan, c->tile_sort.p, s));
   VKGS_HIP_CHECK(launch_diff_raster(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(), c->ranges.as<uint32_t>(),
-                                    c->tile_order.as<uint32_t>(), true, s));
+                                    c->pslot.as<uint32_t>(), c->tile_order.as<uint32_t>(), true, s));
   c->have_forward = true;
   return VKGS_OK;
 }
@@ -330,9 +333,16 @@ int vkgs_diff_backward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t
   VKGS_HIP_CHECK(upload_tables(c, s));
   const DiffLaunch& L = c->launch;
   uint32_t* misc = c->misc.as<uint32_t>();
-  VKGS_HIP_CHECK(launch_diff_raster_backward(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(), c->ranges.as<uint32_t>(),
-                                             c->tile_order.as<uint32_t>(), misc + kMiscVisible, c->cap_items,
-                                             c->g9.as<float>(), s));
+  // per-pair partials: sized by the live pair count of the forward
+  uint32_t m[4];
+  VKGS_HIP_CHECK(hipMemcpy(m, misc, sizeof(m), hipMemcpyDeviceToHost));
+  const uint64_t pairs = m[kMiscPairs] < c->max_pairs ? m[kMiscPairs] : c->max_pairs;
+  if (c->gpart.ensure((size_t)(pairs ? pairs : 1) * 36) != hipSuccess) return VKGS_ERR_HIP;
+  const uint32_t visible = m[kMiscVisible] < c->cap_items ? m[kMiscVisible] : c->cap_items;
+  VKGS_HIP_CHECK(launch_diff_raster_backward(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(),
+                                             c->ranges.as<uint32_t>(), c->pslot.as<uint32_t>(),
+                                             c->tile_order.as<uint32_t>(), misc + kMiscPairs, pairs, visible,
+                                             c->gpart.as<float>(), c->g9.as<float>(), s));
   VKGS_HIP_CHECK(launch_diff_project_backward(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(),
                                               c->g9.as<float>(), s));
   return VKGS_OK;

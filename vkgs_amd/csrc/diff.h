@@ -84,19 +84,26 @@ hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, const
 // rect[k], cnt[k], kview[k] of slot order[k].
 hipError_t launch_diff_bin_gather(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
                                   TileRect* rect, uint32_t* cnt, uint32_t* kview, hipStream_t s);
-// (global tile, slot) pairs at offs[k]; pairs beyond max_pairs set *status.
+// (global tile, pair index) pairs at offs[k] with pslot[pair index] = slot
+// order[k]; pairs beyond max_pairs set *status.
 hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const uint32_t* cnt, const uint32_t* offs,
                                 const uint32_t* kview, const uint32_t* order, const uint32_t* d_count,
-                                uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* status,
-                                hipStream_t s);
-// tile_order (optional): workgroup i rasterizes global tile tile_order[i].
+                                uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* pslot,
+                                uint32_t* status, hipStream_t s);
+// Raster of the tile-sorted pairs (vals = pair indices, pslot maps them to
+// slots); tile_order (optional): workgroup i rasterizes global tile
+// tile_order[i].
 hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
-                              const uint32_t* tile_order, bool keep_state, hipStream_t s);
-// Zeroes g9 (9 words per slot < *d_count), then accumulates the per-slot
-// gradient words of the blend.
+                              const uint32_t* pslot, const uint32_t* tile_order, bool keep_state, hipStream_t s);
+// Blend backward: per sorted pair 9 partial gradient words written at the
+// pair's emission index in gpart (9 floats per pair, pairs = live pair count),
+// then g9[slot] = the sum over the slot's pairs; g9 (max_slots slots) and
+// gpart are zeroed first.  Global atomics only where a slot's run of pairs
+// crosses a 256-pair block edge.
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
-                                       const uint32_t* ranges, const uint32_t* tile_order, const uint32_t* d_count,
-                                       uint32_t max_n, float* g9, hipStream_t s);
+                                       const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
+                                       const uint32_t* d_pairs, uint64_t pairs, uint32_t max_slots, float* gpart,
+                                       float* g9, hipStream_t s);
 // Writes (overwrites) every parameter gradient of every scene.
 hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
                                         const float* g9, hipStream_t s);
