@@ -51,12 +51,15 @@ def parse_args(argv=None):
     return ap.parse_args(argv)
 
 
-def load_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if present."""
+def load_traffic(kernel: str, workload: str, tile: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, if one exists for
+    this workload and tile shape."""
     path = os.path.join(HERE, "profiles", f"traffic_{workload}.json")
     try:
         with open(path) as f:
             data = json.load(f)
+        if data.get("tile", "8x8") != tile:
+            return None
         return data["kernels"][kernel]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
@@ -123,6 +126,9 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
         dt = (time.perf_counter() - t0) / steps
         out[f"{name}_views_per_s"] = round(len(views) / dt, 2)
         out[f"{name}_ms_per_view"] = round(dt * 1e3 / len(views), 3)
+        if name == "batched":
+            out["batched_visible_items"] = D.last_info["visible"]
+            out["batched_pairs"] = D.last_info["pairs"]
     if mode == "both":
         out["batched_over_plain"] = round(out["batched_views_per_s"] / out["plain_views_per_s"], 3)
     out["views"] = len(views)
@@ -249,16 +255,16 @@ def main(argv=None) -> int:
         # vector fp32 work (no stage is a dense contraction, MFMA is unused)
         roof = {"kernel": raster_name, "bound": "valu", "achieved": round(raster_tflops, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(raster_tflops / FP32_PEAK_TFLOPS, 4),
-                "traffic": load_traffic(raster_name, wl.name), "avg_ms": stage_ms["raster"],
+                "traffic": load_traffic(raster_name, wl.name, f"{tw}x{th}"), "avg_ms": stage_ms["raster"],
                 "work": f"{RASTER_FLOPS_PER_EVAL} flop x {p_eval:.4g} (pixel, splat) evaluations"}
     else:
         roof = {"kernel": "fused_write_kernel", "bound": "hbm", "achieved": round(proj_gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic("fused_write_kernel", wl.name),
+                "traffic": load_traffic("fused_write_kernel", wl.name, f"{tw}x{th}"),
                 "avg_ms": stage_ms["project"], "work": f"12N + 172V = {b_proj:.4g} B"}
     roof_proj = {"kernel": "fused_write_kernel", "bound": "hbm", "achieved": round(proj_gbs, 1),
                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4),
-                 "traffic": load_traffic("fused_write_kernel", wl.name), "avg_ms": stage_ms["project"],
+                 "traffic": load_traffic("fused_write_kernel", wl.name, f"{tw}x{th}"), "avg_ms": stage_ms["project"],
                  "work": f"12N + 172V = {b_proj:.4g} B"}
 
     result = {

@@ -219,9 +219,10 @@ int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const 
 int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const uint32_t** d_list_ids,
                              uint32_t* tiles_x, uint32_t* tiles_y);
 /* Autotuner: renders `frames` graph replays of this camera for each tile shape
- * (8x8, 16x8, 8x16, 16x4, 16x16, 32x4), times them with HIP events and keeps
- * the fastest (t_min unchanged).  Synchronous.  The chosen config and its
- * ms/frame are returned when the pointers are non-NULL. */
+ * (8x8, 16x8, 8x16, 16x4, 16x16, 32x4), three rounds visiting the shapes in
+ * turn, times them with HIP events and keeps the shape with the fastest round
+ * (t_min unchanged).  Synchronous.  The chosen config and its ms/frame are
+ * returned when the pointers are non-NULL. */
 int vkgs_renderer_autotune(vkgs_renderer* r, const vkgs_camera* camera, const float model[16], float* d_rgba,
                            uint32_t frames, vkgs_raster_config* chosen, float* best_ms, void* stream);
 /* Enable per-stage hipEvent timing (the reference's 12-timestamp breakdown,

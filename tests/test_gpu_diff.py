@@ -138,3 +138,18 @@ def test_backward_writes_zeros_for_untouched_splats_and_viewless_scenes():
         assert torch.count_nonzero(pa[k].grad[behind]) == 0, k
         assert torch.count_nonzero(pb[k].grad) == 0, k
     assert torch.count_nonzero(pa["means"].grad) > 0
+
+
+def test_backward_is_deterministic_with_8x8_tiles():
+    """No global atomics and one writer per LDS partial: two backward passes
+    over the same forward inputs give bit-identical gradients."""
+    sa, cam_a = small_scene(4000, 38, 96, 80)
+    cam_b = S.orbit_camera(128, 64, 3.0, theta=0.7)
+    grads = []
+    for _ in range(2):
+        p = D.scene_tensors(sa)
+        outs = D.render_batch([p], [(0, cam_a), (0, cam_b)], tile=(8, 8))
+        sum(o.square().sum() for o in outs).backward()
+        grads.append({k: p[k].grad.detach().cpu().numpy().copy() for k in NAMES})
+    for k in NAMES:
+        np.testing.assert_array_equal(grads[0][k], grads[1][k])

@@ -11,10 +11,12 @@
 // that stored state, recomputing alpha per entry and the transmittance before
 // it as T_i = T_{i+1} / (1 - alpha_i); per entry the 9 gradient words (colour
 // 3, opacity 1, conic 3, centre 2) are summed over the tile's pixels (DPP row
-// sums, then LDS atomics) and flushed with one global float atomic per word per
-// tile into the slot's words.  The projection backward then runs one lane per scene splat, chains
-// the words of every view that saw the splat to its raw parameters and writes
-// each gradient once.
+// sums, then LDS atomics) and stored as that (tile, splat) pair's partials at
+// the pair's emission index.  A splat's pairs were emitted contiguously, so a
+// reduction sums each splat's run into its 9 slot words; no global atomics
+// are used and the result does not depend on scheduling.  The projection
+// backward then runs one lane per scene splat, chains the words of every view
+// that saw the splat to its raw parameters and writes each gradient once.
 //
 // alpha is clamped to kAlphaMax so the division above stays bounded; this is
 // the only departure of this path from the reference's blend.
@@ -469,8 +471,10 @@ __device__ __forceinline__ float row_sum16(float x) {
 // with DPP, then one LDS float atomic per row (4 per wave).  The sums are
 // colour 3, opacity 1, dq*dx^2, dq*dx*dy, dq*dy^2, dq*dx, dq*dy (dq = dL/dq of
 // the exponent's quadratic form); the conic and centre words follow from the
-// last five and the entry's conic rows when the batch is flushed, with one
-// global float atomic per nonzero word.
+// last five and the entry's conic rows when the batch is flushed to the pair's
+// 9 words of gpart at its emission index.  Pairs no pixel
+// reached keep the zeros gpart was filled with; a reduction then sums each
+// splat's contiguous run of slots.
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                                       const uint32_t* __restrict__ vals,
@@ -636,51 +640,72 @@ __global__ __launch_bounds__(256) void diff_fill_zero_kernel(float* __restrict__
   if (i < n) g[i] = 0.f;
 }
 
-// g9[slot] = sum of the partials of the slot's pairs.  The pairs of a splat
-// are contiguous in emission order, so each 256-pair block loads its partials
-// coalesced, and the first thread of every run of equal slots sums the run
-// from LDS.  A run that reaches a block edge may continue in the next block:
-// those add atomically (g9 starts at zero), interior runs store.
-__global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* __restrict__ pslot,
+// Sum of x over the 64 lanes of a wave (all lanes active); lane 63 gets it.
+__device__ __forceinline__ float wave_total_last(float x) {
+  x = row_sum16(x);
+  // rows outside the row mask read 0 (the `old` operand)
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return x;
+}
+
+// g9[order[k]] = sum of the partials of the pairs of depth position k, which
+// were emitted at offs[k] .. offs[k] + cnt[k] - 1 (contiguous).  A lane sums a
+// run of up to kEmitSerial pairs itself; longer runs are summed by the whole
+// wave.  Every visible slot is written once (zeros for a splat without
+// pairs), so g9 needs no zero fill.
+__global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* __restrict__ order,
+                                                               const uint32_t* __restrict__ offs,
+                                                               const uint32_t* __restrict__ cnt,
                                                                const float* __restrict__ gpart,
-                                                               const uint32_t* __restrict__ d_pairs,
+                                                               const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                                uint64_t max_pairs, float* __restrict__ g9) {
-  __shared__ float s_v[9][257];
-  __shared__ uint32_t s_slot[256];
-  const uint64_t n = min((uint64_t)*d_pairs, max_pairs);
-  const uint32_t t = threadIdx.x;
-  for (uint64_t base = blockIdx.x * 256ull; base < n; base += (uint64_t)gridDim.x * 256) {
-    const uint64_t j = base + t;
-    const bool valid = j < n;
-    uint32_t slot = 0xffffffffu;
-    if (valid) {
-      slot = pslot[j];
-      const float* src = gpart + 9 * j;
-#pragma unroll
-      for (int w = 0; w < 9; ++w) s_v[w][t] = src[w];
+  const uint32_t n = min(*d_count, max_n);
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = wave * kWave; base < n; base += stride) {  // wave-uniform
+    const uint32_t k = base + lane;
+    uint32_t c = 0, slot = 0;
+    uint64_t o = 0;
+    if (k < n) {
+      o = offs[k];
+      c = cnt[k];
+      slot = order[k];
+      if (o + c > max_pairs) c = o < max_pairs ? (uint32_t)(max_pairs - o) : 0u;
     }
-    s_slot[t] = slot;
-    __syncthreads();
-    if (valid && (t == 0 || s_slot[t - 1] != slot)) {
-      float acc[9];
+    const bool big = c > kEmitSerial;
+    if (k < n && !big) {
+      float acc[9] = {};
+      for (uint32_t i = 0; i < c; ++i) {
+        const float* src = gpart + 9 * (o + i);
 #pragma unroll
-      for (int w = 0; w < 9; ++w) acc[w] = s_v[w][t];
-      uint32_t u = t + 1;
-      for (; u < 256 && s_slot[u] == slot; ++u)
-#pragma unroll
-        for (int w = 0; w < 9; ++w) acc[w] += s_v[w][u];
-      const bool at_edge = (t == 0 && j > 0 && pslot[j - 1] == slot) ||
-                           (u == 256 && base + 256 < n && pslot[base + 256] == slot);
+        for (int w = 0; w < 9; ++w) acc[w] += src[w];
+      }
       float* dst = g9 + 9 * (size_t)slot;
-      if (at_edge) {
 #pragma unroll
-        for (int w = 0; w < 9; ++w) unsafeAtomicAdd(&dst[w], acc[w]);
-      } else {
+      for (int w = 0; w < 9; ++w) dst[w] = acc[w];
+    }
+    uint64_t pending = __ballot(big);
+    while (pending) {
+      const uint32_t src_lane = (uint32_t)__builtin_ctzll(pending);
+      pending &= pending - 1;
+      const uint32_t bc = lane_u32(c, src_lane), bslot = lane_u32(slot, src_lane);
+      const uint64_t bo = ((uint64_t)lane_u32((uint32_t)(o >> 32), src_lane) << 32) | lane_u32((uint32_t)o, src_lane);
+      float acc[9] = {};
+      for (uint32_t i = lane; i < bc; i += kWave) {
+        const float* src = gpart + 9 * (bo + i);
+#pragma unroll
+        for (int w = 0; w < 9; ++w) acc[w] += src[w];
+      }
+#pragma unroll
+      for (int w = 0; w < 9; ++w) acc[w] = wave_total_last(acc[w]);
+      if (lane == kWave - 1) {
+        float* dst = g9 + 9 * (size_t)bslot;
 #pragma unroll
         for (int w = 0; w < 9; ++w) dst[w] = acc[w];
       }
     }
-    __syncthreads();
   }
 }
 
@@ -1051,12 +1076,11 @@ hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint
 
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
-                                       const uint32_t* d_pairs, uint64_t pairs, uint32_t max_slots, float* gpart,
+                                       uint64_t pairs, const uint32_t* order, const uint32_t* offs,
+                                       const uint32_t* cnt, const uint32_t* d_count, uint32_t max_n, float* gpart,
                                        float* g9, hipStream_t s) {
-  // partials of pairs no pixel reached stay zero; slots without pairs keep zero words
+  // partials of pairs no pixel reached stay zero
   if (pairs > 0) diff_fill_zero_kernel<<<grid_for(pairs * 9 / 4 + 1, 256, 8192), 256, 0, s>>>(gpart, pairs * 9);
-  if (max_slots > 0)
-    diff_fill_zero_kernel<<<grid_for((uint64_t)max_slots * 9 / 4 + 1, 256, 8192), 256, 0, s>>>(g9, (uint64_t)max_slots * 9);
   if (L.n_tiles > 0) {
     bool launched = false;
 #define VKGS_CASE(W_, H_)                                                                                  \
@@ -1069,8 +1093,9 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
 #undef VKGS_CASE
     if (!launched) return hipErrorInvalidValue;
   }
-  if (pairs > 0)
-    diff_grad_reduce_kernel<<<grid_for(pairs, 256, 16384), 256, 0, s>>>(pslot, gpart, d_pairs, pairs, g9);
+  if (max_n > 0)
+    diff_grad_reduce_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(order, offs, cnt, gpart, d_count, max_n,
+                                                                       pairs, g9);
   return hipGetLastError();
 }
 

@@ -338,11 +338,11 @@ int vkgs_diff_backward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t
   VKGS_HIP_CHECK(hipMemcpy(m, misc, sizeof(m), hipMemcpyDeviceToHost));
   const uint64_t pairs = m[kMiscPairs] < c->max_pairs ? m[kMiscPairs] : c->max_pairs;
   if (c->gpart.ensure((size_t)(pairs ? pairs : 1) * 36) != hipSuccess) return VKGS_ERR_HIP;
-  const uint32_t visible = m[kMiscVisible] < c->cap_items ? m[kMiscVisible] : c->cap_items;
   VKGS_HIP_CHECK(launch_diff_raster_backward(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(),
                                              c->ranges.as<uint32_t>(), c->pslot.as<uint32_t>(),
-                                             c->tile_order.as<uint32_t>(), misc + kMiscPairs, pairs, visible,
-                                             c->gpart.as<float>(), c->g9.as<float>(), s));
+                                             c->tile_order.as<uint32_t>(), pairs, c->val.as<uint32_t>(),
+                                             c->offs.as<uint32_t>(), c->cnt.as<uint32_t>(), misc + kMiscVisible,
+                                             c->cap_items, c->gpart.as<float>(), c->g9.as<float>(), s));
   VKGS_HIP_CHECK(launch_diff_project_backward(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(),
                                               c->g9.as<float>(), s));
   return VKGS_OK;

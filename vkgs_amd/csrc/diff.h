@@ -96,13 +96,13 @@ hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const
 hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
                               const uint32_t* pslot, const uint32_t* tile_order, bool keep_state, hipStream_t s);
 // Blend backward: per sorted pair 9 partial gradient words written at the
-// pair's emission index in gpart (9 floats per pair, pairs = live pair count),
-// then g9[slot] = the sum over the slot's pairs; g9 (max_slots slots) and
-// gpart are zeroed first.  Global atomics only where a slot's run of pairs
-// crosses a 256-pair block edge.
+// pair's emission index in gpart (9 floats per pair, pairs = live pair count,
+// zeroed first), then g9[order[k]] = the sum over the pairs of depth position
+// k (offs[k] .. offs[k] + cnt[k] - 1) for k < *d_count.  No global atomics.
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
-                                       const uint32_t* d_pairs, uint64_t pairs, uint32_t max_slots, float* gpart,
+                                       uint64_t pairs, const uint32_t* order, const uint32_t* offs,
+                                       const uint32_t* cnt, const uint32_t* d_count, uint32_t max_n, float* gpart,
                                        float* g9, hipStream_t s);
 // Writes (overwrites) every parameter gradient of every scene.
 hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,

@@ -409,36 +409,48 @@ int vkgs_renderer_autotune(vkgs_renderer* r, const vkgs_camera* camera, const fl
   VKGS_HIP_CHECK(hipEventCreate(&e0));
   VKGS_HIP_CHECK(hipEventCreate(&e1));
   int status = VKGS_OK;
-  for (const auto& sh : kShapes) {
-    vkgs_raster_config c = base;
-    c.tile_w = sh[0];
-    c.tile_h = sh[1];
-    if ((status = vkgs_renderer_set_config(r, &c)) != VKGS_OK) break;
-    // first frame sizes the pair storage; repeat while it overflows
-    for (int i = 0; i < 4; ++i) {
-      if ((status = vkgs_renderer_render(r, camera, model, d_rgba, stream)) != VKGS_OK) break;
-      status = vkgs_renderer_frame_info(r, nullptr, nullptr, stream);
-      if (status != VKGS_ERR_OVERFLOW) break;
-    }
-    if (status != VKGS_OK) break;
-    for (uint32_t i = 0; i < 3 && status == VKGS_OK; ++i) status = vkgs_renderer_render(r, camera, model, d_rgba, stream);
-    hipEventRecord(e0, s);
-    for (uint32_t i = 0; i < frames && status == VKGS_OK; ++i)
-      status = vkgs_renderer_render(r, camera, model, d_rgba, stream);
-    hipEventRecord(e1, s);
-    if (status != VKGS_OK) break;
-    if (hipEventSynchronize(e1) != hipSuccess) {
-      status = VKGS_ERR_HIP;
-      break;
-    }
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    ms /= (float)frames;
-    if (ms < best_t) {
-      best_t = ms;
-      best = c;
+  // shapes are timed round-robin and each keeps its fastest round, so clock
+  // ramp-up or a transient slow period does not favour the shapes timed later
+  constexpr int kShapeCount = sizeof(kShapes) / sizeof(kShapes[0]);
+  constexpr int kRounds = 3;
+  float shape_ms[kShapeCount];
+  for (int i = 0; i < kShapeCount; ++i) shape_ms[i] = 1e30f;
+  for (int round = 0; round < kRounds && status == VKGS_OK; ++round) {
+    for (int si = 0; si < kShapeCount && status == VKGS_OK; ++si) {
+      vkgs_raster_config c = base;
+      c.tile_w = kShapes[si][0];
+      c.tile_h = kShapes[si][1];
+      if ((status = vkgs_renderer_set_config(r, &c)) != VKGS_OK) break;
+      // the first frame sizes the pair storage; repeat while it overflows
+      for (int i = 0; i < 4; ++i) {
+        if ((status = vkgs_renderer_render(r, camera, model, d_rgba, stream)) != VKGS_OK) break;
+        status = vkgs_renderer_frame_info(r, nullptr, nullptr, stream);
+        if (status != VKGS_ERR_OVERFLOW) break;
+      }
+      if (status != VKGS_OK) break;
+      for (uint32_t i = 0; i < 2 && status == VKGS_OK; ++i) status = vkgs_renderer_render(r, camera, model, d_rgba, stream);
+      hipEventRecord(e0, s);
+      for (uint32_t i = 0; i < frames && status == VKGS_OK; ++i)
+        status = vkgs_renderer_render(r, camera, model, d_rgba, stream);
+      hipEventRecord(e1, s);
+      if (status != VKGS_OK) break;
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        status = VKGS_ERR_HIP;
+        break;
+      }
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      ms /= (float)frames;
+      if (ms < shape_ms[si]) shape_ms[si] = ms;
     }
   }
+  for (int si = 0; si < kShapeCount && status == VKGS_OK; ++si)
+    if (shape_ms[si] < best_t) {
+      best_t = shape_ms[si];
+      best = base;
+      best.tile_w = kShapes[si][0];
+      best.tile_h = kShapes[si][1];
+    }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   r->timing = timing;

@@ -24,7 +24,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
-constexpr int kIpt = 8;
+#ifndef VKGS_SORT_IPT
+#define VKGS_SORT_IPT 8
+#endif
+constexpr int kIpt = VKGS_SORT_IPT;  // keys per thread per sub-tile
 constexpr int kTile = kThreads * kIpt;  // keys per sub-tile
 #ifndef VKGS_SORT_MAX_BLOCKS
 #define VKGS_SORT_MAX_BLOCKS 1024

@@ -80,6 +80,8 @@ class _Pool:
 
 
 _pool = _Pool()
+# visible (view, splat) items and (tile, splat) pairs of the last forward
+last_info = {"visible": 0, "pairs": 0, "views": 0}
 
 PARAM_NAMES = ("means", "log_scales", "quats", "opacity_logits", "sh")
 IDENTITY = np.eye(4, dtype=np.float32).ravel()
@@ -144,6 +146,7 @@ class _RenderBatch(torch.autograd.Function):
             vis, pairs = C.c_uint32(), C.c_uint64()
             st = _L().vkgs_diff_info(c.h, C.byref(vis), C.byref(pairs), _stream())
             if st == 0:
+                last_info.update(visible=int(vis.value), pairs=int(pairs.value), views=len(views))
                 break
             if st != 6:  # VKGS_ERR_OVERFLOW: the next forward grows the pair storage
                 check(st, "vkgs_diff_info")
