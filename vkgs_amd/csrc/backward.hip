@@ -466,9 +466,22 @@ __device__ __forceinline__ float row_sum16(float x) {
   return dpp_add<0x128>(x);  // row_ror:8
 }
 
+// Half-row sums: after the quad steps and row_ror:4 every lane holds its quad
+// plus a neighbouring quad; lanes 7 and 15 of a row then hold two disjoint
+// halves whose sum is the row's (whichever way the rotation runs).
+__device__ __forceinline__ float half_row_sum(float x) {
+  x = dpp_add<0xb1>(x);
+  x = dpp_add<0x4e>(x);
+  return dpp_add<0x124>(x);
+}
+
 // Re-walks one tile back to front from the stored per-pixel state.  Per list
-// entry 9 sums over the tile's pixels are formed: within each row of 16 lanes
-// with DPP, then one LDS float atomic per row (4 per wave).  The sums are
+// entry 9 sums over the tile's pixels are formed: within each half of a
+// 16-lane row with DPP, then stored by that half row's last lane into a slot
+// of its own (8 per wave per word).  With one wave per tile every slot has one
+// writer and plain LDS stores suffice; LDS float atomics, used only when
+// several waves share a tile, cost most of the kernel's time in a measured
+// variant.  The sums are
 // colour 3, opacity 1, dq*dx^2, dq*dx*dy, dq*dy^2, dq*dx, dq*dy (dq = dL/dq of
 // the exponent's quadratic form); the conic and centre words follow from the
 // last five and the entry's conic rows when the batch is flushed to the pair's
@@ -491,7 +504,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   constexpr int kB = kOneWave ? VKGS_BWD_BATCH : kN;
   static_assert(kB <= kN, "batch larger than the workgroup");
   __shared__ float4 s_rec[3][kB];
-  __shared__ float s_g[9][4][kB + 1];  // per word, one partial sum per DPP row; +1 spreads the rows over banks
+  __shared__ float s_g[9][8][kB + 1];  // per word, one partial per half DPP row; +1 spreads them over banks
   __shared__ uint32_t s_j[kB];         // emission index of each batch entry
   __shared__ uint32_t s_max;
 
@@ -508,8 +521,8 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   const int row1 = row0 + kRowsPerWave - 1;
   const uint32_t end = ranges[2 * gtile + 1];
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-  const bool row_last = (__lane_id() & 15u) == 15u;
-  const uint32_t row = __lane_id() >> 4;
+  const bool half_last = (__lane_id() & 7u) == 7u;  // lanes 7 and 15 of each DPP row
+  const uint32_t row = __lane_id() >> 3;              // half-row slot 0..7
 
   uint32_t last = 0;
   float T = 1.f;
@@ -543,7 +556,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
 #pragma unroll
       for (int j = 0; j < 9; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) s_g[j][r][t] = 0.f;
+        for (int r = 0; r < 8; ++r) s_g[j][r][t] = 0.f;
     }
     __syncthreads();
     for (int e = 0; e < cnt; ++e) {
@@ -578,32 +591,42 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       rg = alpha * c.y + om * rg;
       rb = alpha * c.z + om * rb;
       ra = alpha * alpha + om * ra;
-      const bool clamped = raw > kAlphaMax;
-      const float dop = clamped ? 0.f : dalpha * G;
-      const float dq = clamped ? 0.f : -0.5f * dalpha * alpha;
-      const float dqx = dq * dx, dqy = dq * dy;
-      const float s0 = row_sum16(g.x * ta);
-      const float s1 = row_sum16(g.y * ta);
-      const float s2 = row_sum16(g.z * ta);
-      const float s3 = row_sum16(dop);
-      const float s4 = row_sum16(dqx * dx);
-      const float s5 = row_sum16(dqx * dy);
-      const float s6 = row_sum16(dqy * dy);
-      const float s7 = row_sum16(dqx);
-      const float s8 = row_sum16(dqy);
-      if (row_last) {
-        // each row its own slot (rows of other waves of a larger tile share
-        // it, hence the atomics); the slot index varies over the writing
-        // lanes, so these stay plain ds_add_f32
-        atomicAdd(&s_g[0][row][e], s0);
-        atomicAdd(&s_g[1][row][e], s1);
-        atomicAdd(&s_g[2][row][e], s2);
-        atomicAdd(&s_g[3][row][e], s3);
-        atomicAdd(&s_g[4][row][e], s4);
-        atomicAdd(&s_g[5][row][e], s5);
-        atomicAdd(&s_g[6][row][e], s6);
-        atomicAdd(&s_g[7][row][e], s7);
-        atomicAdd(&s_g[8][row][e], s8);
+      // dL/d(opacity) = dalpha * G and dL/dq = -dalpha * alpha / 2 =
+      // -(opacity / 2) * dop (both 0 where alpha is clamped): the q words are
+      // summed as dop * (...) and scaled by -opacity / 2 at the flush
+      const float dop = raw > kAlphaMax ? 0.f : dalpha * G;
+      const float dpx = dop * dx, dpy = dop * dy;
+      const float s0 = half_row_sum(g.x * ta);
+      const float s1 = half_row_sum(g.y * ta);
+      const float s2 = half_row_sum(g.z * ta);
+      const float s3 = half_row_sum(dop);
+      const float s4 = half_row_sum(dpx * dx);
+      const float s5 = half_row_sum(dpx * dy);
+      const float s6 = half_row_sum(dpy * dy);
+      const float s7 = half_row_sum(dpx);
+      const float s8 = half_row_sum(dpy);
+      if (half_last) {
+        if (kOneWave) {  // one writer per slot: plain LDS stores
+          s_g[0][row][e] = s0;
+          s_g[1][row][e] = s1;
+          s_g[2][row][e] = s2;
+          s_g[3][row][e] = s3;
+          s_g[4][row][e] = s4;
+          s_g[5][row][e] = s5;
+          s_g[6][row][e] = s6;
+          s_g[7][row][e] = s7;
+          s_g[8][row][e] = s8;
+        } else {  // half rows of the tile's other waves share the slot
+          atomicAdd(&s_g[0][row][e], s0);
+          atomicAdd(&s_g[1][row][e], s1);
+          atomicAdd(&s_g[2][row][e], s2);
+          atomicAdd(&s_g[3][row][e], s3);
+          atomicAdd(&s_g[4][row][e], s4);
+          atomicAdd(&s_g[5][row][e], s5);
+          atomicAdd(&s_g[6][row][e], s6);
+          atomicAdd(&s_g[7][row][e], s7);
+          atomicAdd(&s_g[8][row][e], s8);
+        }
       }
     }
     __syncthreads();
@@ -614,7 +637,12 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const float P = a.z * a.z + bb.x * bb.x, Q = a.z * a.w + bb.x * bb.y, R = a.w * a.w + bb.y * bb.y;
       float u[9];
 #pragma unroll
-      for (int j = 0; j < 9; ++j) u[j] = (s_g[j][0][t] + s_g[j][1][t]) + (s_g[j][2][t] + s_g[j][3][t]);
+      for (int j = 0; j < 9; ++j)
+        u[j] = ((s_g[j][0][t] + s_g[j][1][t]) + (s_g[j][2][t] + s_g[j][3][t])) +
+               ((s_g[j][4][t] + s_g[j][5][t]) + (s_g[j][6][t] + s_g[j][7][t]));
+      const float hq = -0.5f * bb.z;  // dq = hq * dop
+#pragma unroll
+      for (int j = 4; j < 9; ++j) u[j] *= hq;
       float* out = gpart + 9 * (size_t)s_j[t];  // at the pair's emission index
       out[0] = u[0];
       out[1] = u[1];
