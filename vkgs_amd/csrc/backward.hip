@@ -393,15 +393,16 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
     p1 = rec[3 * (size_t)k + 1];
     p2 = rec[3 * (size_t)k + 2];
   }
-  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
-  bool done = !inside;
+  // T >= t_min is "still compositing" (T only decreases; pixels outside the
+  // image start at T = 0 and store nothing)
+  float T = inside ? 1.f : 0.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
   uint32_t last = items;
   for (uint32_t b = 0; b < batches; ++b) {
     const int buf = b & 1;
     s_rec[buf][0][t] = p0;
     s_rec[buf][1][t] = p1;
     s_rec[buf][2][t] = p2;
-    if (__syncthreads_count(!done) == 0) break;
+    if (__syncthreads_count(T >= t_min) == 0) break;
     const uint32_t next = (b + 1) * kN + t;
     if (next < items) {
       const uint32_t k = pslot[vals[end - 1 - next]];
@@ -409,8 +410,9 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
       p1 = rec[3 * (size_t)k + 1];
       p2 = rec[3 * (size_t)k + 2];
     }
-    const int cnt = (int)min((uint32_t)kN, items - b * kN);
-    if (__ballot(!done) != 0) {
+    const int cnt = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kN, items - b * kN));
+    bool alive = T >= t_min;
+    if (__ballot(alive) != 0) {
       for (int e = 0; e < cnt; ++e) {
         const float4 a = s_rec[buf][0][e];
         const float4 bb = s_rec[buf][1][e];
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
         const float dx = fx - a.x, dy = fy - a.y;
         const float ux = fmaf(a.z, dx, a.w * dy);
         const float uy = fmaf(bb.x, dx, bb.y * dy);
-        const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && !done;
+        const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && alive;
         const float q = fmaf(ux, ux, uy * uy);
         const float alpha = in ? fminf(bb.z * gauss_falloff(q), kAlphaMax) : 0.f;
         const float ta = T * alpha;
@@ -432,11 +434,10 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
         cb = fmaf(c.z, ta, cb);
         aa = fmaf(alpha, ta, aa);
         T = T - ta;
-        if (in && T < t_min) {
-          done = true;
-          last = b * kN + e + 1;
-        }
-        if (__ballot(!done) == 0) break;
+        const bool was = alive;
+        alive = T >= t_min;
+        if (was && !alive) last = b * kN + e + 1;  // entries processed, nearest first
+        if (__ballot(alive) == 0) break;
       }
     }
   }
