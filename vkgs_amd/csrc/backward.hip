@@ -597,15 +597,18 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       // summed as dop * (...) and scaled by -opacity / 2 at the flush
       const float dop = raw > kAlphaMax ? 0.f : dalpha * G;
       const float dpx = dop * dx, dpy = dop * dy;
-      const float s0 = half_row_sum(g.x * ta);
-      const float s1 = half_row_sum(g.y * ta);
-      const float s2 = half_row_sum(g.z * ta);
-      const float s3 = half_row_sum(dop);
-      const float s4 = half_row_sum(dpx * dx);
-      const float s5 = half_row_sum(dpx * dy);
-      const float s6 = half_row_sum(dpy * dy);
-      const float s7 = half_row_sum(dpx);
-      const float s8 = half_row_sum(dpy);
+      float s0 = half_row_sum(g.x * ta);
+      float s1 = half_row_sum(g.y * ta);
+      float s2 = half_row_sum(g.z * ta);
+      float s3 = half_row_sum(dop);
+      float s4 = half_row_sum(dpx * dx);
+      float s5 = half_row_sum(dpx * dy);
+      float s6 = half_row_sum(dpy * dy);
+      float s7 = half_row_sum(dpx);
+      float s8 = half_row_sum(dpy);
+      // keep the last DPP add ahead of the branch (every lane computes it), so
+      // it folds into v_add_f32_dpp instead of a v_mov_b32_dpp + a later add
+      __asm__ volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(s3), "+v"(s4), "+v"(s5), "+v"(s6), "+v"(s7), "+v"(s8));
       if (half_last) {
         if (kOneWave) {  // one writer per slot: plain LDS stores
           s_g[0][row][e] = s0;
