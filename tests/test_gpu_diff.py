@@ -153,3 +153,22 @@ def test_backward_is_deterministic_with_8x8_tiles():
         grads.append({k: p[k].grad.detach().cpu().numpy().copy() for k in NAMES})
     for k in NAMES:
         np.testing.assert_array_equal(grads[0][k], grads[1][k])
+
+
+@pytest.mark.parametrize("tile", [(16, 16), (16, 8), (32, 4)])
+def test_multi_wave_tiles_match_8x8(tile):
+    """Tiles of several waves take the LDS-atomic path of the backward and one
+    list entry per thread per batch; images match the 8x8 path exactly and
+    gradients to float rounding."""
+    sc, cam = small_scene(3000, 39, 112, 72)
+    ref_p, p = D.scene_tensors(sc), D.scene_tensors(sc)
+    w = torch.tensor(np.random.default_rng(3).uniform(0.5, 1.5, (72, 112, 4)), dtype=torch.float32, device="cuda")
+    img_ref = D.render(**ref_p, camera=cam, tile=(8, 8), t_min=0.0)
+    (img_ref * w).sum().backward()
+    img = D.render(**p, camera=cam, tile=tile, t_min=0.0)
+    (img * w).sum().backward()
+    np.testing.assert_array_equal(img.detach().cpu().numpy(), img_ref.detach().cpu().numpy())
+    for k in NAMES:
+        g, gr = p[k].grad, ref_p[k].grad
+        rel = float(torch.linalg.norm(g - gr) / torch.linalg.norm(gr))
+        assert rel < 1e-5, (k, rel)
