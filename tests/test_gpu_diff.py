@@ -172,3 +172,27 @@ def test_multi_wave_tiles_match_8x8(tile):
         g, gr = p[k].grad, ref_p[k].grad
         rel = float(torch.linalg.norm(g - gr) / torch.linalg.norm(gr))
         assert rel < 1e-5, (k, rel)
+
+
+def test_pair_overflow_retries_and_matches_a_roomy_context(monkeypatch):
+    """Large splats on a fresh context outgrow its first pair storage: the
+    overflowing forwards stay in bounds, the retried forward and its backward
+    match those of a context that already has room."""
+    W, H = 512, 384
+    sc = S.gaussian_cloud(4000, seed=40, pos_std=1.0, logscale=(-0.7, 0.0))
+    cam = S.orbit_camera(W, H, 4.0)
+    w = torch.tensor(np.random.default_rng(4).uniform(0.5, 1.5, (H, W, 4)), dtype=torch.float32, device="cuda")
+    monkeypatch.setattr(D, "_pool", D._Pool())  # fresh context: smallest pair storage
+    runs = []
+    for _ in range(2):
+        p = D.scene_tensors(sc)
+        img = D.render(**p, camera=cam, tile=(8, 8), t_min=0.0)
+        (img * w).sum().backward()
+        runs.append((img.detach().cpu().numpy(), {k: p[k].grad.detach().cpu().numpy() for k in NAMES}))
+        pairs = D.last_info["pairs"]
+        print("pairs", pairs, flush=True)
+        assert pairs > 4 * 4096 + (1 << 20)  # beyond the first storage of a 4000-splat context
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    for k in NAMES:
+        assert np.isfinite(runs[0][1][k]).all(), k
+        np.testing.assert_array_equal(runs[0][1][k], runs[1][1][k])

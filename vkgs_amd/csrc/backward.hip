@@ -256,6 +256,7 @@ __global__ __launch_bounds__(kCompactBlock) void diff_count_kernel(DiffLaunch L,
 __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L, const uint32_t* __restrict__ offsets,
                                                                    const uint64_t* __restrict__ masks,
                                                                    float4* __restrict__ rec, uint4* __restrict__ rc,
+                                                                   uint32_t* __restrict__ scnt,
                                                                    uint32_t* __restrict__ key,
                                                                    uint32_t* __restrict__ val) {
   const uint32_t sb = blockIdx.x;
@@ -301,6 +302,7 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
     rec[3 * (size_t)slot + 1] = make_float4(sp.i10, sp.i11, sp.o, __uint_as_float(pack16(sp.x0, sp.x1)));
     rec[3 * (size_t)slot + 2] = make_float4(sp.r, sp.g, sp.b, __uint_as_float(pack16(sp.y0, sp.y1)));
     rc[slot] = r;
+    scnt[slot] = r.z;
     key[slot] = __float_as_uint(omz);
     val[slot] = slot;
   }
@@ -320,9 +322,44 @@ __global__ __launch_bounds__(256) void diff_bin_gather_kernel(const uint4* __res
   }
 }
 
-// Pairs (global tile, pair index) with pslot[pair index] = slot: the sort
-// carries the pair's emission index, so the backward can map each sorted
-// position back to its splat.
+// pslot[q] = slot for q in soffs[slot] .. soffs[slot] + scnt[slot] - 1, in
+// slot order (so the writes stream): runs of up to kEmitSerial words per lane,
+// longer runs by the whole wave.
+__global__ __launch_bounds__(256) void diff_slot_fill_kernel(const uint32_t* __restrict__ soffs,
+                                                             const uint32_t* __restrict__ scnt,
+                                                             const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                             uint64_t max_pairs, uint32_t* __restrict__ pslot) {
+  const uint32_t n = min(*d_count, max_n);
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = wave * kWave; base < n; base += stride) {  // wave-uniform
+    const uint32_t k = base + lane;
+    uint32_t c = 0;
+    uint64_t o = 0;
+    if (k < n) {
+      o = soffs[k];
+      c = scnt[k];
+      if (o + c > max_pairs) c = o < max_pairs ? (uint32_t)(max_pairs - o) : 0u;
+    }
+    const bool big = c > kEmitSerial;
+    if (!big)
+      for (uint32_t i = 0; i < c; ++i) pslot[o + i] = k;
+    uint64_t pending = __ballot(big);
+    while (pending) {
+      const uint32_t src = (uint32_t)__builtin_ctzll(pending);
+      pending &= pending - 1;
+      const uint32_t bc = lane_u32(c, src);
+      const uint64_t bo = ((uint64_t)lane_u32((uint32_t)(o >> 32), src) << 32) | lane_u32((uint32_t)o, src);
+      for (uint32_t i = lane; i < bc; i += kWave) pslot[bo + i] = base + src;
+    }
+  }
+}
+
+// Pairs (global tile, q), where q numbers the pairs in slot order (soffs =
+// exclusive scan of the per-slot tile counts): the sort carries q, so the
+// backward can place each sorted pair's partials where the slot's pairs sit
+// contiguously, in slot order.
 __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const TileRect* __restrict__ rect,
                                                             const uint32_t* __restrict__ cnt,
                                                             const uint32_t* __restrict__ offs,
@@ -330,7 +367,8 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
                                                             const uint32_t* __restrict__ order,
                                                             const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                             uint64_t max_pairs, uint32_t* __restrict__ keys,
-                                                            uint32_t* __restrict__ vals, uint32_t* __restrict__ pslot,
+                                                            uint32_t* __restrict__ vals,
+                                                            const uint32_t* __restrict__ soffs,
                                                             uint32_t* __restrict__ status) {
   const uint32_t n = min(*d_count, max_n);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -339,7 +377,7 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
   for (uint32_t base = wave * kWave; base < n; base += stride) {
     const uint32_t k = base + __lane_id();
     uint32_t c = 0, slot = 0, key_base = 0, tiles_x = 0;
-    uint64_t o = 0;
+    uint64_t o = 0, q0 = 0;
     TileRect r = {0u, 0u};
     if (k < n) {
       c = cnt[k];
@@ -347,13 +385,14 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
         o = offs[k];
         r = rect[k];
         slot = order[k];
+        q0 = soffs[slot];
         const DiffView& view = L.views[kview[k]];
         key_base = view.tile_base;
         tiles_x = view.fc.tiles_x;
         if (o + c > max_pairs) atomicOr(status, 1u);
       }
     }
-    emit_pairs_wave<true>(c, o, r, slot, key_base, tiles_x, max_pairs, keys, vals, pslot);
+    emit_pairs_wave<true>(c, o, r, slot, key_base, tiles_x, max_pairs, keys, vals, q0);
   }
 }
 
@@ -681,42 +720,69 @@ __device__ __forceinline__ float wave_total_last(float x) {
   return x;
 }
 
-// g9[order[k]] = sum of the partials of the pairs of depth position k, which
-// were emitted at offs[k] .. offs[k] + cnt[k] - 1 (contiguous).  A lane sums a
-// run of up to kEmitSerial pairs itself; longer runs are summed by the whole
-// wave.  Every visible slot is written once (zeros for a splat without
+// g9[slot] = sum of the partials of the slot's pairs, which sit at
+// soffs[slot] .. soffs[slot] + scnt[slot] - 1 (contiguous, slots in order, so
+// reads and writes both stream).  A lane sums a run of up to kEmitSerial pairs
+// itself (from coalesced LDS-staged chunks); longer runs are summed by the
+// whole wave.  Every visible slot is written once (zeros for a splat without
 // pairs), so g9 needs no zero fill.
-__global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* __restrict__ order,
-                                                               const uint32_t* __restrict__ offs,
-                                                               const uint32_t* __restrict__ cnt,
+__global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* __restrict__ soffs,
+                                                               const uint32_t* __restrict__ scnt,
                                                                const float* __restrict__ gpart,
                                                                const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                                uint64_t max_pairs, float* __restrict__ g9) {
+  __shared__ float s_stage[256 / kWave][9 * kWave];  // per wave: the partials of 64 consecutive pairs
   const uint32_t n = min(*d_count, max_n);
   const uint32_t lane = __lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t stride = gridDim.x * blockDim.x;
+  float* stage = s_stage[threadIdx.x / kWave];
   for (uint32_t base = wave * kWave; base < n; base += stride) {  // wave-uniform
     const uint32_t k = base + lane;
-    uint32_t c = 0, slot = 0;
+    uint32_t c = 0, slot = k;
     uint64_t o = 0;
     if (k < n) {
-      o = offs[k];
-      c = cnt[k];
-      slot = order[k];
+      o = soffs[k];
+      c = scnt[k];
       if (o + c > max_pairs) c = o < max_pairs ? (uint32_t)(max_pairs - o) : 0u;
     }
     const bool big = c > kEmitSerial;
-    if (k < n && !big) {
+    // runs of up to kEmitSerial pairs: the wave's runs are contiguous, [r0,
+    // r1); it is staged through LDS 64 pairs at a time with coalesced loads
+    // (lane i reads words i, i + 64, ...) and each lane adds the part of its
+    // run in the chunk.  Chunks that only long runs need are skipped.
+    {
+      const uint32_t last = __builtin_amdgcn_readfirstlane(min(n - base, (uint32_t)kWave) - 1);
+      const uint64_t r0 = ((uint64_t)lane_u32((uint32_t)(o >> 32), 0) << 32) | lane_u32((uint32_t)o, 0);
+      const uint64_t e = o + c;
+      uint64_t r1 = ((uint64_t)lane_u32((uint32_t)(e >> 32), last) << 32) | lane_u32((uint32_t)e, last);
+      r1 = r1 < max_pairs ? r1 : max_pairs;  // runs clipped at max_pairs may start beyond it
       float acc[9] = {};
-      for (uint32_t i = 0; i < c; ++i) {
-        const float* src = gpart + 9 * (o + i);
+      for (uint64_t cs = r0; cs < r1; cs += kWave) {
+        const uint64_t lo = o > cs ? o : cs, hi = e < cs + kWave ? e : cs + kWave;
+        const bool need = !big && lo < hi;
+        if (__ballot(need) == 0) continue;
+        const uint32_t words = 9u * (uint32_t)min((uint64_t)kWave, r1 - cs);
+        const float* src = gpart + 9 * cs;
 #pragma unroll
-        for (int w = 0; w < 9; ++w) acc[w] += src[w];
+        for (uint32_t q = 0; q < 9; ++q) {
+          const uint32_t i = lane + q * kWave;
+          if (i < words) stage[i] = src[i];
+        }
+        __builtin_amdgcn_wave_barrier();  // a wave's LDS operations complete in order
+        if (need)
+          for (uint64_t p = lo; p < hi; ++p) {
+            const float* v = stage + 9 * (uint32_t)(p - cs);
+#pragma unroll
+            for (int w = 0; w < 9; ++w) acc[w] += v[w];
+          }
+        __builtin_amdgcn_wave_barrier();
       }
-      float* dst = g9 + 9 * (size_t)slot;
+      if (k < n && !big) {
+        float* dst = g9 + 9 * (size_t)slot;
 #pragma unroll
-      for (int w = 0; w < 9; ++w) dst[w] = acc[w];
+        for (int w = 0; w < 9; ++w) dst[w] = acc[w];
+      }
     }
     uint64_t pending = __ballot(big);
     while (pending) {
@@ -1059,9 +1125,9 @@ hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint64_t* ma
 size_t diff_count_scan_bytes(uint32_t n_blocks) { return scan_plan(n_blocks * kMaskWords).bytes; }
 
 hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks, float4* rec,
-                             uint4* rc, uint32_t* key, uint32_t* val, hipStream_t s) {
+                             uint4* rc, uint32_t* scnt, uint32_t* key, uint32_t* val, hipStream_t s) {
   if (L.n_blocks == 0) return hipSuccess;
-  diff_write_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, rec, rc, key, val);
+  diff_write_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, rec, rc, scnt, key, val);
   return hipGetLastError();
 }
 
@@ -1075,10 +1141,11 @@ hipError_t launch_diff_bin_gather(const uint4* rc, const uint32_t* order, const 
 hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const uint32_t* cnt, const uint32_t* offs,
                                 const uint32_t* kview, const uint32_t* order, const uint32_t* d_count,
                                 uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* pslot,
-                                uint32_t* status, hipStream_t s) {
+                                const uint32_t* soffs, const uint32_t* scnt, uint32_t* status, hipStream_t s) {
   if (max_n == 0) return hipSuccess;
+  diff_slot_fill_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(soffs, scnt, d_count, max_n, max_pairs, pslot);
   diff_bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, rect, cnt, offs, kview, order, d_count, max_n,
-                                                                  max_pairs, keys, vals, pslot, status);
+                                                                  max_pairs, keys, vals, soffs, status);
   return hipGetLastError();
 }
 
@@ -1108,9 +1175,9 @@ hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint
 
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
-                                       uint64_t pairs, const uint32_t* order, const uint32_t* offs,
-                                       const uint32_t* cnt, const uint32_t* d_count, uint32_t max_n, float* gpart,
-                                       float* g9, hipStream_t s) {
+                                       uint64_t pairs, const uint32_t* soffs, const uint32_t* scnt,
+                                       const uint32_t* d_count, uint32_t max_n, float* gpart, float* g9,
+                                       hipStream_t s) {
   // partials of pairs no pixel reached stay zero
   if (pairs > 0) diff_fill_zero_kernel<<<grid_for(pairs * 9 / 4 + 1, 256, 8192), 256, 0, s>>>(gpart, pairs * 9);
   if (L.n_tiles > 0) {
@@ -1126,8 +1193,8 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
     if (!launched) return hipErrorInvalidValue;
   }
   if (max_n > 0)
-    diff_grad_reduce_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(order, offs, cnt, gpart, d_count, max_n,
-                                                                       pairs, g9);
+    diff_grad_reduce_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(soffs, scnt, gpart, d_count, max_n, pairs,
+                                                                       g9);
   return hipGetLastError();
 }
 

@@ -27,30 +27,26 @@ __device__ __forceinline__ uint32_t lane_u32(uint32_t x, uint32_t lane) {
 // Emits the pairs of one splat per lane: c tiles of rectangle r (tile x in
 // r.x's low/high 16 bits, tile y in r.y's) as keys key_base + ty * tiles_x + tx
 // at o, o + 1, ... in row-major tile order; pairs at or beyond max_pairs are
-// dropped (the caller flags the overflow).  The value of pair j is val, or,
-// with kIndexValues, j itself with val stored in slots[j] (so a later pass can
-// find where the sort moved each pair).  Lanes with up to kEmitSerial tiles
-// write their own pairs; each larger splat is then written by all 64 lanes
-// together, so one near-camera splat does not serialize a wave.  Every lane of
-// the wave must call this (c = 0 for lanes without a splat).
+// dropped (the caller flags the overflow).  The value of the i-th pair is val,
+// or, with kIndexValues, q0 + i: a second numbering of the pairs (the
+// caller's q0 per splat), which a later pass uses to find each pair after the
+// sort (clamped to max_pairs - 1 on overflow, so readers stay in bounds).  Lanes with up to kEmitSerial tiles write
+// their own pairs; each larger splat is then written by all 64 lanes together,
+// so one near-camera splat does not serialize a wave.  Every lane of the wave
+// must call this (c = 0 for lanes without a splat).
 template <bool kIndexValues = false>
 __device__ __forceinline__ void emit_pairs_wave(uint32_t c, uint64_t o, TileRect r, uint32_t val, uint32_t key_base,
                                                 uint32_t tiles_x, uint64_t max_pairs, uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals, uint32_t* __restrict__ slots = nullptr) {
+                                                uint32_t* __restrict__ vals, uint64_t q0 = 0) {
   const bool big = c > kEmitSerial;
   if (c != 0 && !big) {
     const uint32_t tx0 = r.x & 0xffffu, tx1 = r.x >> 16, ty0 = r.y & 0xffffu, ty1 = r.y >> 16;
-    uint64_t j = o;
+    uint64_t j = o, q = q0;
     for (uint32_t ty = ty0; ty <= ty1; ++ty)
-      for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j) {
+      for (uint32_t tx = tx0; tx <= tx1; ++tx, ++j, ++q) {
         if (j < max_pairs) {
           keys[j] = key_base + ty * tiles_x + tx;
-          if (kIndexValues) {
-            vals[j] = (uint32_t)j;
-            slots[j] = val;
-          } else {
-            vals[j] = val;
-          }
+          vals[j] = kIndexValues ? (uint32_t)(q < max_pairs ? q : max_pairs - 1) : val;
         }
       }
   }
@@ -62,6 +58,7 @@ __device__ __forceinline__ void emit_pairs_wave(uint32_t c, uint64_t o, TileRect
     const uint32_t bc = lane_u32(c, src), bval = lane_u32(val, src), bx = lane_u32(r.x, src);
     const uint32_t by = lane_u32(r.y, src), btx = lane_u32(tiles_x, src), bkey = lane_u32(key_base, src);
     const uint64_t bo = ((uint64_t)lane_u32((uint32_t)(o >> 32), src) << 32) | lane_u32((uint32_t)o, src);
+    const uint64_t bq = ((uint64_t)lane_u32((uint32_t)(q0 >> 32), src) << 32) | lane_u32((uint32_t)q0, src);
     const uint32_t tx0 = bx & 0xffffu, w = (bx >> 16) - tx0 + 1, ty0 = by & 0xffffu;
     const uint32_t first = bkey + ty0 * btx + tx0;
     for (uint32_t i = lane; i < bc; i += kWave) {
@@ -69,12 +66,7 @@ __device__ __forceinline__ void emit_pairs_wave(uint32_t c, uint64_t o, TileRect
       const uint64_t j = bo + i;
       if (j < max_pairs) {
         keys[j] = first + row * btx + col;
-        if (kIndexValues) {
-          vals[j] = (uint32_t)j;
-          slots[j] = bval;
-        } else {
-          vals[j] = bval;
-        }
+        vals[j] = kIndexValues ? (uint32_t)(bq + i < max_pairs ? bq + i : max_pairs - 1) : bval;
       }
     }
   }

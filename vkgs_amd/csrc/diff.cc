@@ -52,7 +52,8 @@ struct vkgs_diff_ctx {
   Buf tables;
   // per-item buffers
   uint32_t cap_items = 0;
-  Buf counts, count_scan, rec, rc, key, val, masks, kview, g9, misc, rect, cnt, offs, scan_storage, depth_sort;
+  Buf counts, count_scan, rec, rc, key, val, masks, kview, g9, misc, rect, cnt, offs, scnt, soffs, scan_storage,
+      depth_sort;
   // pairs
   uint64_t max_pairs = 0;
   Buf pkeys, pvals, pslot, gpart, pair_sort, ranges;
@@ -74,7 +75,7 @@ struct vkgs_diff_ctx {
 
 namespace {
 
-enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscTiles = 3 };
+enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscTiles = 3, kMiscSlotPairs = 4 };
 
 template <typename T>
 size_t pack(std::vector<unsigned char>& out, const std::vector<T>& v) {
@@ -127,11 +128,11 @@ int vkgs_diff_create(vkgs_diff_ctx** out) {
 int vkgs_diff_destroy(vkgs_diff_ctx* c) {
   if (!c) return VKGS_ERR_INVALID_ARG;
   (void)hipDeviceSynchronize();
-  Buf* bufs[] = {&c->tables, &c->counts, &c->count_scan, &c->rec,    &c->rc,        &c->key,    &c->val,
-                 &c->masks,  &c->kview,  &c->g9,     &c->misc,      &c->rect,   &c->cnt,
-                 &c->offs,   &c->scan_storage, &c->depth_sort, &c->pkeys, &c->pvals, &c->pair_sort,
-                 &c->ranges, &c->final_t, &c->last, &c->tile_keys, &c->tile_order, &c->tile_sort,
-                 &c->pslot, &c->gpart};
+  Buf* bufs[] = {&c->tables,     &c->counts,   &c->count_scan,   &c->rec,        &c->rc,         &c->key,
+                 &c->val,        &c->masks,    &c->kview,        &c->g9,         &c->misc,       &c->rect,
+                 &c->cnt,        &c->offs,     &c->scnt,         &c->soffs,      &c->scan_storage, &c->depth_sort,
+                 &c->pkeys,      &c->pvals,    &c->pair_sort,    &c->ranges,     &c->final_t,    &c->last,
+                 &c->tile_keys,  &c->tile_order, &c->tile_sort,  &c->pslot,      &c->gpart};
   for (Buf* b : bufs) b->release();
   delete c;
   return VKGS_OK;
@@ -241,6 +242,8 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   ENSURE(rect, (size_t)cap * sizeof(TileRect));
   ENSURE(cnt, (size_t)cap * 4);
   ENSURE(offs, (size_t)cap * 4);
+  ENSURE(scnt, (size_t)cap * 4);
+  ENSURE(soffs, (size_t)cap * 4);
   ENSURE(scan_storage, c->scan.bytes);
   ENSURE(depth_sort, c->depth_plan.bytes);
   ENSURE(pkeys, (size_t)c->max_pairs * 4);
@@ -280,11 +283,14 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   // ---- forward ----
   uint32_t* misc = c->misc.as<uint32_t>();
   uint32_t* order = c->val.as<uint32_t>();
-  VKGS_HIP_CHECK(fill_u32_launch(misc, 0u, 4, s));
+  VKGS_HIP_CHECK(fill_u32_launch(misc, 0u, 5, s));
   VKGS_HIP_CHECK(launch_diff_count(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), misc + kMiscVisible,
                                    c->count_scan.p, s));
   VKGS_HIP_CHECK(launch_diff_write(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), c->rec.as<float4>(),
-                                   c->rc.as<uint4>(), c->key.as<uint32_t>(), order, s));
+                                   c->rc.as<uint4>(), c->scnt.as<uint32_t>(), c->key.as<uint32_t>(), order, s));
+  // slot-order numbering of the pairs (their place in the backward's partials)
+  VKGS_HIP_CHECK(scan_exclusive_launch(c->scan, c->scnt.as<uint32_t>(), c->soffs.as<uint32_t>(), misc + kMiscVisible,
+                                       misc + kMiscSlotPairs, c->scan_storage.p, s));
   VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, 30u,
                                 c->depth_sort.p, s));
   VKGS_HIP_CHECK(launch_diff_bin_gather(c->rc.as<uint4>(), order, misc + kMiscVisible, cap, c->rect.as<TileRect>(),
@@ -294,7 +300,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   VKGS_HIP_CHECK(launch_diff_bin_emit(L, c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),
                                       c->kview.as<uint32_t>(), order, misc + kMiscVisible, cap, c->max_pairs,
                                       c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(), c->pslot.as<uint32_t>(),
-                                      misc + kMiscStatus, s));
+                                      c->soffs.as<uint32_t>(), c->scnt.as<uint32_t>(), misc + kMiscStatus, s));
   VKGS_HIP_CHECK(sort_kv_launch(c->pair_plan, misc + kMiscPairs, c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(),
                                 (uint32_t)bits_for((uint32_t)tiles), c->pair_sort.p, s));
   VKGS_HIP_CHECK(launch_ranges(c->pkeys.as<uint32_t>(), misc + kMiscPairs, c->max_pairs, (uint32_t)tiles,
@@ -340,9 +346,9 @@ int vkgs_diff_backward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t
   if (c->gpart.ensure((size_t)(pairs ? pairs : 1) * 36) != hipSuccess) return VKGS_ERR_HIP;
   VKGS_HIP_CHECK(launch_diff_raster_backward(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(),
                                              c->ranges.as<uint32_t>(), c->pslot.as<uint32_t>(),
-                                             c->tile_order.as<uint32_t>(), pairs, c->val.as<uint32_t>(),
-                                             c->offs.as<uint32_t>(), c->cnt.as<uint32_t>(), misc + kMiscVisible,
-                                             c->cap_items, c->gpart.as<float>(), c->g9.as<float>(), s));
+                                             c->tile_order.as<uint32_t>(), pairs, c->soffs.as<uint32_t>(),
+                                             c->scnt.as<uint32_t>(), misc + kMiscVisible, c->cap_items,
+                                             c->gpart.as<float>(), c->g9.as<float>(), s));
   VKGS_HIP_CHECK(launch_diff_project_backward(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(),
                                               c->g9.as<float>(), s));
   return VKGS_OK;

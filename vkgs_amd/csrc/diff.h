@@ -78,32 +78,35 @@ hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint64_t* ma
 size_t diff_count_scan_bytes(uint32_t n_blocks);
 // Per visible item at its slot offsets[w] + (visible lanes of wave w before
 // it): raster record rec[3 slot..], (tile rect x, tile rect y, tile count,
-// view) in rc[slot], key = bits(1 - z), val = slot.
+// view) in rc[slot], the tile count again in scnt[slot], key = bits(1 - z),
+// val = slot.
 hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks, float4* rec,
-                             uint4* rc, uint32_t* key, uint32_t* val, hipStream_t s);
+                             uint4* rc, uint32_t* scnt, uint32_t* key, uint32_t* val, hipStream_t s);
 // rect[k], cnt[k], kview[k] of slot order[k].
 hipError_t launch_diff_bin_gather(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
                                   TileRect* rect, uint32_t* cnt, uint32_t* kview, hipStream_t s);
-// (global tile, pair index) pairs at offs[k] with pslot[pair index] = slot
-// order[k]; pairs beyond max_pairs set *status.
+// (global tile, q) pairs at offs[k] for depth position k, with q numbering
+// the pairs in slot order from soffs[order[k]], and pslot[q] = slot for the
+// slot's scnt[slot] pairs; pairs beyond max_pairs set *status.
 hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const uint32_t* cnt, const uint32_t* offs,
                                 const uint32_t* kview, const uint32_t* order, const uint32_t* d_count,
                                 uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* pslot,
-                                uint32_t* status, hipStream_t s);
+                                const uint32_t* soffs, const uint32_t* scnt, uint32_t* status, hipStream_t s);
 // Raster of the tile-sorted pairs (vals = pair indices, pslot maps them to
 // slots); tile_order (optional): workgroup i rasterizes global tile
 // tile_order[i].
 hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
                               const uint32_t* pslot, const uint32_t* tile_order, bool keep_state, hipStream_t s);
 // Blend backward: per sorted pair 9 partial gradient words written at the
-// pair's emission index in gpart (9 floats per pair, pairs = live pair count,
-// zeroed first), then g9[order[k]] = the sum over the pairs of depth position
-// k (offs[k] .. offs[k] + cnt[k] - 1) for k < *d_count.  No global atomics.
+// pair's q (its slot-order index, the sorted value) in gpart (9 floats per
+// pair, pairs = live pair count, zeroed first), then g9[slot] = the sum over
+// the slot's pairs soffs[slot] .. soffs[slot] + scnt[slot] - 1 for slots <
+// *d_count.  No global atomics.
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
-                                       uint64_t pairs, const uint32_t* order, const uint32_t* offs,
-                                       const uint32_t* cnt, const uint32_t* d_count, uint32_t max_n, float* gpart,
-                                       float* g9, hipStream_t s);
+                                       uint64_t pairs, const uint32_t* soffs, const uint32_t* scnt,
+                                       const uint32_t* d_count, uint32_t max_n, float* gpart, float* g9,
+                                       hipStream_t s);
 // Writes (overwrites) every parameter gradient of every scene.
 hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
                                         const float* g9, hipStream_t s);
