@@ -246,8 +246,9 @@ def main(argv=None) -> int:
     passes = -(-(int(math.ceil(math.log2(max(T, 2)))) + 32) // 8)
     b_emit, b_sort, b_ranges = 12.0 * K, 24.0 * K * passes, 8.0 * K + 8.0 * T
     ps_gbs = (b_proj + b_emit + b_sort + b_ranges) / t_ps / 1e9
-    # bytes this design moves by its own algorithm (depth sort over V, tile sort over K, 8-B pairs)
-    own_bytes = b_proj + 64.0 * V + 32.0 * K + 20.0 * K + 8.0 * T
+    # bytes this design moves by its own algorithm: depth sort over V with 12-B (key, slot | packed tile
+    # rectangle) items, tile sort over K 8-B pairs, emit and ranges
+    own_bytes = b_proj + 96.0 * V + 32.0 * K + 20.0 * K + 8.0 * T
     own_gbs = own_bytes / t_ps / 1e9
 
     raster_name = "rasterize_kernel" if r_variant == 0 else "rasterize_wave_kernel"

@@ -189,6 +189,42 @@ def test_renderer_fused_matches_reference(c1):
     r.close()
 
 
+def test_renderer_unpacked_rect_path_matches_stage_path(c1):
+    """At 2560x1600 with 8x8 tiles (320 x 200 tiles) a tile rectangle needs
+    2 * (9 + 8) > 32 bits, so the renderer gathers (rect, count) words in depth
+    order instead of carrying a packed rectangle through the depth sort: same
+    sorted ids, tile lists and image as the stage path."""
+    sc, _, st, host = c1
+    W, H = 2560, 1600
+    cam = S.orbit_camera(W, H, 6.0)
+    r = P.Renderer(sc.n)
+    r.load_buffer(sc.ply_buffer(), sc.n)
+    r.set_config(8, 8, 1.0 / 8192)
+    for _ in range(4):  # an overflowing frame grows the pair storage for the next one
+        img = r.render(cam)
+        status, v, k = r.frame_info()
+        if status == 0:
+            break
+    assert status == 0 and v > 0
+    stage = P.reference_frame(cam, st, W, H)
+    sv = int(u32(stage["count"])[0])
+    print("unpacked path visible", v, "pairs", k, flush=True)
+    # the stage draw again with room for all k pairs
+    stage["image"], stage["draw_storage"] = P.draw(stage["instances"], stage["draw_indirect"], W, H,
+                                                   max_pairs=k + 1024)
+    assert stage["draw_storage"].status() == (0, k)
+    assert v == sv
+    ids, keys = r.sorted_ids()
+    np.testing.assert_array_equal(keys, u32(stage["key"])[:sv])
+    np.testing.assert_array_equal(ids, u32(stage["index"])[:sv])
+    ranges, lst_ids = r.tile_lists()
+    sranges, slst = stage["draw_storage"].lists()
+    np.testing.assert_array_equal(ranges, sranges)
+    np.testing.assert_array_equal(lst_ids, u32(stage["index"])[:sv][slst])
+    np.testing.assert_array_equal(img.cpu().numpy(), stage["image"].cpu().numpy())
+    r.close()
+
+
 def test_renderer_empty_and_offscreen():
     sc = S.gaussian_cloud(1000, seed=3)
     sc.rows[:, 0] += 1000.0  # everything far outside the frustum

@@ -146,6 +146,25 @@ hipError_t run_draw_slots(const uint4* rc, const uint32_t* order, const uint32_t
   return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }
 
+hipError_t run_draw_packed(const uint64_t* val64, const uint32_t* d_count, uint32_t max_inst, const RasterShape& rs,
+                           const RectBits& rb, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
+                           hipEvent_t* marks) {
+  hipError_t e;
+  if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
+  if ((e = scan_rect_counts_launch(b.scan, val64, rb, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess)
+    return e;
+  if ((e = launch_bin_emit_packed(val64, b.offs, d_count, max_inst, rs, rb, max_pairs, b.keys, b.vals, b.status,
+                                  s)) != hipSuccess)
+    return e;
+  if (marks) (void)hipEventRecord(marks[0], s);
+  const uint32_t tiles = rs.tiles_x * rs.tiles_y;
+  if ((e = sort_kv_launch(b.sort, b.pairs, b.keys, b.vals, (uint32_t)bits_for(tiles), b.sort_storage, s)) !=
+      hipSuccess)
+    return e;
+  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
+  return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
+}
+
 }  // namespace vkgs
 
 using namespace vkgs;

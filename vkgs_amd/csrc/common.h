@@ -111,6 +111,42 @@ struct TileRect {
   uint32_t x, y;
 };
 
+// A tile rectangle packed into 32 bits: x0, x1 in bx bits each, then y0, y1
+// in by bits each.  Usable when 2 * (bx + by) <= 32 (e.g. 200 x 113 tiles of
+// 8x8 pixels at 1600x900: 8 + 7 bits).  The empty rectangle is x0 = 1,
+// x1 = 0 (count 0).
+struct RectBits {
+  uint32_t bx, by;
+};
+
+VKHD uint32_t coord_bits(uint32_t tiles) {  // bits for values in [0, tiles), at least 1
+  uint32_t b = 1;
+  while (b < 32 && (1u << b) < tiles) ++b;
+  return b;
+}
+
+VKHD bool rect_bits(uint32_t tiles_x, uint32_t tiles_y, RectBits* rb) {
+  rb->bx = coord_bits(tiles_x);
+  rb->by = coord_bits(tiles_y);
+  return 2 * (rb->bx + rb->by) <= 32;
+}
+
+VKHD uint32_t pack_rect(uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, RectBits rb) {
+  return x0 | (x1 << rb.bx) | (y0 << (2 * rb.bx)) | (y1 << (2 * rb.bx + rb.by));
+}
+
+// (x0 | x1 << 16, y0 | y1 << 16) and the tile count of a packed rectangle.
+VKHD TileRect unpack_rect(uint32_t p, RectBits rb, uint32_t* count) {
+  const uint32_t mx = (1u << rb.bx) - 1u, my = (1u << rb.by) - 1u;
+  const uint32_t x0 = p & mx, x1 = (p >> rb.bx) & mx;
+  const uint32_t y0 = (p >> (2 * rb.bx)) & my, y1 = (p >> (2 * rb.bx + rb.by)) & my;
+  *count = (x1 >= x0 && y1 >= y0) ? (x1 - x0 + 1) * (y1 - y0 + 1) : 0u;
+  TileRect r;
+  r.x = x0 | (x1 << 16);
+  r.y = y0 | (y1 << 16);
+  return r;
+}
+
 // Raster set-up of one instance (splat.vert:10-26 restated in framebuffer
 // pixels): centre, inverse of the pixel-space rot_scale, colour, and the
 // conservative pixel rectangle of the +-3 sigma quad.  Returns false when the

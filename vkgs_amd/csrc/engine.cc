@@ -55,7 +55,10 @@ struct vkgs_renderer {
 
   // frame buffers sized by n
   uint32_t frame_n = 0;
-  DevBuf fc, misc, counts, rc, key, val, ids, sort_storage;
+  DevBuf fc, misc, counts, rc, key, val, val64, ids, sort_storage;
+  // tile rectangles packed into the depth sort's values (when they fit)
+  bool packed = false;
+  RectBits rb{};
   // binning sized by (n, pairs, tiles)
   uint64_t max_pairs = 0;
   DevBuf rec, rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges, tile_keys, tile_order, tile_sort;
@@ -106,21 +109,28 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   if (r->max_pairs > 0xffffffffull) r->max_pairs = 0xffffffffull;
   r->rs = rs;
   const uint32_t cap = n ? n : 1;
-  r->depth_plan = sort_plan(cap);
+  r->packed = rect_bits(rs.tiles_x, rs.tiles_y, &r->rb);
+  r->depth_plan = sort_plan(cap, r->packed ? 8u : 4u);
   r->pair_plan = sort_plan((uint32_t)r->max_pairs);
   r->scan = scan_plan(cap);
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   if ((e = r->fc.ensure(sizeof(FrameConst))) != hipSuccess) return e;
   if ((e = r->misc.ensure(kMiscWords * 4)) != hipSuccess) return e;
   if ((e = r->counts.ensure(compact_storage_bytes(cap))) != hipSuccess) return e;
-  if ((e = r->rc.ensure((size_t)cap * 16)) != hipSuccess) return e;
+  if (r->packed) {
+    if ((e = r->val64.ensure((size_t)cap * 8)) != hipSuccess) return e;
+  } else {
+    if ((e = r->rc.ensure((size_t)cap * 16)) != hipSuccess) return e;
+    if ((e = r->val.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  }
   if ((e = r->key.ensure((size_t)cap * 4)) != hipSuccess) return e;
-  if ((e = r->val.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->ids.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->sort_storage.ensure(r->depth_plan.bytes)) != hipSuccess) return e;
   if ((e = r->rec.ensure((size_t)cap * 48)) != hipSuccess) return e;
-  if ((e = r->rect.ensure((size_t)cap * sizeof(TileRect))) != hipSuccess) return e;
-  if ((e = r->cnt.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  if (!r->packed) {  // depth-ordered rectangles and counts of the gather path
+    if ((e = r->rect.ensure((size_t)cap * sizeof(TileRect))) != hipSuccess) return e;
+    if ((e = r->cnt.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  }
   if ((e = r->offs.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->scan_storage.ensure(r->scan.bytes)) != hipSuccess) return e;
   if ((e = r->pkeys.ensure((size_t)r->max_pairs * 4)) != hipSuccess) return e;
@@ -170,22 +180,35 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
                               r->counts.as<uint32_t>(), s)) != hipSuccess)
     return e;
   if (timed) hipEventRecord(r->ev[1], s);
-  if ((e = launch_fused_write(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
-                              r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(),
-                              r->rc.as<uint4>(), r->key.as<uint32_t>(), r->val.as<uint32_t>(), r->ids.as<uint32_t>(),
-                              r->counts.as<uint32_t>(), s)) != hipSuccess)
-    return e;
+  if (r->packed)
+    e = launch_fused_write_packed(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
+                                  r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(),
+                                  r->key.as<uint32_t>(), r->val64.as<uint64_t>(), r->ids.as<uint32_t>(),
+                                  r->counts.as<uint32_t>(), s);
+  else
+    e = launch_fused_write(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
+                           r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(), r->rc.as<uint4>(),
+                           r->key.as<uint32_t>(), r->val.as<uint32_t>(), r->ids.as<uint32_t>(),
+                           r->counts.as<uint32_t>(), s);
+  if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[2], s);
   // keys are bits(1 - z) <= 0x3f800000: 30 significant bits
-  if ((e = sort_kv_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val.as<uint32_t>(), 30u,
-                          r->sort_storage.p, s)) != hipSuccess)
-    return e;
+  if (r->packed)
+    e = sort_kv64_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val64.as<uint64_t>(), 30u,
+                         r->sort_storage.p, s);
+  else
+    e = sort_kv_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val.as<uint32_t>(), 30u,
+                       r->sort_storage.p, s);
+  if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[3], s);
   hipEvent_t marks[2] = {r->ev[4], r->ev[5]};
-  if ((e = run_draw_slots(r->rc.as<uint4>(), r->val.as<uint32_t>(), misc + kMiscVisible, n ? n : 1, r->rs,
-                          r->max_pairs, draw_buffers(r), reinterpret_cast<float4*>(out), s,
-                          timed ? marks : nullptr)) != hipSuccess)
-    return e;
+  if (r->packed)
+    e = run_draw_packed(r->val64.as<uint64_t>(), misc + kMiscVisible, n ? n : 1, r->rs, r->rb, r->max_pairs,
+                        draw_buffers(r), reinterpret_cast<float4*>(out), s, timed ? marks : nullptr);
+  else
+    e = run_draw_slots(r->rc.as<uint4>(), r->val.as<uint32_t>(), misc + kMiscVisible, n ? n : 1, r->rs, r->max_pairs,
+                       draw_buffers(r), reinterpret_cast<float4*>(out), s, timed ? marks : nullptr);
+  if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[6], s);
   return hipSuccess;
 }
@@ -220,10 +243,11 @@ int vkgs_renderer_destroy(vkgs_renderer* r) {
   if (!r) return VKGS_ERR_INVALID_ARG;
   (void)hipDeviceSynchronize();
   drop_graph(r);
-  DevBuf* bufs[] = {&r->position, &r->cov3d, &r->opacity, &r->sh,   &r->fc,   &r->misc,  &r->counts,
-                    &r->rc,       &r->key,   &r->val,     &r->ids,  &r->sort_storage, &r->rec, &r->rect, &r->cnt,
-                    &r->offs,     &r->scan_storage, &r->pkeys, &r->pvals, &r->psort, &r->ranges, &r->tile_keys, &r->tile_order, &r->tile_sort, &r->sorted_ids,
-                    &r->list_ids};
+  DevBuf* bufs[] = {&r->position,   &r->cov3d,      &r->opacity,  &r->sh,         &r->fc,        &r->misc,
+                    &r->counts,     &r->rc,         &r->key,      &r->val,        &r->val64,     &r->ids,
+                    &r->sort_storage, &r->rec,      &r->rect,     &r->cnt,        &r->offs,      &r->scan_storage,
+                    &r->pkeys,      &r->pvals,      &r->psort,    &r->ranges,     &r->tile_keys, &r->tile_order,
+                    &r->tile_sort,  &r->sorted_ids, &r->list_ids};
   for (DevBuf* b : bufs) b->release();
   if (r->capture_stream) hipStreamDestroy(r->capture_stream);
   for (int i = 0; i < kEvents; ++i)
@@ -369,8 +393,13 @@ int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const 
   const uint32_t cap = r->frame_n ? r->frame_n : 1;
   VKGS_HIP_CHECK(r->sorted_ids.ensure((size_t)cap * 4));
   uint32_t* misc = r->misc.as<uint32_t>();
-  VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val.as<uint32_t>(), nullptr, misc + kMiscVisible, cap,
-                                   r->sorted_ids.as<uint32_t>(), nullptr));
+  // sorted slots: the depth sort's values (low words of the packed values)
+  if (r->packed)
+    VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val64.as<uint32_t>(), nullptr, misc + kMiscVisible,
+                                     cap, r->sorted_ids.as<uint32_t>(), nullptr, 2));
+  else
+    VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val.as<uint32_t>(), nullptr, misc + kMiscVisible,
+                                     cap, r->sorted_ids.as<uint32_t>(), nullptr));
   VKGS_HIP_CHECK(hipDeviceSynchronize());
   *d_sorted_ids = r->sorted_ids.as<uint32_t>();
   if (d_keys) *d_keys = r->key.as<uint32_t>();

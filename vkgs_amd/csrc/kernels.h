@@ -44,6 +44,13 @@ hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const fl
                               const uint16_t* sh, uint32_t n, float4* rec, uint4* rc, uint32_t* key, uint32_t* val,
                               uint32_t* ids, const uint32_t* counts, hipStream_t s);
 
+// Same, with the tile rectangle packed into the sort value instead of rc:
+// val64[slot] = slot | packed rect << 32 (frame tiles must fit rect_bits).
+hipError_t launch_fused_write_packed(const FrameConst* d_fc, const float* pos, const float* cov3d,
+                                     const float* opacity, const uint16_t* sh, uint32_t n, float4* rec,
+                                     uint32_t* key, uint64_t* val64, uint32_t* ids, const uint32_t* counts,
+                                     hipStream_t s);
+
 // Copies a FrameConst passed by value into device memory (stream-ordered).
 hipError_t launch_set_frame_const(FrameConst* d_fc, const FrameConst& fc, hipStream_t s);
 
@@ -71,6 +78,11 @@ hipError_t launch_bin_gather(const uint4* rc, const uint32_t* order, const uint3
 hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* order,
                            const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
                            uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s);
+// (tile, slot) pairs at offs[k] from depth-sorted values val64[k] = slot |
+// packed rect << 32 (offs = exclusive scan of the packed tile counts).
+hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
+                                  const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
+                                  uint32_t* vals, uint32_t* status, hipStream_t s);
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s);
 // Tiles ordered longest list first (stable by tile id among equal lengths);
@@ -118,8 +130,15 @@ hipError_t run_draw_slots(const uint4* rc, const uint32_t* order, const uint32_t
                           const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
                           hipEvent_t* marks);
 
-// out[k] = src[idx[k]] (idx2 == nullptr) or src[idx[idx2[k]]] for k < *d_count.
+// Same after the packed fused projection: val64[k] = slot | packed rect << 32
+// of the slot drawn k-th.
+hipError_t run_draw_packed(const uint64_t* val64, const uint32_t* d_count, uint32_t max_inst, const RasterShape& rs,
+                           const RectBits& rb, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
+                           hipEvent_t* marks);
+
+// out[k] = src[idx[k * idx_stride]] (idx2 == nullptr) or
+// src[idx[idx2[k] * idx_stride]] for k < *d_count.
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* idx2, const uint32_t* d_count,
-                             uint32_t max_n, uint32_t* out, hipStream_t s);
+                             uint32_t max_n, uint32_t* out, hipStream_t s, uint32_t idx_stride = 1);
 
 }  // namespace vkgs

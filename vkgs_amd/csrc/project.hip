@@ -258,14 +258,18 @@ __global__ __launch_bounds__(kCompactBlock) void rank_write_kernel(FrameConst fc
 __device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
 // Cull + project + compact, and the raster set-up of the projected quad: the
-// raster record (48 B) and (tile rectangle, tile count) (16 B) are written at
-// the compacted slot with coalesced stores, so binning in depth order later
-// gathers 16 bytes per splat instead of re-reading and re-deriving the quad.
+// raster record (48 B) is written at the compacted slot with coalesced stores,
+// with the tile rectangle either as (rect, count) words in rc (16 B; binning
+// in depth order then gathers them) or, with kPacked, packed into the high
+// word of the 64-bit sort value val64 = slot | rect << 32, so that the depth
+// sort carries it into depth order and binning reads it sequentially.
+template <bool kPacked>
 __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
     const FrameConst* __restrict__ fcp, const float* __restrict__ position, const float* __restrict__ cov3d,
     const float* __restrict__ opacity, const uint16_t* __restrict__ sh, uint32_t n,
     const uint32_t* __restrict__ offsets, float4* __restrict__ rec, uint4* __restrict__ rc,
-    uint32_t* __restrict__ key, uint32_t* __restrict__ val, uint32_t* __restrict__ ids) {
+    uint32_t* __restrict__ key, uint32_t* __restrict__ val, uint64_t* __restrict__ val64,
+    uint32_t* __restrict__ ids) {
   __shared__ uint32_t s_w[kCompactBlock / kWave];
   const FrameConst& fc = *fcp;
   const uint32_t id = blockIdx.x * kCompactBlock + threadIdx.x;
@@ -282,11 +286,15 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
                          o[2].x, o[2].y, o[2].z, o[2].w};
     Splat2D s;
     uint4 r = make_uint4(0u, 0u, 0u, 0u);
+    RectBits rb;
+    rect_bits(fc.tiles_x, fc.tiles_y, &rb);
+    uint32_t packed = pack_rect(1u, 0u, 0u, 0u, rb);  // empty
     if (setup_splat(f, fc.width, fc.height, &s)) {
       const uint32_t shx = 31u - __clz(fc.tile_w), shy = 31u - __clz(fc.tile_h);
       const uint32_t tx0 = (uint32_t)s.x0 >> shx, tx1 = (uint32_t)s.x1 >> shx;
       const uint32_t ty0 = (uint32_t)s.y0 >> shy, ty1 = (uint32_t)s.y1 >> shy;
       r = make_uint4(tx0 | (tx1 << 16), ty0 | (ty1 << 16), (tx1 - tx0 + 1) * (ty1 - ty0 + 1), 0u);
+      packed = pack_rect(tx0, tx1, ty0, ty1, rb);
     } else {
       s.x0 = 1;
       s.x1 = 0;
@@ -296,9 +304,13 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
     rec[3 * (size_t)slot + 0] = make_float4(s.cx, s.cy, s.i00, s.i01);
     rec[3 * (size_t)slot + 1] = make_float4(s.i10, s.i11, s.o, __uint_as_float(pack16(s.x0, s.x1)));
     rec[3 * (size_t)slot + 2] = make_float4(s.r, s.g, s.b, __uint_as_float(pack16(s.y0, s.y1)));
-    rc[slot] = r;
+    if (kPacked) {
+      val64[slot] = (uint64_t)slot | ((uint64_t)packed << 32);
+    } else {
+      rc[slot] = r;
+      val[slot] = slot;
+    }
     key[slot] = __float_as_uint(omz);
-    val[slot] = slot;
     ids[slot] = id;
   }
 }
@@ -430,7 +442,19 @@ hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const fl
                               uint32_t* ids, const uint32_t* counts, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t nb = ceil_div(n, kCompactBlock);
-  fused_write_kernel<<<nb, kCompactBlock, 0, s>>>(d_fc, pos, cov3d, opacity, sh, n, counts, rec, rc, key, val, ids);
+  fused_write_kernel<false>
+      <<<nb, kCompactBlock, 0, s>>>(d_fc, pos, cov3d, opacity, sh, n, counts, rec, rc, key, val, nullptr, ids);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_write_packed(const FrameConst* d_fc, const float* pos, const float* cov3d,
+                                     const float* opacity, const uint16_t* sh, uint32_t n, float4* rec,
+                                     uint32_t* key, uint64_t* val64, uint32_t* ids, const uint32_t* counts,
+                                     hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t nb = ceil_div(n, kCompactBlock);
+  fused_write_kernel<true><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, cov3d, opacity, sh, n, counts, rec, nullptr, key,
+                                                        nullptr, val64, ids);
   return hipGetLastError();
 }
 

@@ -109,6 +109,36 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
   }
 }
 
+// Same from depth-sorted 64-bit values slot | packed rect << 32: pair value =
+// slot, offsets from the scan of the packed rectangles' tile counts.
+__global__ __launch_bounds__(256) void bin_emit_packed_kernel(const uint64_t* __restrict__ val64,
+                                                              const uint32_t* __restrict__ offs,
+                                                              const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                              uint32_t tiles_x, RectBits rb, uint64_t max_pairs,
+                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                              uint32_t* __restrict__ status) {
+  const uint32_t n = min(*d_count, max_n);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // wave-uniform loop: emit_pairs_wave needs every lane
+  for (uint32_t base = wave * kWave; base < n; base += stride) {
+    const uint32_t k = base + __lane_id();
+    uint32_t c = 0, v = 0;
+    uint64_t o = 0;
+    TileRect r = {0u, 0u};
+    if (k < n) {
+      const uint64_t w = val64[k];
+      r = unpack_rect((uint32_t)(w >> 32), rb, &c);
+      if (c != 0) {
+        o = offs[k];
+        v = (uint32_t)w;
+        if (o + c > max_pairs) atomicOr(status, 1u);
+      }
+    }
+    emit_pairs_wave(c, o, r, v, 0u, tiles_x, max_pairs, keys, vals);
+  }
+}
+
 __global__ void ranges_fill_kernel(uint32_t* __restrict__ ranges, uint32_t n2) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n2) ranges[i] = 0u;
@@ -290,11 +320,11 @@ __global__ __launch_bounds__(TW* TH) void rasterize_wave_kernel(const float4* __
 }
 
 __global__ void gather_u32_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx,
-                                  const uint32_t* __restrict__ idx2, const uint32_t* __restrict__ d_count,
-                                  uint32_t max_n, uint32_t* __restrict__ out) {
+                                  const uint32_t* __restrict__ idx2, uint32_t idx_stride,
+                                  const uint32_t* __restrict__ d_count, uint32_t max_n, uint32_t* __restrict__ out) {
   const uint32_t n = min(*d_count, max_n);
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const uint32_t i = idx2 ? idx[idx2[k]] : idx[k];
+    const uint32_t i = idx2 ? idx[(size_t)idx2[k] * idx_stride] : idx[(size_t)k * idx_stride];
     out[k] = src[i];
   }
 }
@@ -362,6 +392,15 @@ hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint
   return hipGetLastError();
 }
 
+hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
+                                  const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
+                                  uint32_t* vals, uint32_t* status, hipStream_t s) {
+  if (max_n == 0) return hipSuccess;
+  bin_emit_packed_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(val64, offs, d_count, max_n, rs.tiles_x, rb,
+                                                                     max_pairs, keys, vals, status);
+  return hipGetLastError();
+}
+
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s) {
   ranges_fill_kernel<<<grid_for(2ull * tiles, 256, 1u << 20), 256, 0, s>>>(ranges, 2 * tiles);
@@ -404,9 +443,9 @@ hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint3
 }
 
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* idx2, const uint32_t* d_count,
-                             uint32_t max_n, uint32_t* out, hipStream_t s) {
+                             uint32_t max_n, uint32_t* out, hipStream_t s, uint32_t idx_stride) {
   if (max_n == 0) return hipSuccess;
-  gather_u32_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(src, idx, idx2, d_count, max_n, out);
+  gather_u32_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(src, idx, idx2, idx_stride, d_count, max_n, out);
   return hipGetLastError();
 }
 

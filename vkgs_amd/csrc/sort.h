@@ -1,4 +1,4 @@
-// LSD radix sort of (u32 key, u32 value) pairs and an exclusive u32 scan,
+// LSD radix sort of (u32 key, u32 or u64 value) pairs and an exclusive u32 scan,
 // both with the element count read from device memory, so a hipGraph can be
 // captured once for a capacity and replayed for any live count.
 #pragma once
@@ -16,12 +16,16 @@ struct SortPlan {
   size_t bytes = 0;
 };
 
-SortPlan sort_plan(uint32_t max_n);
+// Storage plan for up to max_n pairs with val_bytes-byte values (4 or 8).
+SortPlan sort_plan(uint32_t max_n, uint32_t val_bytes = 4);
 
 // Stable ascending sort of min(*d_count, max_n) pairs on key bits
 // [0, key_bits), result in (keys, vals).
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
                           uint32_t key_bits, void* storage, hipStream_t stream);
+// Same with 64-bit values (plan from sort_plan(max_n, 8)).
+hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
+                            uint32_t key_bits, void* storage, hipStream_t stream);
 
 struct ScanPlan {
   uint32_t max_n = 0;
@@ -36,6 +40,12 @@ ScanPlan scan_plan(uint32_t max_n);
 // i < max_n); *d_total = sum of all.  in and out may alias.
 hipError_t scan_exclusive_launch(const ScanPlan& plan, const uint32_t* in, uint32_t* out, const uint32_t* d_count,
                                  uint32_t* d_total, void* storage, hipStream_t stream);
+
+// Same over the tile counts of the packed rectangles in the high words of
+// 64-bit sort values (slot | rect << 32).
+struct RectBits;
+hipError_t scan_rect_counts_launch(const ScanPlan& plan, const uint64_t* val64, const RectBits& rb, uint32_t* out,
+                                   const uint32_t* d_count, uint32_t* d_total, void* storage, hipStream_t stream);
 
 // Single-workgroup exclusive scan of `total` words in place (total <= 2^22);
 // *d_sum (optional) receives the sum.

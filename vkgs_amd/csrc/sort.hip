@@ -161,10 +161,10 @@ __global__ __launch_bounds__(kThreads) void sort_scan_rows(uint32_t* __restrict_
   if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
 }
 
-template <int kBits>
+template <int kBits, typename V>
 __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __restrict__ kin,
-                                                           const uint32_t* __restrict__ vin,
-                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           const V* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout, V* __restrict__ vout,
                                                            const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                            uint32_t shift, const uint32_t* __restrict__ hist,
                                                            const uint32_t* __restrict__ digit_total) {
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
   __shared__ uint32_t s_start[256];
   __shared__ uint32_t s_tmp[kWaves];
   __shared__ uint32_t s_key[kTile];
-  __shared__ uint32_t s_val[kTile];
+  __shared__ V s_val[kTile];
 
   const uint32_t tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const uint32_t n = min(*d_count, max_n);
@@ -195,13 +195,14 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
   for (uint32_t base = begin; base < end; base += kTile) {
     const uint32_t cnt = min((uint32_t)kTile, end - base);
     const uint32_t seg = base + w * (kWave * kIpt);
-    uint32_t k[kIpt], v[kIpt], rank[kIpt];
+    uint32_t k[kIpt], rank[kIpt];
+    V v[kIpt];
 #pragma unroll
     for (int r = 0; r < kIpt; ++r) {
       const uint32_t i = seg + r * kWave + lane;
       const bool valid = i < end;
       k[r] = valid ? kin[i] : 0u;
-      v[r] = valid ? vin[i] : 0u;
+      v[r] = valid ? vin[i] : V(0);
     }
     // stable rank inside this wave's segment, 64 keys at a time.  The wave
     // owns its counter row, so no atomics: every lane reads its digit's count,
@@ -286,23 +287,39 @@ __global__ __launch_bounds__(kScanThreads) void scan_single_block(uint32_t* __re
   if (d_sum != nullptr && tid == 0) *d_sum = all;
 }
 
-__global__ __launch_bounds__(kThreads) void scan_reduce(const uint32_t* __restrict__ in,
-                                                        const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                        uint32_t* __restrict__ partial) {
+// Scan inputs: plain words, or the tile count of the packed rectangle in the
+// high word of 64-bit sort values.
+struct LoadWords {
+  const uint32_t* p;
+  __device__ uint32_t operator()(uint32_t i) const { return p[i]; }
+};
+struct LoadRectCount {
+  const uint64_t* v;
+  RectBits rb;
+  __device__ uint32_t operator()(uint32_t i) const {
+    uint32_t c;
+    unpack_rect((uint32_t)(v[i] >> 32), rb, &c);
+    return c;
+  }
+};
+
+template <typename Load>
+__global__ __launch_bounds__(kThreads) void scan_reduce(Load in, const uint32_t* __restrict__ d_count,
+                                                        uint32_t max_n, uint32_t* __restrict__ partial) {
   __shared__ uint32_t s_tmp[kWaves];
   const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
   uint32_t s = 0;
-  for (uint32_t i = begin + threadIdx.x; i < end; i += kThreads) s += in[i];
+  for (uint32_t i = begin + threadIdx.x; i < end; i += kThreads) s += in(i);
   uint32_t all;
   block_exclusive_scan<kThreads>(s, s_tmp, &all);
   if (threadIdx.x == 0) partial[blockIdx.x] = all;
 }
 
-__global__ __launch_bounds__(kThreads) void scan_apply(const uint32_t* in, uint32_t* out,
-                                                       const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                       const uint32_t* __restrict__ partial) {
+template <typename Load>
+__global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, const uint32_t* __restrict__ d_count,
+                                                       uint32_t max_n, const uint32_t* __restrict__ partial) {
   __shared__ uint32_t s_tmp[kWaves];
   const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
@@ -314,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void scan_apply(const uint32_t* in, uint3
     uint32_t s = 0;
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
-      x[j] = first + j < end ? in[first + j] : 0u;
+      x[j] = first + j < end ? in(first + j) : 0u;
       s += x[j];
     }
     uint32_t all;
@@ -328,8 +345,9 @@ __global__ __launch_bounds__(kThreads) void scan_apply(const uint32_t* in, uint3
   }
 }
 
-__global__ void copy_pairs(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ vs, uint32_t* __restrict__ kd,
-                           uint32_t* __restrict__ vd, const uint32_t* __restrict__ d_count, uint32_t max_n) {
+template <typename V>
+__global__ void copy_pairs(const uint32_t* __restrict__ ks, const V* __restrict__ vs, uint32_t* __restrict__ kd,
+                           V* __restrict__ vd, const uint32_t* __restrict__ d_count, uint32_t max_n) {
   const uint32_t n = min(*d_count, max_n);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     kd[i] = ks[i];
@@ -348,7 +366,7 @@ __global__ void set_words(uint32_t* __restrict__ dst, Words16 words, uint32_t n)
 
 }  // namespace
 
-SortPlan sort_plan(uint32_t max_n) {
+SortPlan sort_plan(uint32_t max_n, uint32_t val_bytes) {
   SortPlan p;
   p.max_n = max_n;
   uint32_t tiles = ceil_div(max_n ? max_n : 1, kTile);
@@ -357,7 +375,7 @@ SortPlan sort_plan(uint32_t max_n) {
   p.off_alt_key = off;
   off = align_up(off + (size_t)max_n * 4, 256);
   p.off_alt_val = off;
-  off = align_up(off + (size_t)max_n * 4, 256);
+  off = align_up(off + (size_t)max_n * val_bytes, 256);
   p.off_hist = off;
   off = align_up(off + (size_t)256 * p.blocks * 4, 256);
   p.off_digit_total = off;
@@ -366,28 +384,31 @@ SortPlan sort_plan(uint32_t max_n) {
   return p;
 }
 
-hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
-                          uint32_t key_bits, void* storage, hipStream_t stream) {
+namespace {
+
+template <typename V>
+hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, V* vals, uint32_t key_bits,
+                       void* storage, hipStream_t stream) {
   if (plan.max_n == 0 || key_bits == 0) return hipSuccess;
   char* base = static_cast<char*>(storage);
   uint32_t* alt_k = reinterpret_cast<uint32_t*>(base + plan.off_alt_key);
-  uint32_t* alt_v = reinterpret_cast<uint32_t*>(base + plan.off_alt_val);
+  V* alt_v = reinterpret_cast<V*>(base + plan.off_alt_val);
   uint32_t* hist = reinterpret_cast<uint32_t*>(base + plan.off_hist);
   uint32_t* digit_total = reinterpret_cast<uint32_t*>(base + plan.off_digit_total);
   uint32_t* src_k = keys;
-  uint32_t* src_v = vals;
+  V* src_v = vals;
   uint32_t* dst_k = alt_k;
-  uint32_t* dst_v = alt_v;
+  V* dst_v = alt_v;
   if (key_bits > 32) key_bits = 32;
   for (uint32_t shift = 0; shift < key_bits; shift += 8) {
     const uint32_t bits = key_bits - shift < 8 ? key_bits - shift : 8;
     sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
     sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total);
     switch (bits) {
-#define VKGS_DOWNSWEEP(B)                                                                                     \
-  case B:                                                                                                     \
-    sort_downsweep<B><<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, plan.max_n, \
-                                                            shift, hist, digit_total);                        \
+#define VKGS_DOWNSWEEP(B)                                                                          \
+  case B:                                                                                          \
+    sort_downsweep<B, V><<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, \
+                                                               plan.max_n, shift, hist, digit_total); \
     break;
       VKGS_DOWNSWEEP(1)
       VKGS_DOWNSWEEP(2)
@@ -400,16 +421,26 @@ hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_
 #undef VKGS_DOWNSWEEP
     }
     uint32_t* tk = src_k;
-    uint32_t* tv = src_v;
+    V* tv = src_v;
     src_k = dst_k;
     src_v = dst_v;
     dst_k = tk;
     dst_v = tv;
   }
-  if (src_k != keys) {
-    copy_pairs<<<1024, 256, 0, stream>>>(src_k, src_v, keys, vals, d_count, plan.max_n);
-  }
+  if (src_k != keys) copy_pairs<V><<<1024, 256, 0, stream>>>(src_k, src_v, keys, vals, d_count, plan.max_n);
   return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
+                          uint32_t key_bits, void* storage, hipStream_t stream) {
+  return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream);
+}
+
+hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
+                            uint32_t key_bits, void* storage, hipStream_t stream) {
+  return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream);
 }
 
 ScanPlan scan_plan(uint32_t max_n) {
@@ -422,13 +453,28 @@ ScanPlan scan_plan(uint32_t max_n) {
   return p;
 }
 
+namespace {
+
+template <typename Load>
+hipError_t scan_launch(const ScanPlan& plan, Load in, uint32_t* out, const uint32_t* d_count, uint32_t* d_total,
+                       void* storage, hipStream_t stream) {
+  uint32_t* partial = reinterpret_cast<uint32_t*>(static_cast<char*>(storage) + plan.off_partial);
+  scan_reduce<Load><<<plan.blocks, kThreads, 0, stream>>>(in, d_count, plan.max_n, partial);
+  scan_single_block<<<1, kScanThreads, 0, stream>>>(partial, plan.blocks, d_total);
+  scan_apply<Load><<<plan.blocks, kThreads, 0, stream>>>(in, out, d_count, plan.max_n, partial);
+  return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t scan_exclusive_launch(const ScanPlan& plan, const uint32_t* in, uint32_t* out, const uint32_t* d_count,
                                  uint32_t* d_total, void* storage, hipStream_t stream) {
-  uint32_t* partial = reinterpret_cast<uint32_t*>(static_cast<char*>(storage) + plan.off_partial);
-  scan_reduce<<<plan.blocks, kThreads, 0, stream>>>(in, d_count, plan.max_n, partial);
-  scan_single_block<<<1, kScanThreads, 0, stream>>>(partial, plan.blocks, d_total);
-  scan_apply<<<plan.blocks, kThreads, 0, stream>>>(in, out, d_count, plan.max_n, partial);
-  return hipGetLastError();
+  return scan_launch(plan, LoadWords{in}, out, d_count, d_total, storage, stream);
+}
+
+hipError_t scan_rect_counts_launch(const ScanPlan& plan, const uint64_t* val64, const RectBits& rb, uint32_t* out,
+                                   const uint32_t* d_count, uint32_t* d_total, void* storage, hipStream_t stream) {
+  return scan_launch(plan, LoadRectCount{val64, rb}, out, d_count, d_total, storage, stream);
 }
 
 hipError_t scan_small_launch(uint32_t* data, uint32_t total, uint32_t* d_sum, hipStream_t stream) {
