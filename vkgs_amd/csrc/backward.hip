@@ -578,7 +578,11 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   if (last) atomicMax(&s_max, last);
   __syncthreads();
   const uint32_t max_last = s_max;
-  float rc = 0.f, rg = 0.f, rb = 0.f, ra = 1.f;  // colour / alpha composited behind the current entry
+  // g . (colour, alpha) composited behind the current entry: the colour
+  // behind starts at 0 and the alpha behind at 1, and both follow the blend
+  // r <- alpha * x + (1 - alpha) * r, so their g-weighted sum does too
+  float acc = g.w;
+  const float gw2 = 2.f * g.w;
 
   // traversal index i (0 = nearest) maps to list position end - 1 - i; walk i downwards
   for (uint32_t hi = max_last; hi > 0;) {
@@ -614,23 +618,22 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const float dx = fx - a.x, dy = fy - a.y;
       const float ux = fmaf(a.z, dx, a.w * dy);
       const float uy = fmaf(bb.x, dx, bb.y * dy);
-      const bool in = i < last && fabsf(ux) <= 3.f && fabsf(uy) <= 3.f;
+      const bool in = (i < last) & (fabsf(ux) <= 3.f) & (fabsf(uy) <= 3.f);
       if (__ballot(in) == 0) continue;
       // every lane takes part from here (DPP reads all lanes); lanes with
-      // !in contribute zeros and keep their state
+      // !in get G = 0, hence alpha = 0: their T (rcp(1) = 1), acc and every
+      // contribution stay unchanged or zero
       const float q = fmaf(ux, ux, uy * uy);
-      const float G = gauss_falloff(q);
+      const float G = in ? gauss_falloff(q) : 0.f;
       const float raw = bb.z * G;
-      const float alpha = in ? fminf(raw, kAlphaMax) : 0.f;
+      const float alpha = fminf(raw, kAlphaMax);
       const float om = 1.f - alpha;
-      T = in ? T * __builtin_amdgcn_rcpf(om) : T;  // transmittance in front of entry i
+      T = T * __builtin_amdgcn_rcpf(om);  // transmittance in front of entry i
       const float ta = T * alpha;
-      const float dalpha =
-          in ? T * (g.x * (c.x - rc) + g.y * (c.y - rg) + g.z * (c.z - rb) + g.w * (2.f * alpha - ra)) : 0.f;
-      rc = alpha * c.x + om * rc;
-      rg = alpha * c.y + om * rg;
-      rb = alpha * c.z + om * rb;
-      ra = alpha * alpha + om * ra;
+      // dL/dalpha = T * (g . (colour, 2 alpha) - g . (behind))
+      const float gc = fmaf(g.z, c.z, fmaf(g.y, c.y, g.x * c.x));
+      const float dalpha = T * (fmaf(gw2, alpha, gc) - acc);
+      acc = fmaf(alpha, fmaf(g.w, alpha, gc), om * acc);
       // dL/d(opacity) = dalpha * G and dL/dq = -dalpha * alpha / 2 =
       // -(opacity / 2) * dop (both 0 where alpha is clamped): the q words are
       // summed as dop * (...) and scaled by -opacity / 2 at the flush
