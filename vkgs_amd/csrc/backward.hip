@@ -10,11 +10,12 @@
 // Backward: the raster backward re-walks each tile list back to front from
 // that stored state, recomputing alpha per entry and the transmittance before
 // it as T_i = T_{i+1} / (1 - alpha_i); per entry the 9 gradient words (colour
-// 3, opacity 1, conic 3, centre 2) are summed over the tile's pixels (DPP row
-// sums, then LDS atomics) and stored as that (tile, splat) pair's partials at
-// the pair's emission index.  A splat's pairs were emitted contiguously, so a
-// reduction sums each splat's run into its 9 slot words; no global atomics
-// are used and the result does not depend on scheduling.  The projection
+// 3, opacity 1, conic 3, centre 2) are summed over the tile's pixels (DPP
+// half-row sums, then one LDS slot per half row) and stored as that (tile,
+// splat) pair's partials at the pair's slot-order index q.  The pairs of a
+// slot have consecutive q, so a reduction sums each slot's run into its 9
+// words, streaming in slot order; no global atomics are used and the result
+// does not depend on scheduling.  The projection
 // backward then runs one lane per scene splat, chains the words of every view
 // that saw the splat to its raw parameters and writes each gradient once.
 //
@@ -406,6 +407,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
                                                              const uint32_t* __restrict__ tile_order, int keep_state) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
+  constexpr bool kOneWave = kN == kWave;  // the wave covers the tile: every binned record touches it
   __shared__ float4 s_rec[2][3][kN];
 
   const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
@@ -456,15 +458,17 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
         const float4 a = s_rec[buf][0][e];
         const float4 bb = s_rec[buf][1][e];
         const float4 c = s_rec[buf][2][e];
-        const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
-        const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
-        const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
-        const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
-        if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
+        if (!kOneWave) {
+          const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
+          const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
+          const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
+          const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
+          if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
+        }
         const float dx = fx - a.x, dy = fy - a.y;
         const float ux = fmaf(a.z, dx, a.w * dy);
         const float uy = fmaf(bb.x, dx, bb.y * dy);
-        const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && alive;
+        const bool in = (fabsf(ux) <= 3.f) & (fabsf(uy) <= 3.f) & alive;
         const float q = fmaf(ux, ux, uy * uy);
         const float alpha = in ? fminf(bb.z * gauss_falloff(q), kAlphaMax) : 0.f;
         const float ta = T * alpha;
@@ -525,9 +529,9 @@ __device__ __forceinline__ float half_row_sum(float x) {
 // colour 3, opacity 1, dq*dx^2, dq*dx*dy, dq*dy^2, dq*dx, dq*dy (dq = dL/dq of
 // the exponent's quadratic form); the conic and centre words follow from the
 // last five and the entry's conic rows when the batch is flushed to the pair's
-// 9 words of gpart at its emission index.  Pairs no pixel
-// reached keep the zeros gpart was filled with; a reduction then sums each
-// splat's contiguous run of slots.
+// 9 words of gpart at its slot-order index q.  Pairs no pixel reached keep
+// the zeros gpart was filled with; a reduction then sums each slot's
+// contiguous run of pairs.
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                                       const uint32_t* __restrict__ vals,
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   static_assert(kB <= kN, "batch larger than the workgroup");
   __shared__ float4 s_rec[3][kB];
   __shared__ float s_g[9][8][kB + 1];  // per word, one partial per half DPP row; +1 spreads them over banks
-  __shared__ uint32_t s_j[kB];         // emission index of each batch entry
+  __shared__ uint32_t s_j[kB];         // slot-order index q of each batch entry
   __shared__ uint32_t s_max;
 
   const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
@@ -689,7 +693,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const float hq = -0.5f * bb.z;  // dq = hq * dop
 #pragma unroll
       for (int j = 4; j < 9; ++j) u[j] *= hq;
-      float* out = gpart + 9 * (size_t)s_j[t];  // at the pair's emission index
+      float* out = gpart + 9 * (size_t)s_j[t];  // at the pair's slot-order index
       out[0] = u[0];
       out[1] = u[1];
       out[2] = u[2];
