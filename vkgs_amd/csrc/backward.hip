@@ -529,9 +529,9 @@ __device__ __forceinline__ float half_row_sum(float x) {
 // colour 3, opacity 1, dq*dx^2, dq*dx*dy, dq*dy^2, dq*dx, dq*dy (dq = dL/dq of
 // the exponent's quadratic form); the conic and centre words follow from the
 // last five and the entry's conic rows when the batch is flushed to the pair's
-// 9 words of gpart at its slot-order index q.  Pairs no pixel reached keep
-// the zeros gpart was filled with; a reduction then sums each slot's
-// contiguous run of pairs.
+// 9 words of gpart at its slot-order index q; the tile's entries past every
+// pixel's last one get zeros, so each pair is written once and gpart needs no
+// fill.  A reduction then sums each slot's contiguous run of pairs.
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                                       const uint32_t* __restrict__ vals,
@@ -706,16 +706,14 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
     }
     hi = lo;
   }
-}
-
-__global__ __launch_bounds__(256) void diff_fill_zero_kernel(float* __restrict__ g, uint64_t n) {
-  const uint64_t n4 = n / 4;
-  float4* g4 = reinterpret_cast<float4*>(g);
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n4; i += stride)
-    g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint64_t i = 4 * n4 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (i < n) g[i] = 0.f;
+  // entries beyond every pixel's last one were never composited: their
+  // partials are zero (each pair is written exactly once, here or above)
+  const uint32_t items = end - ranges[2 * gtile];
+  for (uint32_t i = max_last + t; i < items; i += kN) {
+    float* out = gpart + 9 * (size_t)vals[end - 1 - i];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) out[j] = 0.f;
+  }
 }
 
 // Sum of x over the 64 lanes of a wave (all lanes active); lane 63 gets it.
@@ -1186,7 +1184,6 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
                                        const uint32_t* d_count, uint32_t max_n, float* gpart, float* g9,
                                        hipStream_t s) {
   // partials of pairs no pixel reached stay zero
-  if (pairs > 0) diff_fill_zero_kernel<<<grid_for(pairs * 9 / 4 + 1, 256, 8192), 256, 0, s>>>(gpart, pairs * 9);
   if (L.n_tiles > 0) {
     bool launched = false;
 #define VKGS_CASE(W_, H_)                                                                                  \

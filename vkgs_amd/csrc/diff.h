@@ -99,7 +99,8 @@ hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint
                               const uint32_t* pslot, const uint32_t* tile_order, bool keep_state, hipStream_t s);
 // Blend backward: per sorted pair 9 partial gradient words written at the
 // pair's q (its slot-order index, the sorted value) in gpart (9 floats per
-// pair, pairs = live pair count, zeroed first), then g9[slot] = the sum over
+// pair, pairs = live pair count; every pair written once, zeros for entries
+// no pixel reached), then g9[slot] = the sum over
 // the slot's pairs soffs[slot] .. soffs[slot] + scnt[slot] - 1 for slots <
 // *d_count.  No global atomics.
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
