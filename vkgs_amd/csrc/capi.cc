@@ -86,6 +86,21 @@ int bits_for(uint32_t v) {  // bits needed for values in [0, v)
 
 }  // namespace
 
+// Where binning writes its (tile, value) pairs: the tile sort's alternate
+// buffers when the sort takes an odd number of passes, so that the sorted
+// pairs land in b.keys / b.vals without a final copy; else b.keys / b.vals.
+static void pair_targets(const DrawBuffers& b, uint32_t tile_bits, uint32_t** keys, uint32_t** vals) {
+  *keys = b.keys;
+  *vals = b.vals;
+  if (sort_passes(tile_bits) % 2 == 1) sort_alt_buffers(b.sort, b.sort_storage, keys, vals);
+}
+
+static hipError_t sort_pairs(const DrawBuffers& b, uint32_t tile_bits, hipStream_t s) {
+  if (sort_passes(tile_bits) % 2 == 1)
+    return sort_kv_from_alt_launch(b.sort, b.pairs, b.keys, b.vals, tile_bits, b.sort_storage, s);
+  return sort_kv_launch(b.sort, b.pairs, b.keys, b.vals, tile_bits, b.sort_storage, s);
+}
+
 // Raster with the longest-list-first tile schedule when its buffers are given.
 // Longest-list-first tile order (when b.tile_order is set), then the raster;
 // `mark` (optional) is recorded between the two.
@@ -112,14 +127,15 @@ hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d
   if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
   if ((e = launch_bin_count(inst, order, d_count, max_inst, rs, b.rec, b.rect, b.cnt, s)) != hipSuccess) return e;
   if ((e = scan_exclusive_launch(b.scan, b.cnt, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess) return e;
-  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, nullptr, d_count, max_inst, rs, max_pairs, b.keys, b.vals, b.status,
-                           s)) != hipSuccess)
-    return e;
-  if (marks) (void)hipEventRecord(marks[0], s);
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
-  if ((e = sort_kv_launch(b.sort, b.pairs, b.keys, b.vals, (uint32_t)bits_for(tiles), b.sort_storage, s)) !=
+  const uint32_t tile_bits = (uint32_t)bits_for(tiles);
+  uint32_t *pk, *pv;
+  pair_targets(b, tile_bits, &pk, &pv);
+  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, nullptr, d_count, max_inst, rs, max_pairs, pk, pv, b.status, s)) !=
       hipSuccess)
     return e;
+  if (marks) (void)hipEventRecord(marks[0], s);
+  if ((e = sort_pairs(b, tile_bits, s)) != hipSuccess) return e;
   if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
   return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }
@@ -134,14 +150,15 @@ hipError_t run_draw_slots(const uint4* rc, const uint32_t* order, const uint32_t
   if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
   if ((e = launch_bin_gather(rc, order, d_count, max_inst, b.rect, b.cnt, s)) != hipSuccess) return e;
   if ((e = scan_exclusive_launch(b.scan, b.cnt, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess) return e;
-  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, order, d_count, max_inst, rs, max_pairs, b.keys, b.vals, b.status,
-                           s)) != hipSuccess)
-    return e;
-  if (marks) (void)hipEventRecord(marks[0], s);
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
-  if ((e = sort_kv_launch(b.sort, b.pairs, b.keys, b.vals, (uint32_t)bits_for(tiles), b.sort_storage, s)) !=
+  const uint32_t tile_bits = (uint32_t)bits_for(tiles);
+  uint32_t *pk, *pv;
+  pair_targets(b, tile_bits, &pk, &pv);
+  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, order, d_count, max_inst, rs, max_pairs, pk, pv, b.status, s)) !=
       hipSuccess)
     return e;
+  if (marks) (void)hipEventRecord(marks[0], s);
+  if ((e = sort_pairs(b, tile_bits, s)) != hipSuccess) return e;
   if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
   return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }
@@ -153,14 +170,15 @@ hipError_t run_draw_packed(const uint64_t* val64, const uint32_t* d_count, uint3
   if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
   if ((e = scan_rect_counts_launch(b.scan, val64, rb, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess)
     return e;
-  if ((e = launch_bin_emit_packed(val64, b.offs, d_count, max_inst, rs, rb, max_pairs, b.keys, b.vals, b.status,
-                                  s)) != hipSuccess)
-    return e;
-  if (marks) (void)hipEventRecord(marks[0], s);
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
-  if ((e = sort_kv_launch(b.sort, b.pairs, b.keys, b.vals, (uint32_t)bits_for(tiles), b.sort_storage, s)) !=
+  const uint32_t tile_bits = (uint32_t)bits_for(tiles);
+  uint32_t *pk, *pv;
+  pair_targets(b, tile_bits, &pk, &pv);
+  if ((e = launch_bin_emit_packed(val64, b.offs, d_count, max_inst, rs, rb, max_pairs, pk, pv, b.status, s)) !=
       hipSuccess)
     return e;
+  if (marks) (void)hipEventRecord(marks[0], s);
+  if ((e = sort_pairs(b, tile_bits, s)) != hipSuccess) return e;
   if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
   return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }

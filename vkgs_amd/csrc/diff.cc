@@ -297,12 +297,23 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
                                         c->cnt.as<uint32_t>(), c->kview.as<uint32_t>(), s));
   VKGS_HIP_CHECK(scan_exclusive_launch(c->scan, c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(), misc + kMiscVisible,
                                        misc + kMiscPairs, c->scan_storage.p, s));
+  // with an odd number of tile-sort passes the pairs are emitted into the
+  // sort's alternate buffers, so the sorted result lands in pkeys / pvals
+  // without a final copy
+  const uint32_t tile_bits = (uint32_t)bits_for((uint32_t)tiles);
+  const bool odd = sort_passes(tile_bits) % 2 == 1;
+  uint32_t *ek = c->pkeys.as<uint32_t>(), *ev = c->pvals.as<uint32_t>();
+  if (odd) sort_alt_buffers(c->pair_plan, c->pair_sort.p, &ek, &ev);
   VKGS_HIP_CHECK(launch_diff_bin_emit(L, c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),
-                                      c->kview.as<uint32_t>(), order, misc + kMiscVisible, cap, c->max_pairs,
-                                      c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(), c->pslot.as<uint32_t>(),
-                                      c->soffs.as<uint32_t>(), c->scnt.as<uint32_t>(), misc + kMiscStatus, s));
-  VKGS_HIP_CHECK(sort_kv_launch(c->pair_plan, misc + kMiscPairs, c->pkeys.as<uint32_t>(), c->pvals.as<uint32_t>(),
-                                (uint32_t)bits_for((uint32_t)tiles), c->pair_sort.p, s));
+                                      c->kview.as<uint32_t>(), order, misc + kMiscVisible, cap, c->max_pairs, ek, ev,
+                                      c->pslot.as<uint32_t>(), c->soffs.as<uint32_t>(), c->scnt.as<uint32_t>(),
+                                      misc + kMiscStatus, s));
+  if (odd)
+    VKGS_HIP_CHECK(sort_kv_from_alt_launch(c->pair_plan, misc + kMiscPairs, c->pkeys.as<uint32_t>(),
+                                           c->pvals.as<uint32_t>(), tile_bits, c->pair_sort.p, s));
+  else
+    VKGS_HIP_CHECK(sort_kv_launch(c->pair_plan, misc + kMiscPairs, c->pkeys.as<uint32_t>(),
+                                  c->pvals.as<uint32_t>(), tile_bits, c->pair_sort.p, s));
   VKGS_HIP_CHECK(launch_ranges(c->pkeys.as<uint32_t>(), misc + kMiscPairs, c->max_pairs, (uint32_t)tiles,
                                c->ranges.as<uint32_t>(), s));
   VKGS_HIP_CHECK(fill_u32_launch(misc + kMiscTiles, (uint32_t)tiles, 1, s));

@@ -23,6 +23,15 @@ SortPlan sort_plan(uint32_t max_n, uint32_t val_bytes = 4);
 // [0, key_bits), result in (keys, vals).
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
                           uint32_t key_bits, void* storage, hipStream_t stream);
+// Radix passes of a key_bits-bit sort (8 bits per pass).
+uint32_t sort_passes(uint32_t key_bits);
+// The storage's alternate (key, value) buffers of a 4-byte-value plan.
+void sort_alt_buffers(const SortPlan& plan, void* storage, uint32_t** keys, uint32_t** vals);
+// Same sort with the unsorted pairs in the alternate buffers; the result is
+// in (keys, vals).  With an odd pass count this needs no final copy, so a
+// producer that writes its pairs to the alternate buffers saves one.
+hipError_t sort_kv_from_alt_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
+                                   uint32_t key_bits, void* storage, hipStream_t stream);
 // Same with 64-bit values (plan from sort_plan(max_n, 8)).
 hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
                             uint32_t key_bits, void* storage, hipStream_t stream);

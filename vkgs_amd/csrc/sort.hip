@@ -388,18 +388,22 @@ namespace {
 
 template <typename V>
 hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, V* vals, uint32_t key_bits,
-                       void* storage, hipStream_t stream) {
-  if (plan.max_n == 0 || key_bits == 0) return hipSuccess;
+                       void* storage, hipStream_t stream, bool input_in_alt = false) {
+  if (key_bits > 32) key_bits = 32;
   char* base = static_cast<char*>(storage);
   uint32_t* alt_k = reinterpret_cast<uint32_t*>(base + plan.off_alt_key);
   V* alt_v = reinterpret_cast<V*>(base + plan.off_alt_val);
+  if (plan.max_n == 0) return hipSuccess;
+  if (key_bits == 0) {
+    if (input_in_alt) copy_pairs<V><<<1024, 256, 0, stream>>>(alt_k, alt_v, keys, vals, d_count, plan.max_n);
+    return hipGetLastError();
+  }
   uint32_t* hist = reinterpret_cast<uint32_t*>(base + plan.off_hist);
   uint32_t* digit_total = reinterpret_cast<uint32_t*>(base + plan.off_digit_total);
-  uint32_t* src_k = keys;
-  V* src_v = vals;
-  uint32_t* dst_k = alt_k;
-  V* dst_v = alt_v;
-  if (key_bits > 32) key_bits = 32;
+  uint32_t* src_k = input_in_alt ? alt_k : keys;
+  V* src_v = input_in_alt ? alt_v : vals;
+  uint32_t* dst_k = input_in_alt ? keys : alt_k;
+  V* dst_v = input_in_alt ? vals : alt_v;
   for (uint32_t shift = 0; shift < key_bits; shift += 8) {
     const uint32_t bits = key_bits - shift < 8 ? key_bits - shift : 8;
     sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
@@ -436,6 +440,19 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
                           uint32_t key_bits, void* storage, hipStream_t stream) {
   return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream);
+}
+
+uint32_t sort_passes(uint32_t key_bits) { return ((key_bits > 32 ? 32 : key_bits) + 7) / 8; }
+
+void sort_alt_buffers(const SortPlan& plan, void* storage, uint32_t** keys, uint32_t** vals) {
+  char* base = static_cast<char*>(storage);
+  *keys = reinterpret_cast<uint32_t*>(base + plan.off_alt_key);
+  *vals = reinterpret_cast<uint32_t*>(base + plan.off_alt_val);
+}
+
+hipError_t sort_kv_from_alt_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
+                                   uint32_t key_bits, void* storage, hipStream_t stream) {
+  return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream, true);
 }
 
 hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
