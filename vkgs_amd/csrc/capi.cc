@@ -32,7 +32,6 @@ enum : uint32_t {
   kHdrMagic = 4,
   kHdrOffVals = 5,    // byte offset of the tile-sorted instance list (2 words)
   kHdrOffRanges = 7,  // byte offset of the per-tile ranges (2 words)
-  kHdrTiles = 9,      // tile count (read by the tile-schedule sort)
   kHdrWords = 64
 };
 constexpr uint32_t kDrawMagic = 0x564b4753u;  // "VKGS"
@@ -107,12 +106,7 @@ static hipError_t sort_pairs(const DrawBuffers& b, uint32_t tile_bits, hipStream
 static hipError_t raster_scheduled(const DrawBuffers& b, const RasterShape& rs, uint32_t tiles, float4* out,
                                    hipStream_t s, hipEvent_t mark) {
   if (b.tile_order) {
-    Words16 w = {};
-    w.w[0] = tiles;
-    hipError_t e = set_words_launch(b.tile_count, w, 1, s);
-    if (e != hipSuccess) return e;
-    e = launch_tile_order(b.ranges, tiles, b.tile_keys, b.tile_order, b.tile_count, b.tile_sort,
-                          b.tile_sort_storage, s);
+    hipError_t e = launch_tile_order(b.ranges, tiles, b.tile_keys, b.tile_order, b.tile_sort, b.tile_sort_storage, s);
     if (e != hipSuccess) return e;
   }
   if (mark) (void)hipEventRecord(mark, s);
@@ -124,19 +118,19 @@ hipError_t run_draw(const float4* inst, const uint32_t* order, const uint32_t* d
                     const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
                     hipEvent_t* marks) {
   hipError_t e;
-  if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
+  if (!b.status_cleared && (e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
   if ((e = launch_bin_count(inst, order, d_count, max_inst, rs, b.rec, b.rect, b.cnt, s)) != hipSuccess) return e;
   if ((e = scan_exclusive_launch(b.scan, b.cnt, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess) return e;
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tile_bits = (uint32_t)bits_for(tiles);
   uint32_t *pk, *pv;
   pair_targets(b, tile_bits, &pk, &pv);
-  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, nullptr, d_count, max_inst, rs, max_pairs, pk, pv, b.status, s)) !=
-      hipSuccess)
+  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, nullptr, d_count, max_inst, rs, max_pairs, pk, pv, b.status, s,
+                           b.ranges, 2 * tiles)) != hipSuccess)
     return e;
   if (marks) (void)hipEventRecord(marks[0], s);
   if ((e = sort_pairs(b, tile_bits, s)) != hipSuccess) return e;
-  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
+  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s, true)) != hipSuccess) return e;
   return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }
 
@@ -147,19 +141,19 @@ hipError_t run_draw_slots(const uint4* rc, const uint32_t* order, const uint32_t
                           const RasterShape& rs, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
                           hipEvent_t* marks) {
   hipError_t e;
-  if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
+  if (!b.status_cleared && (e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
   if ((e = launch_bin_gather(rc, order, d_count, max_inst, b.rect, b.cnt, s)) != hipSuccess) return e;
   if ((e = scan_exclusive_launch(b.scan, b.cnt, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess) return e;
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tile_bits = (uint32_t)bits_for(tiles);
   uint32_t *pk, *pv;
   pair_targets(b, tile_bits, &pk, &pv);
-  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, order, d_count, max_inst, rs, max_pairs, pk, pv, b.status, s)) !=
-      hipSuccess)
+  if ((e = launch_bin_emit(b.rect, b.cnt, b.offs, order, d_count, max_inst, rs, max_pairs, pk, pv, b.status, s,
+                           b.ranges, 2 * tiles)) != hipSuccess)
     return e;
   if (marks) (void)hipEventRecord(marks[0], s);
   if ((e = sort_pairs(b, tile_bits, s)) != hipSuccess) return e;
-  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
+  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s, true)) != hipSuccess) return e;
   return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }
 
@@ -167,19 +161,19 @@ hipError_t run_draw_packed(const uint64_t* val64, const uint32_t* d_count, uint3
                            const RectBits& rb, uint64_t max_pairs, const DrawBuffers& b, float4* out, hipStream_t s,
                            hipEvent_t* marks) {
   hipError_t e;
-  if ((e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
+  if (!b.status_cleared && (e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
   if ((e = scan_rect_counts_launch(b.scan, val64, rb, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess)
     return e;
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tile_bits = (uint32_t)bits_for(tiles);
   uint32_t *pk, *pv;
   pair_targets(b, tile_bits, &pk, &pv);
-  if ((e = launch_bin_emit_packed(val64, b.offs, d_count, max_inst, rs, rb, max_pairs, pk, pv, b.status, s)) !=
-      hipSuccess)
+  if ((e = launch_bin_emit_packed(val64, b.offs, d_count, max_inst, rs, rb, max_pairs, pk, pv, b.status, s,
+                                  b.ranges, 2 * tiles)) != hipSuccess)
     return e;
   if (marks) (void)hipEventRecord(marks[0], s);
   if ((e = sort_pairs(b, tile_bits, s)) != hipSuccess) return e;
-  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s)) != hipSuccess) return e;
+  if ((e = launch_ranges(b.keys, b.pairs, max_pairs, tiles, b.ranges, s, true)) != hipSuccess) return e;
   return raster_scheduled(b, rs, tiles, out, s, marks ? marks[1] : nullptr);
 }
 
@@ -287,10 +281,10 @@ int vkgs_draw(const float* d_instances, const uint32_t* d_draw_indirect, uint32_
   db.sort_storage = b + p.off_sort;
   db.ranges = reinterpret_cast<uint32_t*>(b + p.off_ranges);
   db.status = hdr + kHdrStatus;
+  db.status_cleared = true;  // the header words above include status = 0
   db.pairs = hdr + kHdrPairs;
   db.tile_keys = reinterpret_cast<uint32_t*>(b + p.off_tile_keys);
   db.tile_order = reinterpret_cast<uint32_t*>(b + p.off_tile_order);
-  db.tile_count = hdr + kHdrTiles;
   db.tile_sort = p.tile_sort;
   db.tile_sort_storage = b + p.off_tile_sort;
   VKGS_HIP_CHECK(run_draw(reinterpret_cast<const float4*>(d_instances), nullptr, d_draw_indirect + 8, max_instances,

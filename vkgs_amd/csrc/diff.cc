@@ -75,7 +75,7 @@ struct vkgs_diff_ctx {
 
 namespace {
 
-enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscTiles = 3, kMiscSlotPairs = 4 };
+enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscSlotPairs = 3 };
 
 template <typename T>
 size_t pack(std::vector<unsigned char>& out, const std::vector<T>& v) {
@@ -283,7 +283,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   // ---- forward ----
   uint32_t* misc = c->misc.as<uint32_t>();
   uint32_t* order = c->val.as<uint32_t>();
-  VKGS_HIP_CHECK(fill_u32_launch(misc, 0u, 5, s));
+  VKGS_HIP_CHECK(fill_u32_launch(misc, 0u, 4, s));
   VKGS_HIP_CHECK(launch_diff_count(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), misc + kMiscVisible,
                                    c->count_scan.p, s));
   VKGS_HIP_CHECK(launch_diff_write(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), c->rec.as<float4>(),
@@ -316,9 +316,8 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
                                   c->pvals.as<uint32_t>(), tile_bits, c->pair_sort.p, s));
   VKGS_HIP_CHECK(launch_ranges(c->pkeys.as<uint32_t>(), misc + kMiscPairs, c->max_pairs, (uint32_t)tiles,
                                c->ranges.as<uint32_t>(), s));
-  VKGS_HIP_CHECK(fill_u32_launch(misc + kMiscTiles, (uint32_t)tiles, 1, s));
   VKGS_HIP_CHECK(launch_tile_order(c->ranges.as<uint32_t>(), (uint32_t)tiles, c->tile_keys.as<uint32_t>(),
-                                   c->tile_order.as<uint32_t>(), misc + kMiscTiles, c->tile_plan, c->tile_sort.p, s));
+                                   c->tile_order.as<uint32_t>(), c->tile_plan, c->tile_sort.p, s));
   VKGS_HIP_CHECK(launch_diff_raster(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(), c->ranges.as<uint32_t>(),
                                     c->pslot.as<uint32_t>(), c->tile_order.as<uint32_t>(), true, s));
   c->have_forward = true;

@@ -19,7 +19,7 @@ using namespace vkgs;
 
 namespace {
 
-enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscTiles = 3, kMiscWords = 64 };
+enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscWords = 64 };
 constexpr int kEvents = 7;
 constexpr int kStages = 8;
 
@@ -160,10 +160,10 @@ DrawBuffers draw_buffers(vkgs_renderer* r) {
   b.sort_storage = r->psort.p;
   b.ranges = r->ranges.as<uint32_t>();
   b.status = misc + kMiscStatus;
+  b.status_cleared = true;  // the frame-constant launch zeroes the misc words
   b.pairs = misc + kMiscPairs;
   b.tile_keys = r->tile_keys.as<uint32_t>();
   b.tile_order = r->tile_order.as<uint32_t>();
-  b.tile_count = misc + kMiscTiles;
   b.tile_sort = r->tile_plan;
   b.tile_sort_storage = r->tile_sort.p;
   return b;
@@ -175,7 +175,6 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
   const uint32_t n = r->n;
   uint32_t* misc = r->misc.as<uint32_t>();
   if (timed) hipEventRecord(r->ev[0], s);
-  if ((e = fill_u32_launch(misc, 0u, 4, s)) != hipSuccess) return e;
   if ((e = launch_fused_count(r->fc.as<FrameConst>(), r->position.as<float>(), n, misc + kMiscVisible,
                               r->counts.as<uint32_t>(), s)) != hipSuccess)
     return e;
@@ -341,7 +340,8 @@ int vkgs_renderer_render(vkgs_renderer* r, const vkgs_camera* camera, const floa
   fc.tile_h = r->rs.tile_h;
   fc.tiles_x = r->rs.tiles_x;
   fc.tiles_y = r->rs.tiles_y;
-  VKGS_HIP_CHECK(launch_set_frame_const(r->fc.as<FrameConst>(), fc, s));
+  // ... which also zeroes the frame's counters (visible, pairs, status)
+  VKGS_HIP_CHECK(launch_set_frame_const(r->fc.as<FrameConst>(), fc, s, r->misc.as<uint32_t>(), 4));
   if (r->timing) {
     VKGS_HIP_CHECK(record_frame(r, d_rgba, s, true));
     VKGS_HIP_CHECK(hipEventSynchronize(r->ev[6]));

@@ -51,8 +51,10 @@ hipError_t launch_fused_write_packed(const FrameConst* d_fc, const float* pos, c
                                      uint32_t* key, uint64_t* val64, uint32_t* ids, const uint32_t* counts,
                                      hipStream_t s);
 
-// Copies a FrameConst passed by value into device memory (stream-ordered).
-hipError_t launch_set_frame_const(FrameConst* d_fc, const FrameConst& fc, hipStream_t s);
+// Copies a FrameConst passed by value into device memory (stream-ordered)
+// and zeroes n_clear words at clear (optional).
+hipError_t launch_set_frame_const(FrameConst* d_fc, const FrameConst& fc, hipStream_t s, uint32_t* clear = nullptr,
+                                  uint32_t n_clear = 0);
 
 // Binning and raster -----------------------------------------------------
 
@@ -75,20 +77,25 @@ hipError_t launch_bin_gather(const uint4* rc, const uint32_t* order, const uint3
                              TileRect* rect, uint32_t* cnt, hipStream_t s);
 // (tile, k) pairs at offs[k]; pairs beyond max_pairs set *status.  With
 // `order` the pair value is order[k] instead of k.
+// clear (optional): n_clear words zeroed by the same launch (the tile ranges).
 hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* order,
                            const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
-                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s);
+                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s,
+                           uint32_t* clear = nullptr, uint32_t n_clear = 0);
 // (tile, slot) pairs at offs[k] from depth-sorted values val64[k] = slot |
 // packed rect << 32 (offs = exclusive scan of the packed tile counts).
 hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
                                   const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
-                                  uint32_t* vals, uint32_t* status, hipStream_t s);
+                                  uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear = nullptr,
+                                  uint32_t n_clear = 0);
+// [start, end) per tile from the tile-sorted keys; `cleared`: the ranges were
+// already zeroed (by the emit launch).
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
-                         uint32_t* ranges, hipStream_t s);
-// Tiles ordered longest list first (stable by tile id among equal lengths);
-// *d_tiles holds the tile count for the sort.
+                         uint32_t* ranges, hipStream_t s, bool cleared = false);
+// Tiles ordered longest list first, by a log-spaced bucket of the list
+// length (stable by tile id within a bucket); one 8-bit sort pass.
 hipError_t launch_tile_order(const uint32_t* ranges, uint32_t tiles, uint32_t* keys, uint32_t* order,
-                             const uint32_t* d_tiles, const SortPlan& plan, void* sort_storage, hipStream_t s);
+                             const SortPlan& plan, void* sort_storage, hipStream_t s);
 // tile_order (optional): workgroup i rasterizes tile tile_order[i].
 hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
                             const uint32_t* tile_order, float4* out, hipStream_t s);
@@ -107,11 +114,11 @@ struct DrawBuffers {
   void* sort_storage;
   uint32_t* ranges;  // [start, end) per tile
   uint32_t* status;  // overflow flag
+  bool status_cleared;  // the caller zeroed *status for this draw
   uint32_t* pairs;   // pair count
   // longest-first tile schedule (optional: tile_order == nullptr disables it)
   uint32_t* tile_keys;
   uint32_t* tile_order;
-  uint32_t* tile_count;  // device word = number of tiles
   SortPlan tile_sort;
   void* tile_sort_storage;
 };

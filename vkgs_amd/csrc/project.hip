@@ -366,17 +366,20 @@ __global__ __launch_bounds__(256) void projection_kernel(FrameConst fc, const fl
   inst[3 * (size_t)slot + 2] = o[2];
 }
 
-__global__ void set_frame_const_kernel(FrameConst* __restrict__ dst, FrameConst fc) {
+__global__ void set_frame_const_kernel(FrameConst* __restrict__ dst, FrameConst fc, uint32_t* __restrict__ clear,
+                                       uint32_t n_clear) {
   constexpr uint32_t kWords = sizeof(FrameConst) / 4;
   const uint32_t* src = reinterpret_cast<const uint32_t*>(&fc);
   uint32_t* d = reinterpret_cast<uint32_t*>(dst);
   for (uint32_t i = threadIdx.x; i < kWords; i += blockDim.x) d[i] = src[i];
+  for (uint32_t i = threadIdx.x; i < n_clear; i += blockDim.x) clear[i] = 0u;
 }
 
 }  // namespace
 
-hipError_t launch_set_frame_const(FrameConst* d_fc, const FrameConst& fc, hipStream_t s) {
-  set_frame_const_kernel<<<1, 128, 0, s>>>(d_fc, fc);
+hipError_t launch_set_frame_const(FrameConst* d_fc, const FrameConst& fc, hipStream_t s, uint32_t* clear,
+                                  uint32_t n_clear) {
+  set_frame_const_kernel<<<1, 128, 0, s>>>(d_fc, fc, clear, clear ? n_clear : 0u);
   return hipGetLastError();
 }
 

@@ -86,7 +86,9 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
                                                        const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                        uint32_t tiles_x, uint64_t max_pairs,
                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                       uint32_t* __restrict__ status) {
+                                                       uint32_t* __restrict__ status, uint32_t* __restrict__ clear,
+                                                       uint32_t n_clear) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_clear; i += gridDim.x * blockDim.x) clear[i] = 0u;
   const uint32_t n = min(*d_count, max_n);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -116,7 +118,9 @@ __global__ __launch_bounds__(256) void bin_emit_packed_kernel(const uint64_t* __
                                                               const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                               uint32_t tiles_x, RectBits rb, uint64_t max_pairs,
                                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                              uint32_t* __restrict__ status) {
+                                                              uint32_t* __restrict__ status,
+                                                              uint32_t* __restrict__ clear, uint32_t n_clear) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_clear; i += gridDim.x * blockDim.x) clear[i] = 0u;
   const uint32_t n = min(*d_count, max_n);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -154,14 +158,19 @@ __global__ void ranges_mark_kernel(const uint32_t* __restrict__ keys, const uint
   }
 }
 
-// Sort key of each tile for the longest-first schedule: 0xffff - list length
-// (clamped), value = tile id; an ascending stable sort puts long lists first.
+// Sort key of each tile for the longest-first schedule: 255 - a log-spaced
+// bucket of the list length (8 buckets per octave: 8 e + the 3 bits below the
+// leading one of len + 1, e its exponent), value = tile id; an ascending
+// stable one-pass sort puts long lists first.  Lengths within 1/8 of an
+// octave share a bucket, which is fine for a schedule.
 __global__ void tile_order_keys_kernel(const uint32_t* __restrict__ ranges, uint32_t tiles,
                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tiles) return;
-  const uint32_t len = ranges[2 * t + 1] - ranges[2 * t];
-  keys[t] = 0xffffu - (len < 0xffffu ? len : 0xffffu);
+  const uint32_t v = ranges[2 * t + 1] - ranges[2 * t] + 1u;  // >= 1 (a list holds fewer than 2^32 pairs)
+  const uint32_t e = 31u - __clz(v);
+  const uint32_t m = (e >= 3u ? v >> (e - 3u) : v << (3u - e)) & 7u;
+  keys[t] = 255u - (8u * e + m);
   vals[t] = t;
 }
 
@@ -385,37 +394,45 @@ hipError_t launch_bin_gather(const uint4* rc, const uint32_t* order, const uint3
 
 hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint32_t* offs, const uint32_t* order,
                            const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
-                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s) {
-  if (max_n == 0) return hipSuccess;
+                           uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear,
+                           uint32_t n_clear) {
+  if (max_n == 0) return clear && n_clear ? fill_u32_launch(clear, 0u, n_clear, s) : hipSuccess;
   bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rect, cnt, offs, order, d_count, max_n, rs.tiles_x,
-                                                              max_pairs, keys, vals, status);
+                                                              max_pairs, keys, vals, status, clear,
+                                                              clear ? n_clear : 0u);
   return hipGetLastError();
 }
 
 hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
                                   const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
-                                  uint32_t* vals, uint32_t* status, hipStream_t s) {
-  if (max_n == 0) return hipSuccess;
+                                  uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear,
+                                  uint32_t n_clear) {
+  if (max_n == 0) return clear && n_clear ? fill_u32_launch(clear, 0u, n_clear, s) : hipSuccess;
   bin_emit_packed_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(val64, offs, d_count, max_n, rs.tiles_x, rb,
-                                                                     max_pairs, keys, vals, status);
+                                                                     max_pairs, keys, vals, status, clear,
+                                                                     clear ? n_clear : 0u);
   return hipGetLastError();
 }
 
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
-                         uint32_t* ranges, hipStream_t s) {
-  ranges_fill_kernel<<<grid_for(2ull * tiles, 256, 1u << 20), 256, 0, s>>>(ranges, 2 * tiles);
+                         uint32_t* ranges, hipStream_t s, bool cleared) {
+  if (!cleared) ranges_fill_kernel<<<grid_for(2ull * tiles, 256, 1u << 20), 256, 0, s>>>(ranges, 2 * tiles);
   if (max_pairs > 0)
     ranges_mark_kernel<<<grid_for(max_pairs, 256, 8192), 256, 0, s>>>(keys, d_pairs, max_pairs, ranges);
   return hipGetLastError();
 }
 
 hipError_t launch_tile_order(const uint32_t* ranges, uint32_t tiles, uint32_t* keys, uint32_t* order,
-                              const uint32_t* d_tiles, const SortPlan& plan, void* sort_storage, hipStream_t s) {
+                              const SortPlan& plan, void* sort_storage, hipStream_t s) {
   if (tiles == 0) return hipSuccess;
-  tile_order_keys_kernel<<<ceil_div(tiles, 256), 256, 0, s>>>(ranges, tiles, keys, order);
+  // one 8-bit pass: the keys go to the sort's alternate buffers so that the
+  // result lands in (keys, order) without a copy
+  uint32_t *ak, *av;
+  sort_alt_buffers(plan, sort_storage, &ak, &av);
+  tile_order_keys_kernel<<<ceil_div(tiles, 256), 256, 0, s>>>(ranges, tiles, ak, av);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return sort_kv_launch(plan, d_tiles, keys, order, 16u, sort_storage, s);
+  return sort_kv_from_alt_launch(plan, nullptr, keys, order, 8u, sort_storage, s);
 }
 
 hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,

@@ -19,8 +19,8 @@ struct SortPlan {
 // Storage plan for up to max_n pairs with val_bytes-byte values (4 or 8).
 SortPlan sort_plan(uint32_t max_n, uint32_t val_bytes = 4);
 
-// Stable ascending sort of min(*d_count, max_n) pairs on key bits
-// [0, key_bits), result in (keys, vals).
+// Stable ascending sort of min(*d_count, max_n) pairs (d_count == nullptr:
+// max_n) on key bits [0, key_bits), result in (keys, vals).
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
                           uint32_t key_bits, void* storage, hipStream_t stream);
 // Radix passes of a key_bits-bit sort (8 bits per pass).

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restr
   uint32_t* h = s_h[(tid / kWave) * kCopies + (tid % kCopies)];
   for (uint32_t i = tid; i < kWaves * kCopies * 256; i += kThreads) (&s_h[0][0])[i] = 0u;
   __syncthreads();
-  const uint32_t n = min(*d_count, max_n);
+  const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
   const uint32_t mask = (1u << bits) - 1u;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
   __shared__ V s_val[kTile];
 
   const uint32_t tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
-  const uint32_t n = min(*d_count, max_n);
+  const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
   const uint32_t mask = (1u << bits) - 1u;
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, c
 template <typename V>
 __global__ void copy_pairs(const uint32_t* __restrict__ ks, const V* __restrict__ vs, uint32_t* __restrict__ kd,
                            V* __restrict__ vd, const uint32_t* __restrict__ d_count, uint32_t max_n) {
-  const uint32_t n = min(*d_count, max_n);
+  const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     kd[i] = ks[i];
     vd[i] = vs[i];
