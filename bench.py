@@ -95,14 +95,21 @@ def batch_workload(scale: float = 1.0, seed: int = 5):
 
 
 def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
-    """views/s of fwd+bwd (L1 loss) for the batch workload: one launch set per view
-    (plain) and one launch set for all views (batched)."""
+    """views/s of fwd+bwd (L1 loss, the mean over each view summed over views) for the batch
+    workload: one launch set and one loss per view (plain), and one launch set for all views
+    with the loss formed over their flat (P, 4) output (batched)."""
     import torch
     from vkgs_amd import diff as D
 
     scenes, views = batch_workload(scale)
     params = [D.scene_tensors(s) for s in scenes]
     targets = [torch.rand((c.screen_size[1], c.screen_size[0], 4), device="cuda") for _, c in views]
+    # the batched form's loss over the flat (P, 4) output: the same sum of per-view L1 means, with
+    # each pixel weighted by 1 / (4 H W) of its view
+    tgt_flat = torch.cat([t.reshape(-1, 4) for t in targets])
+    w_flat = torch.cat([torch.full((c.screen_size[0] * c.screen_size[1], 1),
+                                   1.0 / (4 * c.screen_size[0] * c.screen_size[1]), device="cuda")
+                        for _, c in views])
 
     def plain_step():
         for (si, c), tgt in zip(views, targets):
@@ -110,8 +117,8 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
             (img - tgt).abs().mean().backward()
 
     def batched_step():
-        imgs = D.render_batch(params, views)
-        sum((img - tgt).abs().mean() for img, tgt in zip(imgs, targets)).backward()
+        flat, _ = D.render_batch_flat(params, views)
+        ((flat - tgt_flat).abs() * w_flat).sum().backward()
 
     out = {}
     for name, fn in (("plain", plain_step), ("batched", batched_step)):

@@ -196,3 +196,33 @@ def test_pair_overflow_retries_and_matches_a_roomy_context(monkeypatch):
     for k in NAMES:
         assert np.isfinite(runs[0][1][k]).all(), k
         np.testing.assert_array_equal(runs[0][1][k], runs[1][1][k])
+
+
+def test_flat_batch_matches_list_batch():
+    """render_batch_flat: the same images as render_batch (bit-exact) and, for the same loss,
+    the same gradients; view_images gives each view's image."""
+    sa, cam_a = small_scene(2500, 41, 96, 64)
+    sb, _ = small_scene(1200, 42, 96, 64)
+    cam_b = S.orbit_camera(80, 48, 4.5, theta=0.9)
+    views = [(0, cam_a), (1, cam_b), (0, cam_b)]
+    pa, pb = D.scene_tensors(sa), D.scene_tensors(sb)
+    rng = np.random.default_rng(5)
+    ws = [torch.tensor(rng.uniform(0.5, 1.5, (c.screen_size[1], c.screen_size[0], 4)), dtype=torch.float32,
+                       device="cuda") for _, c in views]
+    outs = D.render_batch([pa, pb], views, t_min=0.0)
+    sum((o * w).sum() for o, w in zip(outs, ws)).backward()
+    ref = {k: (pa[k].grad.clone(), pb[k].grad.clone()) for k in NAMES}
+    for k in NAMES:
+        pa[k].grad = None
+        pb[k].grad = None
+    flat, spans = D.render_batch_flat([pa, pb], views, t_min=0.0)
+    assert flat.shape == (sum(h * w for _, h, w in spans), 4)
+    imgs = D.view_images(flat, spans)
+    for o, im in zip(outs, imgs):
+        np.testing.assert_array_equal(o.detach().cpu().numpy(), im.detach().cpu().numpy())
+    w_flat = torch.cat([w.reshape(-1, 4) for w in ws])
+    (flat * w_flat).sum().backward()
+    for k in NAMES:
+        for g, gr in zip((pa[k].grad, pb[k].grad), ref[k]):
+            rel = float(torch.linalg.norm(g - gr) / torch.linalg.norm(gr))
+            assert rel < 1e-5, (k, rel)

@@ -128,15 +128,33 @@ def _view_structs(views, outs, grads=None):
     return arr
 
 
+def _spans(views):
+    """(offset, height, width) of each view's pixels in a flat (P, 4) buffer."""
+    spans, off = [], 0
+    for _, cam, _ in views:
+        w, h = cam.screen_size[0], cam.screen_size[1]
+        spans.append((off, h, w))
+        off += h * w
+    return spans, off
+
+
 class _RenderBatch(torch.autograd.Function):
+    """meta = (views, n_scenes, cfg, flat): with flat the views' images are
+    slices of one (P, 4) tensor, which is the single output."""
+
     @staticmethod
     def forward(ctx, meta, *params):
-        views, n_scenes, cfg = meta
+        views, n_scenes, cfg, flat = meta
         groups = [tuple(_aligned(p.detach().contiguous().float()) for p in params[5 * i:5 * i + 5])
                   for i in range(n_scenes)]
         dev = groups[0][0].device
-        outs = [torch.empty((cam.screen_size[1], cam.screen_size[0], 4), dtype=torch.float32, device=dev)
-                for _, cam, _ in views]
+        if flat:
+            spans, total = _spans(views)
+            buf = torch.empty((total, 4), dtype=torch.float32, device=dev)
+            outs = [buf[o:o + h * w].view(h, w, 4) for o, h, w in spans]
+        else:
+            outs = [torch.empty((cam.screen_size[1], cam.screen_size[0], 4), dtype=torch.float32, device=dev)
+                    for _, cam, _ in views]
         c = _pool.get()
         sarr = _scene_structs(groups)
         varr = _view_structs(views, outs)
@@ -154,12 +172,18 @@ class _RenderBatch(torch.autograd.Function):
         ctx.meta = meta
         ctx.groups = groups
         ctx.save_for_backward(*params)
-        return tuple(outs)
+        return buf if flat else tuple(outs)
 
     @staticmethod
     def backward(ctx, *grad_outs):
-        views, n_scenes, cfg = ctx.meta
+        views, n_scenes, cfg, flat = ctx.meta
         groups = ctx.groups
+        if flat:
+            spans, total = _spans(views)
+            g = grad_outs[0]
+            g = (torch.zeros((total, 4), dtype=torch.float32, device=groups[0][0].device) if g is None
+                 else g.contiguous().float())
+            grad_outs = [g[o:o + h * w].view(h, w, 4) for o, h, w in spans]
         gouts = []
         for (scene, cam, _), g in zip(views, grad_outs):
             if g is None:
@@ -180,14 +204,34 @@ class _RenderBatch(torch.autograd.Function):
         return (None, *flat)
 
 
-def render_batch(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0):
-    """scenes: list of dicts with PARAM_NAMES tensors (cuda);
-    views: list of (scene_index, camera[, model]).  Returns a list of (H, W, 4) images."""
+def _batch_args(scenes, views):
     vs = [(v[0], v[1], v[2] if len(v) > 2 else None) for v in views]
     params = []
     for s in scenes:
         params += [s[k] for k in PARAM_NAMES]
-    return list(_RenderBatch.apply((vs, len(scenes), _config(tile, t_min)), *params))
+    return vs, params
+
+
+def render_batch(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0):
+    """scenes: list of dicts with PARAM_NAMES tensors (cuda);
+    views: list of (scene_index, camera[, model]).  Returns a list of (H, W, 4) images."""
+    vs, params = _batch_args(scenes, views)
+    return list(_RenderBatch.apply((vs, len(scenes), _config(tile, t_min), False), *params))
+
+
+def render_batch_flat(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0):
+    """Same as render_batch, with every view's pixels in one (P, 4) tensor: returns (flat, spans),
+    spans[i] = (offset, H, W) of view i, whose image is flat[offset:offset + H * W].view(H, W, 4)
+    (see view_images).  A loss over all views can then be formed with a few whole-batch tensor
+    operations instead of a few per view."""
+    vs, params = _batch_args(scenes, views)
+    flat = _RenderBatch.apply((vs, len(scenes), _config(tile, t_min), True), *params)
+    return flat, _spans(vs)[0]
+
+
+def view_images(flat, spans):
+    """The (H, W, 4) image of each view as a differentiable view of render_batch_flat's output."""
+    return [flat[o:o + h * w].view(h, w, 4) for o, h, w in spans]
 
 
 def render(means, log_scales, quats, opacity_logits, sh, camera, model=None, tile=(8, 8), t_min=1.0 / 8192.0):
