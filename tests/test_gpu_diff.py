@@ -226,3 +226,23 @@ def test_flat_batch_matches_list_batch():
         for g, gr in zip((pa[k].grad, pb[k].grad), ref[k]):
             rel = float(torch.linalg.norm(g - gr) / torch.linalg.norm(gr))
             assert rel < 1e-5, (k, rel)
+
+
+def test_unpacked_rect_path_matches_packed():
+    """At 2560x1600 a tile rectangle of the 8x8 grid (320 x 200 tiles) needs more than 32 bits, so
+    the forward gathers (rect, count, view) words in depth order; the 16x16 grid (160 x 100) packs
+    the rectangle into the depth sort's value.  Same image, gradients to float rounding."""
+    W, H = 2560, 1600
+    sc = S.gaussian_cloud(6000, seed=43, pos_std=1.0, logscale=(-2.5, 0.2))
+    cam = S.orbit_camera(W, H, 4.0)
+    ref_p, p = D.scene_tensors(sc), D.scene_tensors(sc)
+    w = torch.tensor(np.random.default_rng(6).uniform(0.5, 1.5, (H, W, 4)), dtype=torch.float32, device="cuda")
+    img_ref = D.render(**ref_p, camera=cam, tile=(16, 16), t_min=0.0)
+    (img_ref * w).sum().backward()
+    img = D.render(**p, camera=cam, tile=(8, 8), t_min=0.0)
+    (img * w).sum().backward()
+    np.testing.assert_array_equal(img.detach().cpu().numpy(), img_ref.detach().cpu().numpy())
+    for k in NAMES:
+        g, gr = p[k].grad, ref_p[k].grad
+        rel = float(torch.linalg.norm(g - gr) / torch.linalg.norm(gr))
+        assert rel < 1e-5, (k, rel)

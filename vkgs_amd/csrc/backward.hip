@@ -254,12 +254,17 @@ __global__ __launch_bounds__(kCompactBlock) void diff_count_kernel(DiffLaunch L,
 // Projection of the visible items into raster records at their slots (as the
 // plain fused path does), one lane per scene splat: the activated covariance,
 // SH and opacity are formed once and reused for every view of the scene.
+// The tile rectangle goes either to rc (with the count and the view, val =
+// slot) or, with kPacked, into the depth sort's 64-bit value slot | packed
+// rectangle << 32.
+template <bool kPacked>
 __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L, const uint32_t* __restrict__ offsets,
                                                                    const uint64_t* __restrict__ masks,
                                                                    float4* __restrict__ rec, uint4* __restrict__ rc,
                                                                    uint32_t* __restrict__ scnt,
                                                                    uint32_t* __restrict__ key,
-                                                                   uint32_t* __restrict__ val) {
+                                                                   uint32_t* __restrict__ val,
+                                                                   uint64_t* __restrict__ val64, RectBits rb) {
   const uint32_t sb = blockIdx.x;
   const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
   const uint32_t lb = sb - sc.block_base;
@@ -287,12 +292,15 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
     const float f[12] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w, o[2].x, o[2].y, o[2].z, o[2].w};
     Splat2D sp = {};
     uint4 r = make_uint4(0u, 0u, 0u, v);
+    uint32_t packed = 0;
+    if (kPacked) packed = pack_rect(1u, 0u, 0u, 0u, rb);  // empty
     if (setup_splat(f, view.fc.width, view.fc.height, &sp)) {
       const uint32_t tx0 = (uint32_t)sp.x0 >> L.tile_shift_x, tx1 = (uint32_t)sp.x1 >> L.tile_shift_x;
       const uint32_t ty0 = (uint32_t)sp.y0 >> L.tile_shift_y, ty1 = (uint32_t)sp.y1 >> L.tile_shift_y;
       r.x = tx0 | (tx1 << 16);
       r.y = ty0 | (ty1 << 16);
       r.z = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+      if (kPacked) packed = pack_rect(tx0, tx1, ty0, ty1, rb);
     } else {  // empty pixel box: every raster tile test rejects it
       sp.x0 = 1;
       sp.x1 = 0;
@@ -302,11 +310,26 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
     rec[3 * (size_t)slot + 0] = make_float4(sp.cx, sp.cy, sp.i00, sp.i01);
     rec[3 * (size_t)slot + 1] = make_float4(sp.i10, sp.i11, sp.o, __uint_as_float(pack16(sp.x0, sp.x1)));
     rec[3 * (size_t)slot + 2] = make_float4(sp.r, sp.g, sp.b, __uint_as_float(pack16(sp.y0, sp.y1)));
-    rc[slot] = r;
+    if (kPacked) {
+      val64[slot] = (uint64_t)slot | ((uint64_t)packed << 32);
+    } else {
+      rc[slot] = r;
+      val[slot] = slot;
+    }
     scnt[slot] = r.z;
     key[slot] = __float_as_uint(omz);
-    val[slot] = slot;
   }
+}
+
+// First slot of every view (slots are view-major): vslot[v] = the scanned
+// count of the view's first compaction wave; vslot[n_views] = all visible.
+__global__ void diff_view_slots_kernel(DiffLaunch L, const uint32_t* __restrict__ offsets,
+                                       const uint32_t* __restrict__ d_count, uint32_t* __restrict__ vslot) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < L.n_views)
+    vslot[v] = L.view_blocks[v] < L.n_blocks ? offsets[(size_t)L.view_blocks[v] * kMaskWords] : *d_count;
+  else if (v == L.n_views)
+    vslot[v] = *d_count;
 }
 
 __global__ __launch_bounds__(256) void diff_bin_gather_kernel(const uint4* __restrict__ rc,
@@ -388,6 +411,44 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
         slot = order[k];
         q0 = soffs[slot];
         const DiffView& view = L.views[kview[k]];
+        key_base = view.tile_base;
+        tiles_x = view.fc.tiles_x;
+        if (o + c > max_pairs) atomicOr(status, 1u);
+      }
+    }
+    emit_pairs_wave<true>(c, o, r, slot, key_base, tiles_x, max_pairs, keys, vals, q0);
+  }
+}
+
+// Same from the depth-sorted values slot | packed rect << 32: the view of a
+// slot from the view-major slot bases vslot, the pair offsets from the scan
+// of the packed rectangles' tile counts.
+__global__ __launch_bounds__(256) void diff_bin_emit_packed_kernel(DiffLaunch L, const uint64_t* __restrict__ val64,
+                                                                   const uint32_t* __restrict__ offs,
+                                                                   const uint32_t* __restrict__ vslot, RectBits rb,
+                                                                   const uint32_t* __restrict__ d_count,
+                                                                   uint32_t max_n, uint64_t max_pairs,
+                                                                   uint32_t* __restrict__ keys,
+                                                                   uint32_t* __restrict__ vals,
+                                                                   const uint32_t* __restrict__ soffs,
+                                                                   uint32_t* __restrict__ status) {
+  const uint32_t n = min(*d_count, max_n);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // wave-uniform loop: emit_pairs_wave needs every lane
+  for (uint32_t base = wave * kWave; base < n; base += stride) {
+    const uint32_t k = base + __lane_id();
+    uint32_t c = 0, slot = 0, key_base = 0, tiles_x = 0;
+    uint64_t o = 0, q0 = 0;
+    TileRect r = {0u, 0u};
+    if (k < n) {
+      const uint64_t w = val64[k];
+      r = unpack_rect((uint32_t)(w >> 32), rb, &c);
+      if (c != 0) {
+        slot = (uint32_t)w;
+        o = offs[k];
+        q0 = soffs[slot];
+        const DiffView& view = L.views[range_of(vslot, L.n_views, slot)];
         key_base = view.tile_base;
         tiles_x = view.fc.tiles_x;
         if (o + c > max_pairs) atomicOr(status, 1u);
@@ -1132,7 +1193,35 @@ size_t diff_count_scan_bytes(uint32_t n_blocks) { return scan_plan(n_blocks * kM
 hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks, float4* rec,
                              uint4* rc, uint32_t* scnt, uint32_t* key, uint32_t* val, hipStream_t s) {
   if (L.n_blocks == 0) return hipSuccess;
-  diff_write_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, rec, rc, scnt, key, val);
+  diff_write_kernel<false><<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, rec, rc, scnt, key, val,
+                                                                      nullptr, RectBits{0u, 0u});
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_write_packed(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
+                                    float4* rec, uint32_t* scnt, uint32_t* key, uint64_t* val64, const RectBits& rb,
+                                    hipStream_t s) {
+  if (L.n_blocks == 0) return hipSuccess;
+  diff_write_kernel<true><<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, rec, nullptr, scnt, key,
+                                                                     nullptr, val64, rb);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_view_slots(const DiffLaunch& L, const uint32_t* offsets, const uint32_t* d_count,
+                                  uint32_t* vslot, hipStream_t s) {
+  diff_view_slots_kernel<<<ceil_div(L.n_views + 1, 256), 256, 0, s>>>(L, offsets, d_count, vslot);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_bin_emit_packed(const DiffLaunch& L, const uint64_t* val64, const uint32_t* offs,
+                                       const uint32_t* vslot, const RectBits& rb, const uint32_t* d_count,
+                                       uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals,
+                                       uint32_t* pslot, const uint32_t* soffs, const uint32_t* scnt, uint32_t* status,
+                                       hipStream_t s) {
+  if (max_n == 0) return hipSuccess;
+  diff_slot_fill_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(soffs, scnt, d_count, max_n, max_pairs, pslot);
+  diff_bin_emit_packed_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(L, val64, offs, vslot, rb, d_count, max_n,
+                                                                         max_pairs, keys, vals, soffs, status);
   return hipGetLastError();
 }
 

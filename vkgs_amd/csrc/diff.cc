@@ -52,8 +52,8 @@ struct vkgs_diff_ctx {
   Buf tables;
   // per-item buffers
   uint32_t cap_items = 0;
-  Buf counts, count_scan, rec, rc, key, val, masks, kview, g9, misc, rect, cnt, offs, scnt, soffs, scan_storage,
-      depth_sort;
+  Buf counts, count_scan, rec, rc, key, val, val64, vslot, masks, kview, g9, misc, rect, cnt, offs, scnt, soffs,
+      scan_storage, depth_sort;
   // pairs
   uint64_t max_pairs = 0;
   Buf pkeys, pvals, pslot, gpart, pair_sort, ranges;
@@ -132,7 +132,8 @@ int vkgs_diff_destroy(vkgs_diff_ctx* c) {
                  &c->val,        &c->masks,    &c->kview,        &c->g9,         &c->misc,       &c->rect,
                  &c->cnt,        &c->offs,     &c->scnt,         &c->soffs,      &c->scan_storage, &c->depth_sort,
                  &c->pkeys,      &c->pvals,    &c->pair_sort,    &c->ranges,     &c->final_t,    &c->last,
-                 &c->tile_keys,  &c->tile_order, &c->tile_sort,  &c->pslot,      &c->gpart};
+                 &c->tile_keys,  &c->tile_order, &c->tile_sort,  &c->pslot,      &c->gpart,
+                 &c->val64,      &c->vslot};
   for (Buf* b : bufs) b->release();
   delete c;
   return VKGS_OK;
@@ -179,6 +180,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   c->h_view_blocks.assign(n_views + 1, 0u);
   c->h_view_tiles.assign(n_views + 1, 0u);
   uint64_t blocks = 0, tiles = 0, pixels = 0;
+  uint32_t max_tiles_x = 1, max_tiles_y = 1;
   for (uint32_t v = 0; v < n_views; ++v) {
     const vkgs_diff_view& a = views[v];
     if (a.scene >= n_scenes || !a.rgba) return VKGS_ERR_INVALID_ARG;
@@ -200,6 +202,8 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
     blocks += ceil_div(c->h_scenes[a.scene].n, kCompactBlock);
     tiles += (uint64_t)rs.tiles_x * rs.tiles_y;
     pixels += (uint64_t)rs.width * rs.height;
+    max_tiles_x = rs.tiles_x > max_tiles_x ? rs.tiles_x : max_tiles_x;
+    max_tiles_y = rs.tiles_y > max_tiles_y ? rs.tiles_y : max_tiles_y;
     if (blocks * kCompactBlock > (1ull << 30) || tiles > (1u << 24)) return VKGS_ERR_CAPACITY;
   }
   c->h_view_blocks[n_views] = (uint32_t)blocks;
@@ -223,7 +227,11 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
     if (c->max_pairs > 0xffffffffull) c->max_pairs = 0xffffffffull;
     c->grow_pairs = false;
   }
-  c->depth_plan = sort_plan(cap);
+  // tile rectangles packed into the depth sort's value when every view's
+  // tile grid fits (else (rect, count, view) words gathered in depth order)
+  RectBits rb;
+  const bool packed = rect_bits(max_tiles_x, max_tiles_y, &rb);
+  c->depth_plan = sort_plan(cap, packed ? 8u : 4u);
   c->pair_plan = sort_plan((uint32_t)c->max_pairs);
   c->scan = scan_plan(cap);
   c->tile_plan = sort_plan((uint32_t)(tiles ? tiles : 1));
@@ -232,15 +240,20 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   ENSURE(counts, (size_t)(blocks ? blocks : 1) * kMaskWords * 4);
   ENSURE(count_scan, diff_count_scan_bytes((uint32_t)(blocks ? blocks : 1)));
   ENSURE(rec, (size_t)cap * 48);
-  ENSURE(rc, (size_t)cap * 16);
+  if (packed) {
+    ENSURE(val64, (size_t)cap * 8);
+    ENSURE(vslot, ((size_t)n_views + 1) * 4);
+  } else {
+    ENSURE(rc, (size_t)cap * 16);
+    ENSURE(val, (size_t)cap * 4);
+    ENSURE(kview, (size_t)cap * 4);
+    ENSURE(rect, (size_t)cap * sizeof(TileRect));
+    ENSURE(cnt, (size_t)cap * 4);
+  }
   ENSURE(key, (size_t)cap * 4);
-  ENSURE(val, (size_t)cap * 4);
   ENSURE(masks, (size_t)(blocks ? blocks : 1) * kMaskWords * 8);
-  ENSURE(kview, (size_t)cap * 4);
   ENSURE(g9, (size_t)cap * 36);
   ENSURE(misc, 256);
-  ENSURE(rect, (size_t)cap * sizeof(TileRect));
-  ENSURE(cnt, (size_t)cap * 4);
   ENSURE(offs, (size_t)cap * 4);
   ENSURE(scnt, (size_t)cap * 4);
   ENSURE(soffs, (size_t)cap * 4);
@@ -286,17 +299,31 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   VKGS_HIP_CHECK(fill_u32_launch(misc, 0u, 4, s));
   VKGS_HIP_CHECK(launch_diff_count(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), misc + kMiscVisible,
                                    c->count_scan.p, s));
-  VKGS_HIP_CHECK(launch_diff_write(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), c->rec.as<float4>(),
-                                   c->rc.as<uint4>(), c->scnt.as<uint32_t>(), c->key.as<uint32_t>(), order, s));
+  if (packed)
+    VKGS_HIP_CHECK(launch_diff_write_packed(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(),
+                                            c->rec.as<float4>(), c->scnt.as<uint32_t>(), c->key.as<uint32_t>(),
+                                            c->val64.as<uint64_t>(), rb, s));
+  else
+    VKGS_HIP_CHECK(launch_diff_write(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), c->rec.as<float4>(),
+                                     c->rc.as<uint4>(), c->scnt.as<uint32_t>(), c->key.as<uint32_t>(), order, s));
   // slot-order numbering of the pairs (their place in the backward's partials)
   VKGS_HIP_CHECK(scan_exclusive_launch(c->scan, c->scnt.as<uint32_t>(), c->soffs.as<uint32_t>(), misc + kMiscVisible,
                                        misc + kMiscSlotPairs, c->scan_storage.p, s));
-  VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, 30u,
-                                c->depth_sort.p, s));
-  VKGS_HIP_CHECK(launch_diff_bin_gather(c->rc.as<uint4>(), order, misc + kMiscVisible, cap, c->rect.as<TileRect>(),
-                                        c->cnt.as<uint32_t>(), c->kview.as<uint32_t>(), s));
-  VKGS_HIP_CHECK(scan_exclusive_launch(c->scan, c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(), misc + kMiscVisible,
-                                       misc + kMiscPairs, c->scan_storage.p, s));
+  if (packed) {
+    VKGS_HIP_CHECK(sort_kv64_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(),
+                                    c->val64.as<uint64_t>(), 30u, c->depth_sort.p, s));
+    VKGS_HIP_CHECK(launch_diff_view_slots(L, c->counts.as<uint32_t>(), misc + kMiscVisible, c->vslot.as<uint32_t>(),
+                                          s));
+    VKGS_HIP_CHECK(scan_rect_counts_launch(c->scan, c->val64.as<uint64_t>(), rb, c->offs.as<uint32_t>(),
+                                           misc + kMiscVisible, misc + kMiscPairs, c->scan_storage.p, s));
+  } else {
+    VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, 30u,
+                                  c->depth_sort.p, s));
+    VKGS_HIP_CHECK(launch_diff_bin_gather(c->rc.as<uint4>(), order, misc + kMiscVisible, cap,
+                                          c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->kview.as<uint32_t>(), s));
+    VKGS_HIP_CHECK(scan_exclusive_launch(c->scan, c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),
+                                         misc + kMiscVisible, misc + kMiscPairs, c->scan_storage.p, s));
+  }
   // with an odd number of tile-sort passes the pairs are emitted into the
   // sort's alternate buffers, so the sorted result lands in pkeys / pvals
   // without a final copy
@@ -304,10 +331,16 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   const bool odd = sort_passes(tile_bits) % 2 == 1;
   uint32_t *ek = c->pkeys.as<uint32_t>(), *ev = c->pvals.as<uint32_t>();
   if (odd) sort_alt_buffers(c->pair_plan, c->pair_sort.p, &ek, &ev);
-  VKGS_HIP_CHECK(launch_diff_bin_emit(L, c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),
-                                      c->kview.as<uint32_t>(), order, misc + kMiscVisible, cap, c->max_pairs, ek, ev,
-                                      c->pslot.as<uint32_t>(), c->soffs.as<uint32_t>(), c->scnt.as<uint32_t>(),
-                                      misc + kMiscStatus, s));
+  if (packed)
+    VKGS_HIP_CHECK(launch_diff_bin_emit_packed(L, c->val64.as<uint64_t>(), c->offs.as<uint32_t>(),
+                                               c->vslot.as<uint32_t>(), rb, misc + kMiscVisible, cap, c->max_pairs,
+                                               ek, ev, c->pslot.as<uint32_t>(), c->soffs.as<uint32_t>(),
+                                               c->scnt.as<uint32_t>(), misc + kMiscStatus, s));
+  else
+    VKGS_HIP_CHECK(launch_diff_bin_emit(L, c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),
+                                        c->kview.as<uint32_t>(), order, misc + kMiscVisible, cap, c->max_pairs, ek,
+                                        ev, c->pslot.as<uint32_t>(), c->soffs.as<uint32_t>(),
+                                        c->scnt.as<uint32_t>(), misc + kMiscStatus, s));
   if (odd)
     VKGS_HIP_CHECK(sort_kv_from_alt_launch(c->pair_plan, misc + kMiscPairs, c->pkeys.as<uint32_t>(),
                                            c->pvals.as<uint32_t>(), tile_bits, c->pair_sort.p, s));

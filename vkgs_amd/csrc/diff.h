@@ -82,6 +82,22 @@ size_t diff_count_scan_bytes(uint32_t n_blocks);
 // val = slot.
 hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks, float4* rec,
                              uint4* rc, uint32_t* scnt, uint32_t* key, uint32_t* val, hipStream_t s);
+// Same with the tile rectangle packed into the depth sort's value:
+// val64[slot] = slot | packed rect << 32 (rb from the largest view's tile
+// grid), no rc or val.
+hipError_t launch_diff_write_packed(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
+                                    float4* rec, uint32_t* scnt, uint32_t* key, uint64_t* val64, const RectBits& rb,
+                                    hipStream_t s);
+// vslot[v] = first slot of view v (n_views + 1 words, the last = *d_count).
+hipError_t launch_diff_view_slots(const DiffLaunch& L, const uint32_t* offsets, const uint32_t* d_count,
+                                  uint32_t* vslot, hipStream_t s);
+// Pairs as launch_diff_bin_emit from the depth-sorted packed values, offs =
+// exclusive scan of their tile counts.
+hipError_t launch_diff_bin_emit_packed(const DiffLaunch& L, const uint64_t* val64, const uint32_t* offs,
+                                       const uint32_t* vslot, const RectBits& rb, const uint32_t* d_count,
+                                       uint32_t max_n, uint64_t max_pairs, uint32_t* keys, uint32_t* vals,
+                                       uint32_t* pslot, const uint32_t* soffs, const uint32_t* scnt, uint32_t* status,
+                                       hipStream_t s);
 // rect[k], cnt[k], kview[k] of slot order[k].
 hipError_t launch_diff_bin_gather(const uint4* rc, const uint32_t* order, const uint32_t* d_count, uint32_t max_n,
                                   TileRect* rect, uint32_t* cnt, uint32_t* kview, hipStream_t s);
