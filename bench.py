@@ -111,12 +111,19 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
                                    1.0 / (4 * c.screen_size[0] * c.screen_size[1]), device="cuda")
                         for _, c in views])
 
+    def zero_grad():  # as an optimizer step's zero_grad(set_to_none=True) would
+        for p in params:
+            for t in p.values():
+                t.grad = None
+
     def plain_step():
+        zero_grad()
         for (si, c), tgt in zip(views, targets):
             img = D.render(**params[si], camera=c)
             (img - tgt).abs().mean().backward()
 
     def batched_step():
+        zero_grad()
         flat, _ = D.render_batch_flat(params, views)
         ((flat - tgt_flat).abs() * w_flat).sum().backward()
 
