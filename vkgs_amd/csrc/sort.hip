@@ -404,8 +404,10 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
   V* src_v = input_in_alt ? alt_v : vals;
   uint32_t* dst_k = input_in_alt ? keys : alt_k;
   V* dst_v = input_in_alt ? vals : alt_v;
-  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
-    const uint32_t bits = key_bits - shift < 8 ? key_bits - shift : 8;
+  // the narrow digit (when key_bits is not a multiple of 8) goes first
+  const uint32_t first = key_bits - 8 * ((key_bits + 7) / 8 - 1);
+  for (uint32_t shift = 0; shift < key_bits;) {
+    const uint32_t bits = shift == 0 ? first : 8;
     sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
     sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total);
     switch (bits) {
@@ -424,6 +426,7 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
       VKGS_DOWNSWEEP(8)
 #undef VKGS_DOWNSWEEP
     }
+    shift += bits;
     uint32_t* tk = src_k;
     V* tv = src_v;
     src_k = dst_k;
