@@ -320,27 +320,46 @@ __global__ __launch_bounds__(kThreads) void scan_reduce(Load in, const uint32_t*
 template <typename Load>
 __global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, const uint32_t* __restrict__ d_count,
                                                        uint32_t max_n, const uint32_t* __restrict__ partial) {
+  // the sub-tile passes through LDS so that global loads and stores are
+  // lane-strided (coalesced) while each thread scans kIpt consecutive words;
+  // index i is stored at i + i / 32, which spreads a thread's run over banks
   __shared__ uint32_t s_tmp[kWaves];
+  __shared__ uint32_t s_x[kTile + kTile / 32];
+  const uint32_t tid = threadIdx.x;
   const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
   uint32_t carry = partial[blockIdx.x];
   for (uint32_t base = begin; base < end; base += kTile) {
-    const uint32_t first = base + threadIdx.x * kIpt;
+#pragma unroll
+    for (int r = 0; r < kIpt; ++r) {
+      const uint32_t i = r * kThreads + tid;
+      s_x[i + (i >> 5)] = base + i < end ? in(base + i) : 0u;
+    }
+    __syncthreads();
     uint32_t x[kIpt];
     uint32_t s = 0;
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
-      x[j] = first + j < end ? in(first + j) : 0u;
+      const uint32_t i = tid * kIpt + j;
+      x[j] = s_x[i + (i >> 5)];
       s += x[j];
     }
     uint32_t all;
     uint32_t run = carry + block_exclusive_scan<kThreads>(s, s_tmp, &all);
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
-      if (first + j < end) out[first + j] = run;
+      const uint32_t i = tid * kIpt + j;
+      s_x[i + (i >> 5)] = run;
       run += x[j];
     }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kIpt; ++r) {
+      const uint32_t i = r * kThreads + tid;
+      if (base + i < end) out[base + i] = s_x[i + (i >> 5)];
+    }
+    __syncthreads();
     carry += all;
   }
 }
