@@ -27,7 +27,10 @@ if HERE not in sys.path:
 METRIC = "FPS @6M splats 1600x900; batched vs plain fwd+bwd views/s; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-RASTER_FLOPS_PER_EVAL = 20  # per (pixel, splat) evaluation, exp counted as 1
+# per (pixel, splat) evaluation, an FMA counted as 2 flops (as the peak counts it) and exp2 as 1:
+# dx, dy 2; the two rows of the inverse rot_scale 3 + 3; q 3; falloff 2; alpha 1; T * alpha 1;
+# colour and alpha FMAs 4 x 2; T update 1
+RASTER_FLOPS_PER_EVAL = 24
 
 
 def parse_args(argv=None):
