@@ -114,7 +114,7 @@ struct DrawBuffers {
   void* sort_storage;
   uint32_t* ranges;  // [start, end) per tile
   uint32_t* status;  // overflow flag
-  bool status_cleared;  // the caller zeroed *status for this draw
+  bool status_cleared = false;  // the caller zeroed *status for this draw
   uint32_t* pairs;   // pair count
   // longest-first tile schedule (optional: tile_order == nullptr disables it)
   uint32_t* tile_keys;
