@@ -216,6 +216,17 @@ def main(argv=None) -> int:
     ms = (t1 - t0) * 1e3 / args.steps
     fps = 1e3 / ms
     log(f"{ms:.3f} ms/frame, {fps:.1f} FPS")
+    # per-frame spread: HIP events around each replay on the renderer's stream
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in evs:
+        a.record()
+        r.render(cam, out)
+        b.record()
+    torch.cuda.synchronize()
+    fr = sorted(a.elapsed_time(b) for a, b in evs)
+    frame_ms = {"median": round(fr[len(fr) // 2], 4), "p10": round(fr[len(fr) // 10], 4),
+                "p90": round(fr[(9 * len(fr)) // 10], 4)}
+    log("frame ms", frame_ms)
 
     # ---- per-stage split with HIP events on the renderer's stream ----
     r.set_timing(True)
@@ -301,6 +312,7 @@ def main(argv=None) -> int:
         "config": {"workload": f"plain forward {wl.name}: {wl.n} splats, {wl.width}x{wl.height}, orbit r={wl.radius} m",
                    "splats": n, "width": wl.width, "height": wl.height, "visible": V, "pairs": K,
                    "tile": f"{tw}x{th}", "t_min": args.t_min, "graph": True},
+        "frame_ms": frame_ms,
         "stages_ms": stage_ms,
         "roofline": roof,
         "roofline_projection": roof_proj,
