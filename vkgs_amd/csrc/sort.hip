@@ -33,6 +33,7 @@ constexpr int kTile = kThreads * kIpt;  // keys per sub-tile
 #define VKGS_SORT_MAX_BLOCKS 1024
 #endif
 constexpr int kMaxBlocks = VKGS_SORT_MAX_BLOCKS;
+constexpr uint32_t kRawHistBlocks = 16;
 constexpr int kScanThreads = 1024;
 
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
@@ -167,7 +168,8 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
                                                            uint32_t* __restrict__ kout, V* __restrict__ vout,
                                                            const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                            uint32_t shift, const uint32_t* __restrict__ hist,
-                                                           const uint32_t* __restrict__ digit_total) {
+                                                           const uint32_t* __restrict__ digit_total,
+                                                           uint32_t raw_hist) {
   constexpr uint32_t bits = kBits;
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_wcnt[kWaves][256];
@@ -183,10 +185,24 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
   const uint32_t mask = (1u << bits) - 1u;
   {
     // global offset of digit tid for this block: keys of smaller digits
-    // everywhere plus keys of this digit in earlier blocks
+    // everywhere plus keys of this digit in earlier blocks.  raw_hist: the
+    // row scan was skipped (small grids), so each block sums the raw counts.
+    uint32_t total = 0, before = 0;
+    if (tid <= mask) {
+      if (raw_hist) {
+        for (uint32_t b = 0; b < gridDim.x; ++b) {
+          const uint32_t h = hist[tid * gridDim.x + b];
+          total += h;
+          before += b < blockIdx.x ? h : 0u;
+        }
+      } else {
+        total = digit_total[tid];
+        before = hist[tid * gridDim.x + blockIdx.x];
+      }
+    }
     uint32_t all;
-    const uint32_t smaller = block_exclusive_scan<kThreads>(tid <= mask ? digit_total[tid] : 0u, s_tmp, &all);
-    s_base[tid] = tid <= mask ? smaller + hist[tid * gridDim.x + blockIdx.x] : 0u;
+    const uint32_t smaller = block_exclusive_scan<kThreads>(total, s_tmp, &all);
+    s_base[tid] = tid <= mask ? smaller + before : 0u;
   }
 #pragma unroll
   for (int i = 0; i < kWaves; ++i) s_wcnt[i][tid] = 0u;
@@ -423,17 +439,21 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
   V* src_v = input_in_alt ? alt_v : vals;
   uint32_t* dst_k = input_in_alt ? keys : alt_k;
   V* dst_v = input_in_alt ? vals : alt_v;
+  // small grids: each downsweep block sums the raw block counts itself,
+  // which is cheaper than a row-scan launch
+  const bool raw = plan.blocks <= kRawHistBlocks;
   // the narrow digit (when key_bits is not a multiple of 8) goes first
   const uint32_t first = key_bits - 8 * ((key_bits + 7) / 8 - 1);
   for (uint32_t shift = 0; shift < key_bits;) {
     const uint32_t bits = shift == 0 ? first : 8;
     sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
-    sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total);
+    if (!raw) sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total);
     switch (bits) {
 #define VKGS_DOWNSWEEP(B)                                                                          \
   case B:                                                                                          \
     sort_downsweep<B, V><<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, \
-                                                               plan.max_n, shift, hist, digit_total); \
+                                                               plan.max_n, shift, hist, digit_total, \
+                                                               raw ? 1u : 0u);                     \
     break;
       VKGS_DOWNSWEEP(1)
       VKGS_DOWNSWEEP(2)
