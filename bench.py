@@ -100,7 +100,8 @@ def batch_workload(scale: float = 1.0, seed: int = 5):
 def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
     """views/s of fwd+bwd (L1 loss, the mean over each view summed over views) for the batch
     workload: one launch set and one loss per view (plain), and one launch set for all views
-    with the loss formed over their flat (P, 4) output (batched)."""
+    with the loss formed over their flat (P, 4) output and the 32-view scene processed in
+    spatial order (batched)."""
     import torch
     from vkgs_amd import diff as D
 
@@ -127,7 +128,7 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
 
     def batched_step():
         zero_grad()
-        flat, _ = D.render_batch_flat(params, views)
+        flat, _ = D.render_batch_flat(params, views, spatial_order=True)
         ((flat - tgt_flat).abs() * w_flat).sum().backward()
 
     out = {}

@@ -161,8 +161,19 @@ typedef struct vkgs_raster_config {
   float t_min;             /* stop a pixel once transmittance < t_min (0 = composite every splat) */
   uint32_t variant;        /* 0: the workgroup stages batches of records in LDS; 1: each wave
                               fetches records with scalar loads.  Both give the same image. */
-  uint32_t reserved[4];
+  uint32_t flags;          /* VKGS_DIFF_SPATIAL_ORDER (differentiable path only), or 0 */
+  uint32_t reserved[3];
 } vkgs_raster_config;
+
+/* vkgs_diff_forward: process the splats of every scene that has four or more
+ * views in the call in a spatial (Morton) order of their means, so that a
+ * wave's splats mostly share their visibility.  Faster for many views per
+ * scene; images and gradients then differ from the default order only in the
+ * blend order of splats with equal depth keys (the default order, ascending
+ * splat id, is the one single-view calls and the plain renderer use).  The
+ * order is cached per context while the scenes' means pointers and sizes stay
+ * the same and refreshed every 32 forwards. */
+#define VKGS_DIFF_SPATIAL_ORDER 1u
 
 /* default: 8x8 tiles, t_min = 1/8192 */
 void vkgs_raster_config_default(vkgs_raster_config* cfg);

@@ -63,7 +63,8 @@ class RasterConfig(C.Structure):
         ("tile_h", C.c_uint32),
         ("t_min", C.c_float),
         ("variant", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("flags", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 

@@ -218,6 +218,97 @@ __device__ __forceinline__ uint32_t range_of(const uint32_t* __restrict__ bases,
   return lo;
 }
 
+// ------------------------------------------------------ processing order ---
+
+// Floats as unsigned words that order like the floats (for min / max).
+__device__ __forceinline__ uint32_t float_order(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float order_float(uint32_t o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// bbox_part[6 sb ..]: min x, y, z and max x, y, z (order words) of scene block sb.
+__global__ __launch_bounds__(kCompactBlock) void diff_bbox_kernel(DiffLaunch L, uint32_t* __restrict__ bbox_part) {
+  __shared__ uint32_t s_b[6][kCompactBlock / kWave];
+  const uint32_t sb = blockIdx.x;
+  const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
+  const uint32_t li = (sb - sc.block_base) * kCompactBlock + threadIdx.x;
+  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  if (li < sc.n)
+    for (int a = 0; a < 3; ++a) v[a] = v[3 + a] = float_order(sc.means[3 * (size_t)li + a]);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      v[a] = min(v[a], (uint32_t)__shfl_xor((int)v[a], d, kWave));
+      v[3 + a] = max(v[3 + a], (uint32_t)__shfl_xor((int)v[3 + a], d, kWave));
+    }
+  if (__lane_id() == 0)
+    for (int a = 0; a < 6; ++a) s_b[a][threadIdx.x / kWave] = v[a];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const uint32_t a = threadIdx.x;
+    uint32_t r = s_b[a][0];
+    for (uint32_t i = 1; i < kCompactBlock / kWave; ++i) r = a < 3 ? min(r, s_b[a][i]) : max(r, s_b[a][i]);
+    bbox_part[6 * (size_t)sb + a] = r;
+  }
+}
+
+// bbox[6 s ..] from the scene's block partials; one workgroup per scene.
+__global__ __launch_bounds__(kCompactBlock) void diff_bbox_reduce_kernel(DiffLaunch L,
+                                                                         const uint32_t* __restrict__ bbox_part,
+                                                                         uint32_t* __restrict__ bbox) {
+  __shared__ uint32_t s_r[6][kCompactBlock];
+  const uint32_t s = blockIdx.x;
+  const uint32_t b0 = L.scene_blocks[s], b1 = L.scene_blocks[s + 1];
+  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  for (uint32_t b = b0 + threadIdx.x; b < b1; b += kCompactBlock)
+    for (int a = 0; a < 6; ++a) v[a] = a < 3 ? min(v[a], bbox_part[6 * (size_t)b + a]) : max(v[a], bbox_part[6 * (size_t)b + a]);
+  for (int a = 0; a < 6; ++a) s_r[a][threadIdx.x] = v[a];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const uint32_t a = threadIdx.x;
+    uint32_t r = s_r[a][0];
+    for (uint32_t i = 1; i < kCompactBlock; ++i) r = a < 3 ? min(r, s_r[a][i]) : max(r, s_r[a][i]);
+    bbox[6 * s + a] = r;
+  }
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {  // bit i of x to bit 3 i (x < 1024)
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+__global__ __launch_bounds__(kCompactBlock) void diff_morton_kernel(DiffLaunch L, const uint32_t* __restrict__ bbox,
+                                                                    uint32_t scene_shift, uint32_t axis_bits,
+                                                                    uint32_t* __restrict__ keys,
+                                                                    uint32_t* __restrict__ vals) {
+  const uint32_t sb = blockIdx.x;
+  const uint32_t si = range_of(L.scene_blocks, L.n_scenes, sb);
+  const DiffScene& sc = L.scenes[si];
+  const uint32_t li = (sb - sc.block_base) * kCompactBlock + threadIdx.x;
+  const size_t g = (size_t)sb * kCompactBlock + threadIdx.x;
+  uint32_t key = 0xffffffffu;
+  if (li < sc.n) {
+    const float top = (float)((1u << axis_bits) - 1u);
+    uint32_t code = 0;
+    for (int a = 0; a < 3; ++a) {
+      const float lo = order_float(bbox[6 * si + a]), hi = order_float(bbox[6 * si + 3 + a]);
+      const float t = hi > lo ? (sc.means[3 * (size_t)li + a] - lo) / (hi - lo) : 0.f;
+      const float qf = fminf(fmaxf(t * top, 0.f), top);  // NaN positions go to 0
+      code |= spread3((uint32_t)qf) << a;
+    }
+    key = (si << scene_shift) | code;
+  }
+  keys[g] = key;
+  vals[g] = li;
+}
+
 // --------------------------------------------------------------- forward ---
 
 // Visibility of every (view, splat) item, one lane per scene splat visiting
@@ -229,9 +320,10 @@ __global__ __launch_bounds__(kCompactBlock) void diff_count_kernel(DiffLaunch L,
   const uint32_t sb = blockIdx.x;
   const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
   const uint32_t lb = sb - sc.block_base;
-  const uint32_t id = lb * kCompactBlock + threadIdx.x;
+  const uint32_t li = lb * kCompactBlock + threadIdx.x;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const bool valid = id < sc.n;
+  const bool valid = li < sc.n;
+  const uint32_t id = valid && sc.perm ? sc.perm[li] : li;
   float x = 0.f, y = 0.f, z = 0.f;
   if (valid) {
     x = sc.means[3 * (size_t)id];
@@ -268,7 +360,7 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
   const uint32_t sb = blockIdx.x;
   const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
   const uint32_t lb = sb - sc.block_base;
-  const uint32_t id = lb * kCompactBlock + threadIdx.x;
+  const uint32_t li = lb * kCompactBlock + threadIdx.x;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint32_t lane = __lane_id();
   // a wave with no visible item in any view skips the parameter loads
@@ -277,7 +369,7 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
     any = masks[(size_t)(L.views[L.scene_views[vi]].block_base + lb) * kMaskWords + w] != 0;
   if (!any) return;
   SplatInputs in;
-  if (id < sc.n) load_inputs(sc, id, in);
+  if (li < sc.n) load_inputs(sc, sc.perm ? sc.perm[li] : li, in);
   for (uint32_t vi = sc.view_begin; vi < sc.view_end; ++vi) {
     const uint32_t v = L.scene_views[vi];
     const DiffView& view = L.views[v];
@@ -1070,10 +1162,11 @@ __global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(Di
   const uint32_t sb = blockIdx.x;
   const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
   const uint32_t lb = sb - sc.block_base;
-  const uint32_t id = lb * kCompactBlock + threadIdx.x;
+  const uint32_t li = lb * kCompactBlock + threadIdx.x;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint32_t lane = __lane_id();
-  const bool valid = id < sc.n;
+  const bool valid = li < sc.n;
+  const uint32_t id = valid && sc.perm ? sc.perm[li] : li;
 
   SplatState P;
   float op = 0.5f;
@@ -1179,6 +1272,19 @@ uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
 }
 
 }  // namespace
+
+hipError_t launch_diff_morton_keys(const DiffLaunch& L, uint32_t* bbox_part, uint32_t* bbox, uint32_t* keys,
+                                   uint32_t* vals, hipStream_t s) {
+  if (L.n_scene_blocks == 0) return hipSuccess;
+  uint32_t sbits = 0;
+  while ((1u << sbits) < L.n_scenes) ++sbits;
+  uint32_t axis_bits = (31u - sbits) / 3u;  // keeps every key below the padding key 0xffffffff
+  if (axis_bits > 10u) axis_bits = 10u;
+  diff_bbox_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, bbox_part);
+  diff_bbox_reduce_kernel<<<L.n_scenes, kCompactBlock, 0, s>>>(L, bbox_part, bbox);
+  diff_morton_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, bbox, 3u * axis_bits, axis_bits, keys, vals);
+  return hipGetLastError();
+}
 
 hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint64_t* masks, uint32_t* d_visible,
                              void* scan_storage, hipStream_t s) {

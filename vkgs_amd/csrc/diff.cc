@@ -61,6 +61,16 @@ struct vkgs_diff_ctx {
   Buf tile_keys, tile_order, tile_sort;
   // per-pixel state
   Buf final_t, last;
+  // spatial processing order of the scenes' splats (Morton order in each
+  // scene's bounding box, so a wave's splats tend to share their visibility):
+  // reused while every scene's means pointer and size stay the same, and
+  // refreshed every kPermRefresh forwards; any permutation is correct, the
+  // order only affects speed
+  Buf perm_keys, perm_vals, perm_bbox_part, perm_bbox, perm_sort;
+  SortPlan perm_plan{};
+  std::vector<std::pair<const float*, uint32_t>> perm_for;
+  const void* perm_buf = nullptr;  // perm_vals storage the cached order lives in
+  uint32_t perm_age = 0;
   // host mirrors of the last forward
   std::vector<DiffView> h_views;
   std::vector<DiffScene> h_scenes;
@@ -76,6 +86,8 @@ struct vkgs_diff_ctx {
 namespace {
 
 enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscSlotPairs = 3 };
+constexpr uint32_t kPermRefresh = 32;
+constexpr uint32_t kSpatialViews = 4;  // views of a scene in one call for the spatial order to pay
 
 template <typename T>
 size_t pack(std::vector<unsigned char>& out, const std::vector<T>& v) {
@@ -133,7 +145,8 @@ int vkgs_diff_destroy(vkgs_diff_ctx* c) {
                  &c->cnt,        &c->offs,     &c->scnt,         &c->soffs,      &c->scan_storage, &c->depth_sort,
                  &c->pkeys,      &c->pvals,    &c->pair_sort,    &c->ranges,     &c->final_t,    &c->last,
                  &c->tile_keys,  &c->tile_order, &c->tile_sort,  &c->pslot,      &c->gpart,
-                 &c->val64,      &c->vslot};
+                 &c->val64,      &c->vslot,    &c->perm_keys,    &c->perm_vals,  &c->perm_bbox_part,
+                 &c->perm_bbox,  &c->perm_sort};
   for (Buf* b : bufs) b->release();
   delete c;
   return VKGS_OK;
@@ -269,8 +282,39 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   ENSURE(tile_sort, c->tile_plan.bytes);
   ENSURE(final_t, (size_t)(pixels ? pixels : 1) * 4);
   ENSURE(last, (size_t)(pixels ? pixels : 1) * 4);
+  const bool spatial = (cfg->flags & VKGS_DIFF_SPATIAL_ORDER) != 0;
+  bool use_perm = false;
+  const size_t padded = (size_t)(sblocks ? sblocks : 1) * kCompactBlock;
+  if (spatial) {
+    c->perm_plan = sort_plan((uint32_t)padded);
+    ENSURE(perm_keys, padded * 4);
+    ENSURE(perm_vals, padded * 4);
+    ENSURE(perm_bbox_part, (size_t)(sblocks ? sblocks : 1) * 6 * 4);
+    ENSURE(perm_bbox, (size_t)n_scenes * 6 * 4);
+    ENSURE(perm_sort, c->perm_plan.bytes);
+  }
 #undef ENSURE
   if (e != hipSuccess) return VKGS_ERR_HIP;
+  {
+    std::vector<std::pair<const float*, uint32_t>> want(n_scenes);
+    uint64_t base = 0;
+    for (uint32_t i = 0; i < n_scenes; ++i) {
+      want[i] = {c->h_scenes[i].means, c->h_scenes[i].n};
+      const bool many = c->h_scenes[i].view_end - c->h_scenes[i].view_begin >= kSpatialViews;
+      if (spatial && many) {
+        c->h_scenes[i].perm = c->perm_vals.as<uint32_t>() + base;
+        use_perm = true;
+      }
+      base += c->h_scenes[i].n;
+    }
+    // recomputed below (after the tables are on the device) when the scenes
+    // changed, the storage moved or the order is due for a refresh
+    if (use_perm && (want != c->perm_for || c->perm_buf != c->perm_vals.p || c->perm_age >= kPermRefresh)) {
+      c->perm_for = want;
+      c->perm_buf = c->perm_vals.p;
+      c->perm_age = kPermRefresh;  // "compute now"
+    }
+  }
   uint64_t pix = 0;
   for (uint32_t v = 0; v < n_views; ++v) {
     DiffView& d = c->h_views[v];
@@ -292,6 +336,16 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   L.tile_shift_y = rs0.tile_shift_y;
   L.t_min = cfg->t_min;
   c->cap_items = cap;
+
+  // ---- processing order (when due) ----
+  if (use_perm && c->perm_age == kPermRefresh) {
+    VKGS_HIP_CHECK(launch_diff_morton_keys(L, c->perm_bbox_part.as<uint32_t>(), c->perm_bbox.as<uint32_t>(),
+                                           c->perm_keys.as<uint32_t>(), c->perm_vals.as<uint32_t>(), s));
+    VKGS_HIP_CHECK(sort_kv_launch(c->perm_plan, nullptr, c->perm_keys.as<uint32_t>(), c->perm_vals.as<uint32_t>(),
+                                  32u, c->perm_sort.p, s));
+    c->perm_age = 0;
+  }
+  if (use_perm) ++c->perm_age;
 
   // ---- forward ----
   uint32_t* misc = c->misc.as<uint32_t>();

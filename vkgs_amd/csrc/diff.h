@@ -39,6 +39,7 @@ struct DiffScene {
   float* g_quats;
   float* g_opacity;
   float* g_sh;
+  const uint32_t* perm;  // processing order: local slot i handles splat perm[i] (nullptr: i)
   uint32_t n;
   uint32_t block_base;  // first projection-backward block of this scene
   uint32_t view_begin;  // views of this scene: scene_views[view_begin .. view_end)
@@ -68,6 +69,15 @@ struct DiffLaunch {
   uint32_t tile_w, tile_h, tile_shift_x, tile_shift_y;
   float t_min;
 };
+
+// Spatial processing order of every scene's splats: keys[g] = scene index in
+// the high bits | Morton code of the position quantized in the scene's
+// bounding box (10 bits per axis at most), vals[g] = local splat index, for g
+// = scene block * kCompactBlock + lane (padding lanes get key 0xffffffff).
+// Sorted, each scene's indices are contiguous from the sum of the earlier
+// scenes' sizes.  bbox_part holds 6 words per scene block, bbox 6 per scene.
+hipError_t launch_diff_morton_keys(const DiffLaunch& L, uint32_t* bbox_part, uint32_t* bbox, uint32_t* keys,
+                                   uint32_t* vals, hipStream_t s);
 
 // Visibility ballots and per-wave visible counts of every (view, splat) item
 // (masks and counts hold kMaskWords words per compaction block), then the

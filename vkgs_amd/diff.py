@@ -96,9 +96,13 @@ def _stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
 
-def _config(tile=(8, 8), t_min=1.0 / 8192.0):
+SPATIAL_ORDER = 1  # VKGS_DIFF_SPATIAL_ORDER
+
+
+def _config(tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False):
     cfg = _lib.RasterConfig()
     cfg.tile_w, cfg.tile_h, cfg.t_min = tile[0], tile[1], t_min
+    cfg.flags = SPATIAL_ORDER if spatial_order else 0
     return cfg
 
 
@@ -212,20 +216,22 @@ def _batch_args(scenes, views):
     return vs, params
 
 
-def render_batch(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0):
+def render_batch(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False):
     """scenes: list of dicts with PARAM_NAMES tensors (cuda);
-    views: list of (scene_index, camera[, model]).  Returns a list of (H, W, 4) images."""
+    views: list of (scene_index, camera[, model]).  Returns a list of (H, W, 4) images.
+    spatial_order: process the splats of scenes with four or more views in this call in a spatial
+    (Morton) order (faster; equal depth keys then blend in that order instead of ascending id)."""
     vs, params = _batch_args(scenes, views)
-    return list(_RenderBatch.apply((vs, len(scenes), _config(tile, t_min), False), *params))
+    return list(_RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order), False), *params))
 
 
-def render_batch_flat(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0):
+def render_batch_flat(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False):
     """Same as render_batch, with every view's pixels in one (P, 4) tensor: returns (flat, spans),
     spans[i] = (offset, H, W) of view i, whose image is flat[offset:offset + H * W].view(H, W, 4)
     (see view_images).  A loss over all views can then be formed with a few whole-batch tensor
     operations instead of a few per view."""
     vs, params = _batch_args(scenes, views)
-    flat = _RenderBatch.apply((vs, len(scenes), _config(tile, t_min), True), *params)
+    flat = _RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order), True), *params)
     return flat, _spans(vs)[0]
 
 
