@@ -225,6 +225,35 @@ def test_renderer_unpacked_rect_path_matches_stage_path(c1):
     r.close()
 
 
+def test_three_pass_tile_sort_matches_stage_path(c1):
+    """At 4096x2304 with 8x8 tiles (512 x 288 = 147,456 tiles, 18-bit keys) the tile sort takes three
+    passes, so binning writes its pairs to the sort's alternate buffers; the renderer and the stage
+    path still agree on the tile lists and the image."""
+    sc, _, st, host = c1
+    W, H = 4096, 2304
+    cam = S.orbit_camera(W, H, 6.0)
+    r = P.Renderer(sc.n)
+    r.load_buffer(sc.ply_buffer(), sc.n)
+    for _ in range(4):  # an overflowing frame grows the pair storage for the next one
+        img = r.render(cam)
+        status, v, k = r.frame_info()
+        if status == 0:
+            break
+    assert status == 0 and v > 0
+    stage = P.reference_frame(cam, st, W, H)
+    sv = int(u32(stage["count"])[0])
+    assert v == sv
+    stage["image"], stage["draw_storage"] = P.draw(stage["instances"], stage["draw_indirect"], W, H,
+                                                   max_pairs=k + 1024)
+    assert stage["draw_storage"].status() == (0, k)
+    ranges, lst_ids = r.tile_lists()
+    sranges, slst = stage["draw_storage"].lists()
+    np.testing.assert_array_equal(ranges, sranges)
+    np.testing.assert_array_equal(lst_ids, u32(stage["index"])[:sv][slst])
+    np.testing.assert_array_equal(img.cpu().numpy(), stage["image"].cpu().numpy())
+    r.close()
+
+
 def test_renderer_empty_and_offscreen():
     sc = S.gaussian_cloud(1000, seed=3)
     sc.rows[:, 0] += 1000.0  # everything far outside the frustum
