@@ -333,9 +333,13 @@ __global__ __launch_bounds__(kThreads) void scan_reduce(Load in, const uint32_t*
   if (threadIdx.x == 0) partial[blockIdx.x] = all;
 }
 
+// partial[] holds the raw per-block sums of scan_reduce: each block adds up
+// those of the blocks before it (a few per thread) instead of a separate
+// single-workgroup scan launch; the last block also writes *d_total.
 template <typename Load>
 __global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, const uint32_t* __restrict__ d_count,
-                                                       uint32_t max_n, const uint32_t* __restrict__ partial) {
+                                                       uint32_t max_n, const uint32_t* __restrict__ partial,
+                                                       uint32_t* __restrict__ d_total) {
   // the sub-tile passes through LDS so that global loads and stores are
   // lane-strided (coalesced) while each thread scans kIpt consecutive words;
   // index i is stored at i + i / 32, which spreads a thread's run over banks
@@ -345,7 +349,13 @@ __global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, c
   const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
-  uint32_t carry = partial[blockIdx.x];
+  uint32_t carry;
+  {
+    uint32_t before = 0;
+    for (uint32_t i = tid; i < blockIdx.x; i += kThreads) before += partial[i];
+    block_exclusive_scan<kThreads>(before, s_tmp, &carry);
+    if (d_total != nullptr && tid == 0 && blockIdx.x == gridDim.x - 1) *d_total = carry + partial[blockIdx.x];
+  }
   for (uint32_t base = begin; base < end; base += kTile) {
 #pragma unroll
     for (int r = 0; r < kIpt; ++r) {
@@ -519,8 +529,7 @@ hipError_t scan_launch(const ScanPlan& plan, Load in, uint32_t* out, const uint3
                        void* storage, hipStream_t stream) {
   uint32_t* partial = reinterpret_cast<uint32_t*>(static_cast<char*>(storage) + plan.off_partial);
   scan_reduce<Load><<<plan.blocks, kThreads, 0, stream>>>(in, d_count, plan.max_n, partial);
-  scan_single_block<<<1, kScanThreads, 0, stream>>>(partial, plan.blocks, d_total);
-  scan_apply<Load><<<plan.blocks, kThreads, 0, stream>>>(in, out, d_count, plan.max_n, partial);
+  scan_apply<Load><<<plan.blocks, kThreads, 0, stream>>>(in, out, d_count, plan.max_n, partial, d_total);
   return hipGetLastError();
 }
 
