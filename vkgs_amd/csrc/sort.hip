@@ -6,14 +6,18 @@
 //   upsweep   : every block counts the digits of its contiguous chunk with
 //               one LDS atomic per key into lane-interleaved copies of a
 //               per-wave histogram, then writes hist[d][block];
-//   scan      : one workgroup turns hist into exclusive global offsets;
+//   scan      : one workgroup per digit turns hist into exclusive offsets
+//               (skipped for grids of at most 16 blocks, whose downsweep
+//               blocks sum the raw counts themselves);
 //   downsweep : every block walks its chunk in 2048-key sub-tiles, ranks keys
 //               stably (wave multisplit: ballots + LDS counters, then a
 //               cross-wave prefix), reorders the sub-tile in LDS and writes
 //               runs of equal digits contiguously to global memory.
-// No block ever waits on another block, so the sort cannot hang on forward
-// progress.  Counts are read from device memory (indirect semantics); the grid
-// is fixed by the capacity and each block derives its chunk from the count.
+// The narrow digit of a key width that is not a multiple of 8 goes first.  No
+// block ever waits on another block, so the sort cannot hang on forward
+// progress.  Counts are read from device memory (indirect semantics) or given
+// by the host; the grid is fixed by the capacity and each block derives its
+// chunk from the count.  Values are 32 or 64 bits.
 #pragma clang fp contract(off)
 
 #include "common.h"
