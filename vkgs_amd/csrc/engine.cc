@@ -6,7 +6,10 @@
 // on a stream.  The recording is captured once per (N, width, height, raster
 // config, output pointer) into a hipGraph and replayed; the camera and model
 // reach the kernels through a device-resident FrameConst written by a
-// one-wave kernel launched before each replay, so no host sync happens in a frame.
+// one-wave kernel launched before each replay (it also zeroes the frame's
+// counters), so no host sync happens in a frame.  When the tile grid allows
+// (rect_bits), the tile rectangle travels through the depth sort packed into
+// the value's high word instead of being gathered in depth order.
 #include <string.h>
 
 #include <vector>

@@ -6,11 +6,14 @@
 // bin_count  : one lane per draw-order position k: raster set-up of instance
 //              order[k] (centre, inverse pixel-space rot_scale, colour, pixel
 //              rectangle) into a 48-byte raster record rec[k], plus its tile
-//              rectangle and tile count;
+//              rectangle and tile count (the fused renderer writes the record
+//              at projection time instead and carries the packed rectangle in
+//              the depth sort's value: bin_emit_packed);
 // scan       : exclusive scan of the tile counts;
-// bin_emit   : (tile, k) pairs, tiles of one splat in row-major order;
+// bin_emit   : (tile, k) pairs, tiles of one splat in row-major order (the
+//              launch also zeroes the tile ranges);
 // tile sort  : stable radix sort by tile, so each tile's list keeps draw order;
-// ranges     : [start, end) of every tile;
+// ranges     : [start, end) of every tile; then the longest-first schedule;
 // rasterize  : one workgroup per tile, one lane per pixel.  Records are staged
 //              in LDS in batches of one per lane; the next batch is gathered
 //              into registers while the current one is composited, so global
