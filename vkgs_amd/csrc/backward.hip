@@ -37,10 +37,6 @@ constexpr float kAlphaMax = 0.999f;
 #define VKGS_BWD_BATCH 32  // list entries per batch of the one-wave raster backward
 #endif
 
-__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
 __device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
 // ------------------------------------------------------ forward projection ---

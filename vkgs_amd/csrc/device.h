@@ -1,5 +1,6 @@
 // Device-only helpers shared by the kernels of the plain and the batched
-// paths: the Gaussian falloff of the raster and the pair emission of binning.
+// paths: the Gaussian falloff of the raster, the wave digit match of the
+// radix ranking, and the pair emission of binning.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,6 +16,29 @@ namespace vkgs {
 __device__ __forceinline__ float gauss_falloff(float q) {
   constexpr float kNegHalfLog2e = -0.5f * 1.44269504088896340736f;
   return __builtin_amdgcn_exp2f(q * kNegHalfLog2e);
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Mask of the lanes whose kBits-bit digit equals this lane's (valid lanes
+// only): per bit one ballot and m &= ~(ballot ^ this lane's bit broadcast).
+template <int kBits>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+  const uint64_t m0 = __ballot(valid);
+  uint32_t lo = (uint32_t)m0, hi = (uint32_t)(m0 >> 32);
+#pragma unroll
+  for (int b = 0; b < kBits; ++b) {
+    const uint32_t sel = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // bit b of d, all ones or zero
+    const uint64_t bb = __ballot(sel != 0u);
+    // m &= ~(bb ^ sel): v_bitop3 with m as the middle operand and the
+    // symmetric bb / sel on either side, truth table 0x84
+    lo = __builtin_amdgcn_bitop3_b32((uint32_t)bb, lo, sel, 0x84);
+    hi = __builtin_amdgcn_bitop3_b32((uint32_t)(bb >> 32), hi, sel, 0x84);
+  }
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // Splats covering more tiles than this are emitted by a whole wave.

@@ -13,6 +13,7 @@
 #pragma clang fp contract(off)
 
 #include "common.h"
+#include "device.h"
 #include "kernels.h"
 #include "sort.h"
 
@@ -25,10 +26,6 @@ __device__ __forceinline__ float h2f(uint32_t bits16) {
 
 __device__ __forceinline__ uint32_t f2h(float f) {
   return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
-}
-
-__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 __device__ __forceinline__ const FrameConst& fcref(const FrameConst& f) { return f; }

@@ -8,7 +8,8 @@
 //              rectangle) into a 48-byte raster record rec[k], plus its tile
 //              rectangle and tile count (the fused renderer writes the record
 //              at projection time instead and carries the packed rectangle in
-//              the depth sort's value: bin_emit_packed);
+//              the depth sort's value, and the scan of its tile counts emits
+//              the pairs itself: scan_rect_counts_emit_launch in sort.hip);
 // scan       : exclusive scan of the tile counts;
 // bin_emit   : (tile, k) pairs, tiles of one splat in row-major order (the
 //              launch also zeroes the tile ranges);
@@ -114,50 +115,47 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
   }
 }
 
-// Same from depth-sorted 64-bit values slot | packed rect << 32: pair value =
-// slot, offsets from the scan of the packed rectangles' tile counts.
-__global__ __launch_bounds__(256) void bin_emit_packed_kernel(const uint64_t* __restrict__ val64,
-                                                              const uint32_t* __restrict__ offs,
-                                                              const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                              uint32_t tiles_x, RectBits rb, uint64_t max_pairs,
-                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                              uint32_t* __restrict__ status,
-                                                              uint32_t* __restrict__ clear, uint32_t n_clear) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_clear; i += gridDim.x * blockDim.x) clear[i] = 0u;
-  const uint32_t n = min(*d_count, max_n);
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  // wave-uniform loop: emit_pairs_wave needs every lane
-  for (uint32_t base = wave * kWave; base < n; base += stride) {
-    const uint32_t k = base + __lane_id();
-    uint32_t c = 0, v = 0;
-    uint64_t o = 0;
-    TileRect r = {0u, 0u};
-    if (k < n) {
-      const uint64_t w = val64[k];
-      r = unpack_rect((uint32_t)(w >> 32), rb, &c);
-      if (c != 0) {
-        o = offs[k];
-        v = (uint32_t)w;
-        if (o + c > max_pairs) atomicOr(status, 1u);
-      }
-    }
-    emit_pairs_wave(c, o, r, v, 0u, tiles_x, max_pairs, keys, vals);
-  }
-}
-
 __global__ void ranges_fill_kernel(uint32_t* __restrict__ ranges, uint32_t n2) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n2) ranges[i] = 0u;
 }
 
-__global__ void ranges_mark_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ d_pairs,
-                                   uint64_t max_pairs, uint32_t* __restrict__ ranges) {
+// A wave takes kRows x 64 consecutive keys per step, all loads in flight
+// before any compare; the neighbours come from the adjacent lanes (and the
+// adjacent row's end lanes), so only the two keys around the step are loaded
+// twice.  Positions before 0 or at K compare as kNoTile.
+__global__ __launch_bounds__(256) void ranges_mark_kernel(const uint32_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ d_pairs, uint64_t max_pairs,
+                                                          uint32_t* __restrict__ ranges) {
+  constexpr int kRows = 4;
+  constexpr uint32_t kNoTile = 0xffffffffu;  // tile ids are below 2^24
   const uint64_t K = min((uint64_t)*d_pairs, max_pairs);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t t = keys[i];
-    if (i == 0 || keys[i - 1] != t) ranges[2 * t] = (uint32_t)i;
-    if (i + 1 == K || keys[i + 1] != t) ranges[2 * t + 1] = (uint32_t)(i + 1);
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x / kWave * (kRows * kWave);
+  for (uint64_t base = wave * (kRows * kWave); base < K; base += step) {
+    uint32_t t[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const uint64_t i = base + r * kWave + lane;
+      t[r] = i < K ? keys[i] : kNoTile;
+    }
+    const uint32_t before = __builtin_amdgcn_readfirstlane(lane == 0 && base > 0 ? keys[base - 1] : kNoTile);
+    const uint64_t after_i = base + kRows * kWave;
+    const uint32_t after = __builtin_amdgcn_readfirstlane(lane == 0 && after_i < K ? keys[after_i] : kNoTile);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const uint32_t row_prev = r == 0 ? before : __builtin_amdgcn_readlane(t[r > 0 ? r - 1 : 0], kWave - 1);
+      const uint32_t row_next = r == kRows - 1 ? after : __builtin_amdgcn_readlane(t[r < kRows - 1 ? r + 1 : r], 0);
+      uint32_t prev = __shfl_up(t[r], 1, kWave), next = __shfl_down(t[r], 1, kWave);
+      if (lane == 0) prev = row_prev;
+      if (lane == kWave - 1) next = row_next;
+      const uint64_t i = base + r * kWave + lane;
+      if (i < K) {
+        if (prev != t[r]) ranges[2 * t[r]] = (uint32_t)i;
+        if (next != t[r]) ranges[2 * t[r] + 1] = (uint32_t)(i + 1);
+      }
+    }
   }
 }
 
@@ -166,14 +164,19 @@ __global__ void ranges_mark_kernel(const uint32_t* __restrict__ keys, const uint
 // leading one of len + 1, e its exponent), value = tile id; an ascending
 // stable one-pass sort puts long lists first.  Lengths within 1/8 of an
 // octave share a bucket, which is fine for a schedule.
+__device__ __forceinline__ uint32_t tile_order_key(const uint32_t* __restrict__ ranges, uint32_t t) {
+  const uint2 r = reinterpret_cast<const uint2*>(ranges)[t];
+  const uint32_t v = r.y - r.x + 1u;  // >= 1 (a list holds fewer than 2^32 pairs)
+  const uint32_t e = 31u - __clz(v);
+  const uint32_t m = (e >= 3u ? v >> (e - 3u) : v << (3u - e)) & 7u;
+  return 255u - (8u * e + m);
+}
+
 __global__ void tile_order_keys_kernel(const uint32_t* __restrict__ ranges, uint32_t tiles,
                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tiles) return;
-  const uint32_t v = ranges[2 * t + 1] - ranges[2 * t] + 1u;  // >= 1 (a list holds fewer than 2^32 pairs)
-  const uint32_t e = 31u - __clz(v);
-  const uint32_t m = (e >= 3u ? v >> (e - 3u) : v << (3u - e)) & 7u;
-  keys[t] = 255u - (8u * e + m);
+  keys[t] = tile_order_key(ranges, t);
   vals[t] = t;
 }
 
@@ -406,22 +409,11 @@ hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint
   return hipGetLastError();
 }
 
-hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
-                                  const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
-                                  uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear,
-                                  uint32_t n_clear) {
-  if (max_n == 0) return clear && n_clear ? fill_u32_launch(clear, 0u, n_clear, s) : hipSuccess;
-  bin_emit_packed_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(val64, offs, d_count, max_n, rs.tiles_x, rb,
-                                                                     max_pairs, keys, vals, status, clear,
-                                                                     clear ? n_clear : 0u);
-  return hipGetLastError();
-}
-
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s, bool cleared) {
   if (!cleared) ranges_fill_kernel<<<grid_for(2ull * tiles, 256, 1u << 20), 256, 0, s>>>(ranges, 2 * tiles);
   if (max_pairs > 0)
-    ranges_mark_kernel<<<grid_for(max_pairs, 256, 8192), 256, 0, s>>>(keys, d_pairs, max_pairs, ranges);
+    ranges_mark_kernel<<<grid_for(max_pairs, 1024, 2048), 256, 0, s>>>(keys, d_pairs, max_pairs, ranges);
   return hipGetLastError();
 }
 

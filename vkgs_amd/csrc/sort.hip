@@ -18,9 +18,15 @@
 // progress.  Counts are read from device memory (indirect semantics) or given
 // by the host; the grid is fixed by the capacity and each block derives its
 // chunk from the count.  Values are 32 or 64 bits.
+//
+// The exclusive scan is two kernels (block sums, then each block adds the
+// sums before it and scans its chunk through LDS); its output goes to a sink:
+// stored offsets, or, for the packed tile rectangles of binning, the pairs
+// themselves.
 #pragma clang fp contract(off)
 
 #include "common.h"
+#include "device.h"
 #include "sort.h"
 
 namespace vkgs {
@@ -38,29 +44,6 @@ constexpr int kTile = kThreads * kIpt;  // keys per sub-tile
 #endif
 constexpr int kMaxBlocks = VKGS_SORT_MAX_BLOCKS;
 constexpr uint32_t kRawHistBlocks = 16;
-constexpr int kScanThreads = 1024;
-
-__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Mask of the lanes whose kBits-bit digit equals this lane's (valid lanes
-// only): per bit one ballot and m &= ~(ballot ^ this lane's bit broadcast).
-template <int kBits>
-__device__ __forceinline__ uint64_t match_digit_t(uint32_t d, bool valid) {
-  const uint64_t m0 = __ballot(valid);
-  uint32_t lo = (uint32_t)m0, hi = (uint32_t)(m0 >> 32);
-#pragma unroll
-  for (int b = 0; b < kBits; ++b) {
-    const uint32_t sel = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // bit b of d, all ones or zero
-    const uint64_t bb = __ballot(sel != 0u);
-    // m &= ~(bb ^ sel): v_bitop3 with m as the middle operand and the
-    // symmetric bb / sel on either side, truth table 0x84
-    lo = __builtin_amdgcn_bitop3_b32((uint32_t)bb, lo, sel, 0x84);
-    hi = __builtin_amdgcn_bitop3_b32((uint32_t)(bb >> 32), hi, sel, 0x84);
-  }
-  return ((uint64_t)hi << 32) | lo;
-}
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   const int lane = __lane_id();
@@ -233,7 +216,7 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
     for (int r = 0; r < kIpt; ++r) {
       const bool valid = seg + r * kWave + lane < end;
       const uint32_t d = (k[r] >> shift) & mask;
-      const uint64_t peers = match_digit_t<kBits>(d, valid);
+      const uint64_t peers = match_digit<kBits>(d, valid);
       const uint32_t below = mbcnt(peers);
       const uint32_t old = valid ? s_wcnt[w][d] : 0u;
       __builtin_amdgcn_wave_barrier();
@@ -288,25 +271,6 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
   }
 }
 
-// Exclusive scan of `total` words in place by one workgroup.
-__global__ __launch_bounds__(kScanThreads) void scan_single_block(uint32_t* __restrict__ data, uint32_t total,
-                                                                  uint32_t* __restrict__ d_sum) {
-  __shared__ uint32_t s_tmp[kScanThreads / kWave];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t per = (total + kScanThreads - 1) / kScanThreads;
-  const uint32_t start = min(tid * per, total), stop = min(start + per, total);
-  uint32_t s = 0;
-  for (uint32_t i = start; i < stop; ++i) s += data[i];
-  uint32_t all;
-  uint32_t run = block_exclusive_scan<kScanThreads>(s, s_tmp, &all);
-  for (uint32_t i = start; i < stop; ++i) {
-    const uint32_t v = data[i];
-    data[i] = run;
-    run += v;
-  }
-  if (d_sum != nullptr && tid == 0) *d_sum = all;
-}
-
 // Scan inputs: plain words, or the tile count of the packed rectangle in the
 // high word of 64-bit sort values.
 struct LoadWords {
@@ -320,6 +284,43 @@ struct LoadRectCount {
     uint32_t c;
     unpack_rect((uint32_t)(v[i] >> 32), rb, &c);
     return c;
+  }
+};
+
+// Scan outputs, called by every lane of a wave for consecutive items i with
+// the exclusive offset: store it, or emit the (tile, slot) pairs of packed
+// sort value i at that offset (binning straight from the scan, so the
+// offsets never go through memory).  begin() runs once per thread first.
+struct StoreOffsets {
+  uint32_t* out;
+  __device__ void begin() const {}
+  __device__ void operator()(uint32_t i, bool valid, uint32_t off) const {
+    if (valid) out[i] = off;
+  }
+};
+struct EmitPackedPairs {
+  const uint64_t* v;
+  RectBits rb;
+  uint32_t tiles_x;
+  uint64_t max_pairs;
+  uint32_t* keys;
+  uint32_t* vals;
+  uint32_t* status;
+  uint32_t* clear;  // n_clear words zeroed by the launch (the tile ranges)
+  uint32_t n_clear;
+  __device__ void begin() const {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_clear; i += gridDim.x * blockDim.x) clear[i] = 0u;
+  }
+  __device__ void operator()(uint32_t i, bool valid, uint32_t off) const {
+    uint32_t c = 0, slot = 0;
+    TileRect r = {0u, 0u};
+    if (valid) {
+      const uint64_t w = v[i];
+      r = unpack_rect((uint32_t)(w >> 32), rb, &c);
+      slot = (uint32_t)w;
+      if (c != 0 && (uint64_t)off + c > max_pairs) atomicOr(status, 1u);
+    }
+    emit_pairs_wave(c, off, r, slot, 0u, tiles_x, max_pairs, keys, vals);
   }
 };
 
@@ -340,8 +341,8 @@ __global__ __launch_bounds__(kThreads) void scan_reduce(Load in, const uint32_t*
 // partial[] holds the raw per-block sums of scan_reduce: each block adds up
 // those of the blocks before it (a few per thread) instead of a separate
 // single-workgroup scan launch; the last block also writes *d_total.
-template <typename Load>
-__global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, const uint32_t* __restrict__ d_count,
+template <typename Load, typename Sink>
+__global__ __launch_bounds__(kThreads) void scan_apply(Load in, Sink sink, const uint32_t* __restrict__ d_count,
                                                        uint32_t max_n, const uint32_t* __restrict__ partial,
                                                        uint32_t* __restrict__ d_total) {
   // the sub-tile passes through LDS so that global loads and stores are
@@ -350,6 +351,7 @@ __global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, c
   __shared__ uint32_t s_tmp[kWaves];
   __shared__ uint32_t s_x[kTile + kTile / 32];
   const uint32_t tid = threadIdx.x;
+  sink.begin();
   const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, c
 #pragma unroll
     for (int r = 0; r < kIpt; ++r) {
       const uint32_t i = r * kThreads + tid;
-      if (base + i < end) out[base + i] = s_x[i + (i >> 5)];
+      sink(base + i, base + i < end, s_x[i + (i >> 5)]);
     }
     __syncthreads();
     carry += all;
@@ -516,11 +518,12 @@ hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint3
   return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream);
 }
 
-ScanPlan scan_plan(uint32_t max_n) {
+ScanPlan scan_plan(uint32_t max_n, uint32_t max_blocks) {
   ScanPlan p;
   p.max_n = max_n;
+  const uint32_t cap = max_blocks ? max_blocks : (uint32_t)kMaxBlocks;
   uint32_t tiles = ceil_div(max_n ? max_n : 1, kTile);
-  p.blocks = tiles < (uint32_t)kMaxBlocks ? tiles : (uint32_t)kMaxBlocks;
+  p.blocks = tiles < cap ? tiles : cap;
   p.off_partial = 0;
   p.bytes = align_up((size_t)p.blocks * 4, 256);
   return p;
@@ -528,12 +531,12 @@ ScanPlan scan_plan(uint32_t max_n) {
 
 namespace {
 
-template <typename Load>
-hipError_t scan_launch(const ScanPlan& plan, Load in, uint32_t* out, const uint32_t* d_count, uint32_t* d_total,
+template <typename Load, typename Sink>
+hipError_t scan_launch(const ScanPlan& plan, Load in, Sink sink, const uint32_t* d_count, uint32_t* d_total,
                        void* storage, hipStream_t stream) {
   uint32_t* partial = reinterpret_cast<uint32_t*>(static_cast<char*>(storage) + plan.off_partial);
   scan_reduce<Load><<<plan.blocks, kThreads, 0, stream>>>(in, d_count, plan.max_n, partial);
-  scan_apply<Load><<<plan.blocks, kThreads, 0, stream>>>(in, out, d_count, plan.max_n, partial, d_total);
+  scan_apply<Load, Sink><<<plan.blocks, kThreads, 0, stream>>>(in, sink, d_count, plan.max_n, partial, d_total);
   return hipGetLastError();
 }
 
@@ -541,17 +544,20 @@ hipError_t scan_launch(const ScanPlan& plan, Load in, uint32_t* out, const uint3
 
 hipError_t scan_exclusive_launch(const ScanPlan& plan, const uint32_t* in, uint32_t* out, const uint32_t* d_count,
                                  uint32_t* d_total, void* storage, hipStream_t stream) {
-  return scan_launch(plan, LoadWords{in}, out, d_count, d_total, storage, stream);
+  return scan_launch(plan, LoadWords{in}, StoreOffsets{out}, d_count, d_total, storage, stream);
 }
 
 hipError_t scan_rect_counts_launch(const ScanPlan& plan, const uint64_t* val64, const RectBits& rb, uint32_t* out,
                                    const uint32_t* d_count, uint32_t* d_total, void* storage, hipStream_t stream) {
-  return scan_launch(plan, LoadRectCount{val64, rb}, out, d_count, d_total, storage, stream);
+  return scan_launch(plan, LoadRectCount{val64, rb}, StoreOffsets{out}, d_count, d_total, storage, stream);
 }
 
-hipError_t scan_small_launch(uint32_t* data, uint32_t total, uint32_t* d_sum, hipStream_t stream) {
-  scan_single_block<<<1, kScanThreads, 0, stream>>>(data, total, d_sum);
-  return hipGetLastError();
+hipError_t scan_rect_counts_emit_launch(const ScanPlan& plan, const uint64_t* val64, const RectBits& rb,
+                                        const uint32_t* d_count, uint32_t* d_total, void* storage, uint32_t tiles_x,
+                                        uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* status,
+                                        uint32_t* clear, uint32_t n_clear, hipStream_t stream) {
+  const EmitPackedPairs sink{val64, rb, tiles_x, max_pairs, keys, vals, status, clear, clear ? n_clear : 0u};
+  return scan_launch(plan, LoadRectCount{val64, rb}, sink, d_count, d_total, storage, stream);
 }
 
 hipError_t fill_u32_launch(uint32_t* dst, uint32_t value, size_t n, hipStream_t stream) {
