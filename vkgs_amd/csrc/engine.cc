@@ -115,9 +115,7 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   r->packed = rect_bits(rs.tiles_x, rs.tiles_y, &r->rb);
   r->depth_plan = sort_plan(cap, r->packed ? 8u : 4u);
   r->pair_plan = sort_plan((uint32_t)r->max_pairs);
-  // up to 4096 workgroups: the packed path's scan also emits the pairs, which
-  // wants more waves in flight than a plain scan
-  r->scan = scan_plan(cap, 4096);
+  r->scan = scan_plan(cap);
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   if ((e = r->fc.ensure(sizeof(FrameConst))) != hipSuccess) return e;
   if ((e = r->misc.ensure(kMiscWords * 4)) != hipSuccess) return e;
@@ -132,11 +130,11 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   if ((e = r->ids.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->sort_storage.ensure(r->depth_plan.bytes)) != hipSuccess) return e;
   if ((e = r->rec.ensure((size_t)cap * 48)) != hipSuccess) return e;
-  if (!r->packed) {  // depth-ordered rectangles, counts and their scan of the gather path
+  if (!r->packed) {  // depth-ordered rectangles and counts of the gather path
     if ((e = r->rect.ensure((size_t)cap * sizeof(TileRect))) != hipSuccess) return e;
     if ((e = r->cnt.ensure((size_t)cap * 4)) != hipSuccess) return e;
-    if ((e = r->offs.ensure((size_t)cap * 4)) != hipSuccess) return e;
   }
+  if ((e = r->offs.ensure((size_t)cap * 4)) != hipSuccess) return e;
   if ((e = r->scan_storage.ensure(r->scan.bytes)) != hipSuccess) return e;
   if ((e = r->pkeys.ensure((size_t)r->max_pairs * 4)) != hipSuccess) return e;
   if ((e = r->pvals.ensure((size_t)r->max_pairs * 4)) != hipSuccess) return e;

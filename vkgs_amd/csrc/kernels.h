@@ -82,6 +82,12 @@ hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint
                            const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
                            uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s,
                            uint32_t* clear = nullptr, uint32_t n_clear = 0);
+// (tile, slot) pairs at offs[k] from depth-sorted values val64[k] = slot |
+// packed rect << 32 (offs = exclusive scan of the packed tile counts).
+hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
+                                  const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
+                                  uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear = nullptr,
+                                  uint32_t n_clear = 0);
 // [start, end) per tile from the tile-sorted keys; `cleared`: the ranges were
 // already zeroed (by the emit launch).
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,

@@ -8,8 +8,7 @@
 //              rectangle) into a 48-byte raster record rec[k], plus its tile
 //              rectangle and tile count (the fused renderer writes the record
 //              at projection time instead and carries the packed rectangle in
-//              the depth sort's value, and the scan of its tile counts emits
-//              the pairs itself: scan_rect_counts_emit_launch in sort.hip);
+//              the depth sort's value: bin_emit_packed);
 // scan       : exclusive scan of the tile counts;
 // bin_emit   : (tile, k) pairs, tiles of one splat in row-major order (the
 //              launch also zeroes the tile ranges);
@@ -115,6 +114,38 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
   }
 }
 
+// Same from depth-sorted 64-bit values slot | packed rect << 32: pair value =
+// slot, offsets from the scan of the packed rectangles' tile counts.
+__global__ __launch_bounds__(256) void bin_emit_packed_kernel(const uint64_t* __restrict__ val64,
+                                                              const uint32_t* __restrict__ offs,
+                                                              const uint32_t* __restrict__ d_count, uint32_t max_n,
+                                                              uint32_t tiles_x, RectBits rb, uint64_t max_pairs,
+                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                              uint32_t* __restrict__ status,
+                                                              uint32_t* __restrict__ clear, uint32_t n_clear) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_clear; i += gridDim.x * blockDim.x) clear[i] = 0u;
+  const uint32_t n = min(*d_count, max_n);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // wave-uniform loop: emit_pairs_wave needs every lane
+  for (uint32_t base = wave * kWave; base < n; base += stride) {
+    const uint32_t k = base + __lane_id();
+    uint32_t c = 0, v = 0;
+    uint64_t o = 0;
+    TileRect r = {0u, 0u};
+    if (k < n) {
+      const uint64_t w = val64[k];
+      r = unpack_rect((uint32_t)(w >> 32), rb, &c);
+      if (c != 0) {
+        o = offs[k];
+        v = (uint32_t)w;
+        if (o + c > max_pairs) atomicOr(status, 1u);
+      }
+    }
+    emit_pairs_wave(c, o, r, v, 0u, tiles_x, max_pairs, keys, vals);
+  }
+}
+
 __global__ void ranges_fill_kernel(uint32_t* __restrict__ ranges, uint32_t n2) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n2) ranges[i] = 0u;
@@ -164,19 +195,14 @@ __global__ __launch_bounds__(256) void ranges_mark_kernel(const uint32_t* __rest
 // leading one of len + 1, e its exponent), value = tile id; an ascending
 // stable one-pass sort puts long lists first.  Lengths within 1/8 of an
 // octave share a bucket, which is fine for a schedule.
-__device__ __forceinline__ uint32_t tile_order_key(const uint32_t* __restrict__ ranges, uint32_t t) {
-  const uint2 r = reinterpret_cast<const uint2*>(ranges)[t];
-  const uint32_t v = r.y - r.x + 1u;  // >= 1 (a list holds fewer than 2^32 pairs)
-  const uint32_t e = 31u - __clz(v);
-  const uint32_t m = (e >= 3u ? v >> (e - 3u) : v << (3u - e)) & 7u;
-  return 255u - (8u * e + m);
-}
-
 __global__ void tile_order_keys_kernel(const uint32_t* __restrict__ ranges, uint32_t tiles,
                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tiles) return;
-  keys[t] = tile_order_key(ranges, t);
+  const uint32_t v = ranges[2 * t + 1] - ranges[2 * t] + 1u;  // >= 1 (a list holds fewer than 2^32 pairs)
+  const uint32_t e = 31u - __clz(v);
+  const uint32_t m = (e >= 3u ? v >> (e - 3u) : v << (3u - e)) & 7u;
+  keys[t] = 255u - (8u * e + m);
   vals[t] = t;
 }
 
@@ -406,6 +432,17 @@ hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint
   bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rect, cnt, offs, order, d_count, max_n, rs.tiles_x,
                                                               max_pairs, keys, vals, status, clear,
                                                               clear ? n_clear : 0u);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
+                                  const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
+                                  uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear,
+                                  uint32_t n_clear) {
+  if (max_n == 0) return clear && n_clear ? fill_u32_launch(clear, 0u, n_clear, s) : hipSuccess;
+  bin_emit_packed_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(val64, offs, d_count, max_n, rs.tiles_x, rb,
+                                                                     max_pairs, keys, vals, status, clear,
+                                                                     clear ? n_clear : 0u);
   return hipGetLastError();
 }
 

@@ -43,9 +43,7 @@ struct ScanPlan {
   size_t bytes = 0;
 };
 
-// Workgroups of 2048-item chunks, at most max_blocks (0: the sort's default
-// of 1024); each workgroup of the apply kernel sums the partials before it.
-ScanPlan scan_plan(uint32_t max_n, uint32_t max_blocks = 0);
+ScanPlan scan_plan(uint32_t max_n);
 
 // out[i] = sum(in[0..i)) for i < min(*d_count, max_n) (d_count == nullptr:
 // i < max_n); *d_total = sum of all.  in and out may alias.
@@ -58,14 +56,6 @@ struct RectBits;
 hipError_t scan_rect_counts_launch(const ScanPlan& plan, const uint64_t* val64, const RectBits& rb, uint32_t* out,
                                    const uint32_t* d_count, uint32_t* d_total, void* storage, hipStream_t stream);
 
-// The same scan feeding binning directly: each packed value's (tile, slot)
-// pairs are written at its exclusive offset as keys / vals (row-major tiles;
-// pairs at or beyond max_pairs dropped and *status |= 1); *d_total receives
-// the pair count; n_clear words at clear (optional) are zeroed by the launch.
-hipError_t scan_rect_counts_emit_launch(const ScanPlan& plan, const uint64_t* val64, const RectBits& rb,
-                                        const uint32_t* d_count, uint32_t* d_total, void* storage, uint32_t tiles_x,
-                                        uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* status,
-                                        uint32_t* clear, uint32_t n_clear, hipStream_t stream);
 
 // Fills n words with a value (graph-safe replacement for hipMemsetAsync).
 hipError_t fill_u32_launch(uint32_t* dst, uint32_t value, size_t n, hipStream_t stream);

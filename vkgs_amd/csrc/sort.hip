@@ -19,10 +19,8 @@
 // by the host; the grid is fixed by the capacity and each block derives its
 // chunk from the count.  Values are 32 or 64 bits.
 //
-// The exclusive scan is two kernels (block sums, then each block adds the
-// sums before it and scans its chunk through LDS); its output goes to a sink:
-// stored offsets, or, for the packed tile rectangles of binning, the pairs
-// themselves.
+// The exclusive scan is two kernels: block sums, then each block adds the
+// sums before it and scans its chunk through LDS.
 #pragma clang fp contract(off)
 
 #include "common.h"
@@ -287,43 +285,6 @@ struct LoadRectCount {
   }
 };
 
-// Scan outputs, called by every lane of a wave for consecutive items i with
-// the exclusive offset: store it, or emit the (tile, slot) pairs of packed
-// sort value i at that offset (binning straight from the scan, so the
-// offsets never go through memory).  begin() runs once per thread first.
-struct StoreOffsets {
-  uint32_t* out;
-  __device__ void begin() const {}
-  __device__ void operator()(uint32_t i, bool valid, uint32_t off) const {
-    if (valid) out[i] = off;
-  }
-};
-struct EmitPackedPairs {
-  const uint64_t* v;
-  RectBits rb;
-  uint32_t tiles_x;
-  uint64_t max_pairs;
-  uint32_t* keys;
-  uint32_t* vals;
-  uint32_t* status;
-  uint32_t* clear;  // n_clear words zeroed by the launch (the tile ranges)
-  uint32_t n_clear;
-  __device__ void begin() const {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_clear; i += gridDim.x * blockDim.x) clear[i] = 0u;
-  }
-  __device__ void operator()(uint32_t i, bool valid, uint32_t off) const {
-    uint32_t c = 0, slot = 0;
-    TileRect r = {0u, 0u};
-    if (valid) {
-      const uint64_t w = v[i];
-      r = unpack_rect((uint32_t)(w >> 32), rb, &c);
-      slot = (uint32_t)w;
-      if (c != 0 && (uint64_t)off + c > max_pairs) atomicOr(status, 1u);
-    }
-    emit_pairs_wave(c, off, r, slot, 0u, tiles_x, max_pairs, keys, vals);
-  }
-};
-
 template <typename Load>
 __global__ __launch_bounds__(kThreads) void scan_reduce(Load in, const uint32_t* __restrict__ d_count,
                                                         uint32_t max_n, uint32_t* __restrict__ partial) {
@@ -341,8 +302,8 @@ __global__ __launch_bounds__(kThreads) void scan_reduce(Load in, const uint32_t*
 // partial[] holds the raw per-block sums of scan_reduce: each block adds up
 // those of the blocks before it (a few per thread) instead of a separate
 // single-workgroup scan launch; the last block also writes *d_total.
-template <typename Load, typename Sink>
-__global__ __launch_bounds__(kThreads) void scan_apply(Load in, Sink sink, const uint32_t* __restrict__ d_count,
+template <typename Load>
+__global__ __launch_bounds__(kThreads) void scan_apply(Load in, uint32_t* out, const uint32_t* __restrict__ d_count,
                                                        uint32_t max_n, const uint32_t* __restrict__ partial,
                                                        uint32_t* __restrict__ d_total) {
   // the sub-tile passes through LDS so that global loads and stores are
@@ -351,7 +312,6 @@ __global__ __launch_bounds__(kThreads) void scan_apply(Load in, Sink sink, const
   __shared__ uint32_t s_tmp[kWaves];
   __shared__ uint32_t s_x[kTile + kTile / 32];
   const uint32_t tid = threadIdx.x;
-  sink.begin();
   const uint32_t n = d_count ? min(*d_count, max_n) : max_n;
   uint32_t begin, end;
   block_range(n, begin, end);
@@ -389,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void scan_apply(Load in, Sink sink, const
 #pragma unroll
     for (int r = 0; r < kIpt; ++r) {
       const uint32_t i = r * kThreads + tid;
-      sink(base + i, base + i < end, s_x[i + (i >> 5)]);
+      if (base + i < end) out[base + i] = s_x[i + (i >> 5)];
     }
     __syncthreads();
     carry += all;
@@ -518,12 +478,11 @@ hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint3
   return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream);
 }
 
-ScanPlan scan_plan(uint32_t max_n, uint32_t max_blocks) {
+ScanPlan scan_plan(uint32_t max_n) {
   ScanPlan p;
   p.max_n = max_n;
-  const uint32_t cap = max_blocks ? max_blocks : (uint32_t)kMaxBlocks;
   uint32_t tiles = ceil_div(max_n ? max_n : 1, kTile);
-  p.blocks = tiles < cap ? tiles : cap;
+  p.blocks = tiles < (uint32_t)kMaxBlocks ? tiles : (uint32_t)kMaxBlocks;
   p.off_partial = 0;
   p.bytes = align_up((size_t)p.blocks * 4, 256);
   return p;
@@ -531,12 +490,12 @@ ScanPlan scan_plan(uint32_t max_n, uint32_t max_blocks) {
 
 namespace {
 
-template <typename Load, typename Sink>
-hipError_t scan_launch(const ScanPlan& plan, Load in, Sink sink, const uint32_t* d_count, uint32_t* d_total,
+template <typename Load>
+hipError_t scan_launch(const ScanPlan& plan, Load in, uint32_t* out, const uint32_t* d_count, uint32_t* d_total,
                        void* storage, hipStream_t stream) {
   uint32_t* partial = reinterpret_cast<uint32_t*>(static_cast<char*>(storage) + plan.off_partial);
   scan_reduce<Load><<<plan.blocks, kThreads, 0, stream>>>(in, d_count, plan.max_n, partial);
-  scan_apply<Load, Sink><<<plan.blocks, kThreads, 0, stream>>>(in, sink, d_count, plan.max_n, partial, d_total);
+  scan_apply<Load><<<plan.blocks, kThreads, 0, stream>>>(in, out, d_count, plan.max_n, partial, d_total);
   return hipGetLastError();
 }
 
@@ -544,20 +503,12 @@ hipError_t scan_launch(const ScanPlan& plan, Load in, Sink sink, const uint32_t*
 
 hipError_t scan_exclusive_launch(const ScanPlan& plan, const uint32_t* in, uint32_t* out, const uint32_t* d_count,
                                  uint32_t* d_total, void* storage, hipStream_t stream) {
-  return scan_launch(plan, LoadWords{in}, StoreOffsets{out}, d_count, d_total, storage, stream);
+  return scan_launch(plan, LoadWords{in}, out, d_count, d_total, storage, stream);
 }
 
 hipError_t scan_rect_counts_launch(const ScanPlan& plan, const uint64_t* val64, const RectBits& rb, uint32_t* out,
                                    const uint32_t* d_count, uint32_t* d_total, void* storage, hipStream_t stream) {
-  return scan_launch(plan, LoadRectCount{val64, rb}, StoreOffsets{out}, d_count, d_total, storage, stream);
-}
-
-hipError_t scan_rect_counts_emit_launch(const ScanPlan& plan, const uint64_t* val64, const RectBits& rb,
-                                        const uint32_t* d_count, uint32_t* d_total, void* storage, uint32_t tiles_x,
-                                        uint64_t max_pairs, uint32_t* keys, uint32_t* vals, uint32_t* status,
-                                        uint32_t* clear, uint32_t n_clear, hipStream_t stream) {
-  const EmitPackedPairs sink{val64, rb, tiles_x, max_pairs, keys, vals, status, clear, clear ? n_clear : 0u};
-  return scan_launch(plan, LoadRectCount{val64, rb}, sink, d_count, d_total, storage, stream);
+  return scan_launch(plan, LoadRectCount{val64, rb}, out, d_count, d_total, storage, stream);
 }
 
 hipError_t fill_u32_launch(uint32_t* dst, uint32_t value, size_t n, hipStream_t stream) {
