@@ -189,9 +189,9 @@ int vkgs_draw(const float* d_instances, const uint32_t* d_draw_indirect, uint32_
 int vkgs_draw_status(const void* d_storage, size_t storage_bytes, uint64_t* pairs, void* stream);
 
 /* Tile lists of the last vkgs_draw on this storage (device pointers into the
- * storage): d_ranges = uint32 [tiles][2] = [start, end), d_list = instance
- * indices in draw order per tile.  Synchronous (reads the storage header).
- * For parity checks and tools. */
+ * storage): d_ranges = uint32 [tiles][2] = [start, end) ([0, 0) for a tile no
+ * splat touches), d_list = instance indices in draw order per tile.
+ * Synchronous (reads the storage header).  For parity checks and tools. */
 int vkgs_draw_lists(const void* d_storage, size_t storage_bytes, const uint32_t** d_ranges,
                     const uint32_t** d_list, uint32_t* tiles_x, uint32_t* tiles_y);
 
