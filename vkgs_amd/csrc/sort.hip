@@ -41,6 +41,10 @@ constexpr int kTile = kThreads * kIpt;  // keys per sub-tile
 #define VKGS_SORT_MAX_BLOCKS 1024
 #endif
 constexpr int kMaxBlocks = VKGS_SORT_MAX_BLOCKS;
+// A scan's apply blocks each sum the partials before them, which stays cheap
+// at twice the sort's grid and gives the binning scan more waves in flight
+// (6M splats: bin 0.063 -> 0.059 ms; the sort itself is fastest at 1024).
+constexpr int kMaxScanBlocks = 2048;
 constexpr uint32_t kRawHistBlocks = 16;
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
@@ -488,7 +492,7 @@ ScanPlan scan_plan(uint32_t max_n) {
   ScanPlan p;
   p.max_n = max_n;
   uint32_t tiles = ceil_div(max_n ? max_n : 1, kTile);
-  p.blocks = tiles < (uint32_t)kMaxBlocks ? tiles : (uint32_t)kMaxBlocks;
+  p.blocks = tiles < (uint32_t)kMaxScanBlocks ? tiles : (uint32_t)kMaxScanBlocks;
   p.off_partial = 0;
   p.bytes = align_up((size_t)p.blocks * 4, 256);
   return p;
