@@ -42,7 +42,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
 }
 
 // Splats covering more tiles than this are emitted by a whole wave.
-constexpr uint32_t kEmitSerial = 16;
+constexpr uint32_t kEmitSerial = 8;
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t x, uint32_t lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)lane);
