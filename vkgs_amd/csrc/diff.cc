@@ -365,14 +365,14 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
                                        misc + kMiscSlotPairs, c->scan_storage.p, s));
   if (packed) {
     VKGS_HIP_CHECK(sort_kv64_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(),
-                                    c->val64.as<uint64_t>(), 30u, c->depth_sort.p, s));
+                                    c->val64.as<uint64_t>(), kDepthKeyBits, c->depth_sort.p, s, kDepthNarrowPass));
     VKGS_HIP_CHECK(launch_diff_view_slots(L, c->counts.as<uint32_t>(), misc + kMiscVisible, c->vslot.as<uint32_t>(),
                                           s));
     VKGS_HIP_CHECK(scan_rect_counts_launch(c->scan, c->val64.as<uint64_t>(), rb, c->offs.as<uint32_t>(),
                                            misc + kMiscVisible, misc + kMiscPairs, c->scan_storage.p, s));
   } else {
-    VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, 30u,
-                                  c->depth_sort.p, s));
+    VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, kDepthKeyBits,
+                                  c->depth_sort.p, s, kDepthNarrowPass));
     VKGS_HIP_CHECK(launch_diff_bin_gather(c->rc.as<uint4>(), order, misc + kMiscVisible, cap,
                                           c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->kview.as<uint32_t>(), s));
     VKGS_HIP_CHECK(scan_exclusive_launch(c->scan, c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),

@@ -196,11 +196,11 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
   if (timed) hipEventRecord(r->ev[2], s);
   // keys are bits(1 - z) <= 0x3f800000: 30 significant bits
   if (r->packed)
-    e = sort_kv64_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val64.as<uint64_t>(), 30u,
-                         r->sort_storage.p, s);
+    e = sort_kv64_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val64.as<uint64_t>(),
+                         kDepthKeyBits, r->sort_storage.p, s, kDepthNarrowPass);
   else
-    e = sort_kv_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val.as<uint32_t>(), 30u,
-                       r->sort_storage.p, s);
+    e = sort_kv_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val.as<uint32_t>(),
+                       kDepthKeyBits, r->sort_storage.p, s, kDepthNarrowPass);
   if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[3], s);
   hipEvent_t marks[2] = {r->ev[4], r->ev[5]};

@@ -13,7 +13,8 @@
 //               stably (wave multisplit: ballots + LDS counters, then a
 //               cross-wave prefix), reorders the sub-tile in LDS and writes
 //               runs of equal digits contiguously to global memory.
-// The narrow digit of a key width that is not a multiple of 8 goes first.  No
+// The narrow digit of a key width that is not a multiple of 8 goes first
+// unless the caller names another pass (depth keys, see sort.h).  No
 // block ever waits on another block, so the sort cannot hang on forward
 // progress.  Counts are read from device memory (indirect semantics) or given
 // by the host; the grid is fixed by the capacity and each block derives its
@@ -409,7 +410,7 @@ namespace {
 
 template <typename V>
 hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, V* vals, uint32_t key_bits,
-                       void* storage, hipStream_t stream, bool input_in_alt = false) {
+                       void* storage, hipStream_t stream, bool input_in_alt = false, uint32_t narrow_pass = 0) {
   if (key_bits > 32) key_bits = 32;
   char* base = static_cast<char*>(storage);
   uint32_t* alt_k = reinterpret_cast<uint32_t*>(base + plan.off_alt_key);
@@ -428,10 +429,13 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
   // small grids: each downsweep block sums the raw block counts itself,
   // which is cheaper than a row-scan launch
   const bool raw = plan.blocks <= kRawHistBlocks;
-  // the narrow digit (when key_bits is not a multiple of 8) goes first
-  const uint32_t first = key_bits - 8 * ((key_bits + 7) / 8 - 1);
-  for (uint32_t shift = 0; shift < key_bits;) {
-    const uint32_t bits = shift == 0 ? first : 8;
+  // the narrow digit (when key_bits is not a multiple of 8) goes in pass
+  // narrow_pass, the first by default
+  const uint32_t n_pass = (key_bits + 7) / 8;
+  const uint32_t narrow = key_bits - 8 * (n_pass - 1);
+  const uint32_t narrow_at = narrow_pass < n_pass ? narrow_pass : n_pass - 1;
+  for (uint32_t shift = 0, pass = 0; shift < key_bits; ++pass) {
+    const uint32_t bits = pass == narrow_at ? narrow : 8;
     sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
     if (!raw) sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total);
     switch (bits) {
@@ -466,8 +470,8 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
 }  // namespace
 
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
-                          uint32_t key_bits, void* storage, hipStream_t stream) {
-  return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream);
+                          uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass) {
+  return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass);
 }
 
 uint32_t sort_passes(uint32_t key_bits) { return ((key_bits > 32 ? 32 : key_bits) + 7) / 8; }
@@ -484,8 +488,8 @@ hipError_t sort_kv_from_alt_launch(const SortPlan& plan, const uint32_t* d_count
 }
 
 hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
-                            uint32_t key_bits, void* storage, hipStream_t stream) {
-  return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream);
+                            uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass) {
+  return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass);
 }
 
 ScanPlan scan_plan(uint32_t max_n) {
