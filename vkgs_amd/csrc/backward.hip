@@ -434,8 +434,12 @@ __global__ __launch_bounds__(256) void diff_bin_gather_kernel(const uint4* __res
   }
 }
 
+// Per-slot runs of pairs up to this length are handled by one lane in the
+// slot fill and the gradient reduction; longer runs by the whole wave.
+constexpr uint32_t kRunSerial = 16;
+
 // pslot[q] = slot for q in soffs[slot] .. soffs[slot] + scnt[slot] - 1, in
-// slot order (so the writes stream): runs of up to kEmitSerial words per lane,
+// slot order (so the writes stream): runs of up to kRunSerial words per lane,
 // longer runs by the whole wave.
 __global__ __launch_bounds__(256) void diff_slot_fill_kernel(const uint32_t* __restrict__ soffs,
                                                              const uint32_t* __restrict__ scnt,
@@ -454,7 +458,7 @@ __global__ __launch_bounds__(256) void diff_slot_fill_kernel(const uint32_t* __r
       c = scnt[k];
       if (o + c > max_pairs) c = o < max_pairs ? (uint32_t)(max_pairs - o) : 0u;
     }
-    const bool big = c > kEmitSerial;
+    const bool big = c > kRunSerial;
     if (!big)
       for (uint32_t i = 0; i < c; ++i) pslot[o + i] = k;
     uint64_t pending = __ballot(big);
@@ -876,7 +880,7 @@ __device__ __forceinline__ float wave_total_last(float x) {
 
 // g9[slot] = sum of the partials of the slot's pairs, which sit at
 // soffs[slot] .. soffs[slot] + scnt[slot] - 1 (contiguous, slots in order, so
-// reads and writes both stream).  A lane sums a run of up to kEmitSerial pairs
+// reads and writes both stream).  A lane sums a run of up to kRunSerial pairs
 // itself (from coalesced LDS-staged chunks); longer runs are summed by the
 // whole wave.  Every visible slot is written once (zeros for a splat without
 // pairs), so g9 needs no zero fill.
@@ -900,8 +904,8 @@ __global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* _
       c = scnt[k];
       if (o + c > max_pairs) c = o < max_pairs ? (uint32_t)(max_pairs - o) : 0u;
     }
-    const bool big = c > kEmitSerial;
-    // runs of up to kEmitSerial pairs: the wave's runs are contiguous, [r0,
+    const bool big = c > kRunSerial;
+    // runs of up to kRunSerial pairs: the wave's runs are contiguous, [r0,
     // r1); it is staged through LDS 64 pairs at a time with coalesced loads
     // (lane i reads words i, i + 64, ...) and each lane adds the part of its
     // run in the chunk.  Chunks that only long runs need are skipped.
