@@ -34,7 +34,8 @@ namespace {
 constexpr float kAlphaMax = 0.999f;
 
 #ifndef VKGS_BWD_BATCH
-#define VKGS_BWD_BATCH 32  // list entries per batch of the one-wave raster backward
+#define VKGS_BWD_BATCH 16  // list entries per batch of the one-wave raster backward; batched fwd+bwd
+                           // views/s against 16: 8 entries -11%, 32 -2%, 64 -17%
 #endif
 
 __device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
