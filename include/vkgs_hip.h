@@ -126,6 +126,24 @@ int vkgs_sort_kv(uint32_t max_n, const uint32_t* d_count, uint32_t* d_key, uint3
                  void* d_storage, size_t storage_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Stage 2 without allocation.  No counterpart in the reference: this is for
+ * callers that cannot allocate device memory or hand over a workspace (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same sort as vkgs_sort_kv, bit for bit, with its scratch in
+ * a static device buffer sized for VKGS_SORT_NOALLOC_MAX_N pairs (the
+ * reference's MAX_SPLAT_COUNT, engine.cc:1653; about 65 MiB on every GPU the
+ * library runs on, held until the process ends).  max_n above that returns
+ * VKGS_ERR_CAPACITY.  No hipMalloc, hipFree or allocating library call; the
+ * first call on a device (with that device current) looks up its buffer's
+ * address.  Every call on one GPU shares that GPU's buffer: calls that
+ * overlap on one GPU (several streams or threads) race, so the caller
+ * serializes them.
+ * ------------------------------------------------------------------------- */
+#define VKGS_SORT_NOALLOC_MAX_N (1u << 23)
+int vkgs_sort_kv_noalloc(uint32_t max_n, const uint32_t* d_count, uint32_t* d_key, uint32_t* d_value,
+                         uint32_t key_bits, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Stage 3: inverse index (inverse_index.comp:55-60 after the -1 fill of
  * engine.cc:1226): d_inverse[d_index[k]] = k for k < *d_count, -1 elsewhere.
  * ------------------------------------------------------------------------- */

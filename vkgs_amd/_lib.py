@@ -89,6 +89,7 @@ SIGNATURES = {
     "vkgs_rank": (_I, [_P, _P, _P, _U32, _P, _P, _P, _P, _SZ, _P]),
     "vkgs_sort_kv_storage_bytes": (_SZ, [_U32]),
     "vkgs_sort_kv": (_I, [_U32, _P, _P, _P, _U32, _P, _SZ, _P]),
+    "vkgs_sort_kv_noalloc": (_I, [_U32, _P, _P, _P, _U32, _P]),
     "vkgs_inverse_index": (_I, [_P, _P, _U32, _P, _P]),
     "vkgs_projection": (_I, [_P, _P, _P, _P, _P, _P, _P, _U32, _P, _P, _P, _P]),
     "vkgs_raster_config_default": (None, [_P]),

@@ -23,7 +23,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libvkgs_hip.so")
 INCLUDE = os.path.join(ROOT, "include")
 
-SOURCES = ["sort.hip", "project.hip", "raster.hip", "backward.hip", "host.cc", "capi.cc", "engine.cc", "diff.cc"]
+SOURCES = ["sort.hip", "project.hip", "raster.hip", "backward.hip", "noalloc.hip", "host.cc", "capi.cc", "engine.cc",
+           "diff.cc"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
