@@ -5,7 +5,7 @@ autograd gradients are pinned against FP64 central finite differences, and the
 HIP backward is compared against them.
 
 It restates, in float64 and vectorised over (splat, pixel):
-  activation      parse_ply.comp:106-156 (scale = exp, normalized quaternion,
+  activation      parse_ply.comp:46-96 (scale = exp, normalized quaternion,
                   cov3d = R diag(s^2) R^T, opacity = sigmoid; SH kept in fp32)
   projection      projection.comp:80-179 (cov2d = P2 J W Sigma W^T J^T P2 with
                   the 1/W^2, 1/H^2 low-pass; eigendecomposition with atan2, as
@@ -91,8 +91,18 @@ def project(means, log_scales, quats, opacity_logits, sh, P, V, eye, width, heig
 
 
 def render(means, log_scales, quats, opacity_logits, sh, P, V, eye, width, height, order, alpha_max=0.999,
-           return_mask=False):
-    """(H, W, 4) image; order = splat ids in draw order (far to near)."""
+           return_mask=False, t_min=0.0, return_edge=False):
+    """(H, W, 4) image; order = splat ids in draw order (far to near).
+
+    t_min: a pixel stops compositing once its transmittance falls below t_min -- a splat
+    contributes only if the transmittance in front of it is >= t_min, and the alpha channel's
+    T term is the transmittance after the last contributing splat (the rule of the GPU raster;
+    0 composites every splat).  The stop is a discrete choice, held constant under autograd.
+
+    return_edge: also return, per pixel (H, W), the smallest distance of max(|u_x|, |u_y|) to the
+    footprint's edge 3 over the drawn splats (in front of the far plane), in eigenframe units: a
+    float32 renderer can decide coverage differently from this float64 one only where it is
+    within rounding of 0."""
     ndc, depth, rs, cov2, color, opacity = project(means, log_scales, quats, opacity_logits, sh, P, V, eye, width,
                                                    height)
     idx = torch.as_tensor(order, dtype=torch.long)
@@ -112,12 +122,24 @@ def render(means, log_scales, quats, opacity_logits, sh, P, V, eye, width, heigh
     # front to back: reverse draw order
     af = alpha.flip(0)
     cf = color.flip(0)
+    if t_min > 0:
+        with torch.no_grad():  # T only decreases, so "T in front >= t_min" holds up to the stop
+            t_front = torch.cumprod(torch.cat([torch.ones_like(af[:1]), 1 - af[:-1]], 0), 0)
+            keep = t_front >= t_min
+        af = torch.where(keep, af, torch.zeros_like(af))
     one_minus = 1 - af
     T = torch.cumprod(torch.cat([torch.ones_like(af[:1]), one_minus[:-1]], 0), 0)  # T in front of each
     wgt = af * T
     rgb = torch.einsum("kp,kc->pc", wgt, cf)
     a_out = (wgt * af).sum(0) + T[-1] * one_minus[-1]
     img = torch.cat([rgb, a_out[:, None]], 1).reshape(height, width, 4)
+    out = (img,)
     if return_mask:
-        return img, cover
-    return img
+        out += (cover,)
+    if return_edge:
+        with torch.no_grad():
+            dist = (u.abs().amax(-1) - 3).abs()
+            dist = torch.where(((depth >= 0) & (depth < 1))[:, None], dist, torch.full_like(dist, math.inf))
+            edge = dist.amin(0) if dist.shape[0] else torch.full((height * width,), math.inf, dtype=dist.dtype)
+        out += (edge.reshape(height, width),)
+    return out[0] if len(out) == 1 else out

@@ -20,7 +20,12 @@ from vkgs_amd import diff as D  # noqa: E402
 from vkgs_amd import pipeline as P  # noqa: E402
 from vkgs_amd import scene as S  # noqa: E402
 
+from _spatial import draw_order, morton_rank  # noqa: E402
+from _stack import held_rgb, stack_closed_form, stacked_scene  # noqa: E402
+
 NAMES = D.PARAM_NAMES
+DEFAULT_T_MIN = 1.0 / 8192.0
+DIFF_T_MIN_FLOOR = 2.0 ** -20  # the differentiable raster stops at max(t_min, 2^-20)
 
 
 def small_scene(n, seed, w, h, r=4.0):
@@ -175,11 +180,12 @@ def test_multi_wave_tiles_match_8x8(tile):
 
 
 def test_pair_overflow_retries_and_matches_a_roomy_context(monkeypatch):
-    """Large splats on a fresh context outgrow its first pair storage: the
-    overflowing forwards stay in bounds, the retried forward and its backward
-    match those of a context that already has room."""
-    W, H = 512, 384
-    sc = S.gaussian_cloud(4000, seed=40, pos_std=1.0, logscale=(-0.7, 0.0))
+    """Metre-sized splats on a fresh context need more than 8x its first pair storage: the
+    overflowing forward stays in bounds and reports the whole pair count, the one retry sizes
+    the storage for it, and the result and its backward match those of a context that already
+    has room."""
+    W, H = 768, 512
+    sc = S.gaussian_cloud(4000, seed=40, pos_std=1.0, logscale=(0.0, 0.0))
     cam = S.orbit_camera(W, H, 4.0)
     w = torch.tensor(np.random.default_rng(4).uniform(0.5, 1.5, (H, W, 4)), dtype=torch.float32, device="cuda")
     monkeypatch.setattr(D, "_pool", D._Pool())  # fresh context: smallest pair storage
@@ -191,7 +197,7 @@ def test_pair_overflow_retries_and_matches_a_roomy_context(monkeypatch):
         runs.append((img.detach().cpu().numpy(), {k: p[k].grad.detach().cpu().numpy() for k in NAMES}))
         pairs = D.last_info["pairs"]
         print("pairs", pairs, flush=True)
-        assert pairs > 4 * 4096 + (1 << 20)  # beyond the first storage of a 4000-splat context
+        assert pairs > 8 * (4 * 4096 + (1 << 20))  # 8x the first storage of a 4000-splat context
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     for k in NAMES:
         assert np.isfinite(runs[0][1][k]).all(), k
@@ -332,3 +338,153 @@ def test_full_batch_workload_equals_single_views():
             den = float(torch.linalg.norm(p[k].grad))
             rel = float(torch.linalg.norm(gb[si][k] - p[k].grad)) / (den if den > 0 else 1.0)
             assert rel < 1e-5, (si, k, rel)
+
+
+def cam_t64(cam):
+    Pm, Vm, eye = S.camera_arrays(cam)
+    return (torch.tensor(Pm.reshape(4, 4).T, dtype=torch.float64),
+            torch.tensor(Vm.reshape(4, 4).T, dtype=torch.float64), torch.tensor(eye, dtype=torch.float64))
+
+
+def params_t64(sc):
+    return [torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, requires_grad=True)
+            for a in (sc.means(), sc.log_scales(), sc.quats_wxyz(), sc.opacity_logits(), sc.sh_channel_major())]
+
+
+@pytest.mark.parametrize("t_min,m_want", [(DEFAULT_T_MIN, 14), (0.0, 21)])
+def test_diff_stop_rule_on_stacked_splats(t_min, m_want):
+    """30 equal splats (alpha 0.5) on the centre pixel: the raster composites exactly the splats
+    whose transmittance in front is >= max(t_min, 2^-20) -- 14 at the default t_min = 2^-13, 21 at
+    t_min = 0 -- as the closed form says (and the float64 reference's stop rule, pinned to it)."""
+    sc = stacked_scene(30, 0.0, (0.8, 0.3, 0.55))
+    cam = S.orbit_camera(33, 33, 4.0)
+    p = D.scene_tensors(sc, requires_grad=False)
+    img = D.render(**p, camera=cam, t_min=t_min).cpu().numpy()
+    want, m = stack_closed_form(0.5, held_rgb(sc), 30, max(t_min, DIFF_T_MIN_FLOOR))
+    assert m == m_want
+    print("centre", img[16, 16], "closed form", want, flush=True)
+    np.testing.assert_allclose(img[16, 16], want, rtol=0, atol=2e-6)
+
+
+def test_diff_gradients_through_saturated_stack():
+    """24 near-opaque splats (alpha 0.9975) stacked on the image centre, composited with t_min = 0:
+    the transmittance falls below 2^-20 after three of them, the raster stops there, and the
+    backward's recovered transmittance stays exact, so the front splats' gradients (and every
+    group's) match float64 autograd within 1e-3 (with no stop, the stored T is 0 and the front
+    splat's gradients vanish)."""
+    rgb = np.random.default_rng(11).uniform(0.1, 0.9, (24, 3))
+    sc = stacked_scene(24, 6.0, rgb, log_scale=(-1.5, -1.8, -1.6), jitter=0.02, seed=12)
+    W = H = 33
+    cam = S.orbit_camera(W, H, 4.0)
+    pos, _, _, _ = oracle.parse_ply(sc.rows, S.standard_offsets())
+    order = draw_order(cam, pos)
+    assert len(order) == 24
+    w = np.random.default_rng(13).uniform(0.5, 1.5, (H, W, 4))
+    p64 = params_t64(sc)
+    ref = torch_ref.render(*p64, *cam_t64(cam), W, H, order, t_min=DIFF_T_MIN_FLOOR)
+    (ref * torch.tensor(w)).sum().backward()
+    p = D.scene_tensors(sc)
+    img = D.render(**p, camera=cam, t_min=0.0)
+    (img * torch.tensor(w, dtype=torch.float32, device="cuda")).sum().backward()
+    d = np.abs(img.detach().cpu().numpy() - ref.detach().numpy()).max()
+    print("stack image max abs", d, flush=True)
+    assert d < 1e-4
+    front = int(order[-1])
+    for name, t64 in zip(NAMES, p64):
+        g = p[name].grad.detach().cpu().double()
+        gr = t64.grad.detach()
+        rel = float(torch.linalg.norm(g - gr) / torch.linalg.norm(gr))
+        rel_front = float(torch.linalg.norm(g[front] - gr[front]) / torch.linalg.norm(gr[front]))
+        print(f"grad {name}: rel err {rel:.2e}, front splat {rel_front:.2e}", flush=True)
+        assert float(torch.linalg.norm(gr[front])) > 0
+        assert rel <= 1e-3 and rel_front <= 1e-3, (name, rel, rel_front)
+
+
+def test_bench_batched_form_matches_float64():
+    """The benchmarked batched fwd+bwd form, small: render_batch_flat with spatial_order=True at the
+    default t_min, five views of one scene (processed in Morton order) and two of a second scene
+    (default order), the weighted L1 loss over the flat output.  Against the float64 reference fed
+    each view's draw order (ascending depth key; equal keys in Morton order for the first scene,
+    ascending id for the second) and the same stop rule: images within 1/255 away from coverage
+    edges and PSNR >= 50 dB, every gradient group within 1e-3."""
+    sa, cam0 = small_scene(1500, 51, 48, 48)
+    sb, _ = small_scene(800, 52, 64, 40)
+    cams_a = [cam0] + [S.orbit_camera(48, 48, 4.0 + 0.3 * i, theta=0.4 * i) for i in range(1, 5)]
+    cams_b = [S.orbit_camera(64, 40, 3.8, phi=1.0), S.orbit_camera(64, 40, 4.4, theta=-0.6)]
+    views = [(0, c) for c in cams_a] + [(1, c) for c in cams_b]
+    scenes = [sa, sb]
+    ranks = [morton_rank(sa.means()), None]  # the five-view scene is the spatially ordered one
+    rng = np.random.default_rng(14)
+    targets = [rng.uniform(0.0, 1.0, (c.screen_size[1], c.screen_size[0], 4)) for _, c in views]
+    weights = [1.0 / (4 * c.screen_size[0] * c.screen_size[1]) for _, c in views]
+
+    params = [D.scene_tensors(s) for s in scenes]
+    flat, spans = D.render_batch_flat(params, views, spatial_order=True)
+    tgt_flat = torch.cat([torch.tensor(t.reshape(-1, 4), dtype=torch.float32, device="cuda") for t in targets])
+    w_flat = torch.cat([torch.full((h * w_, 1), wt, device="cuda") for (_, h, w_), wt in zip(spans, weights)])
+    ((flat - tgt_flat).abs() * w_flat).sum().backward()
+    imgs = [im.detach().cpu().numpy() for im in D.view_images(flat, spans)]
+
+    p64 = [params_t64(s) for s in scenes]
+    pos = [oracle.parse_ply(s.rows, S.standard_offsets())[0] for s in scenes]
+    loss = 0.0
+    for vi, ((si, cam), tgt, wt) in enumerate(zip(views, targets, weights)):
+        W, H = cam.screen_size[0], cam.screen_size[1]
+        order = draw_order(cam, pos[si], ranks[si])
+        ref, edge = torch_ref.render(*p64[si], *cam_t64(cam), W, H, order, t_min=DEFAULT_T_MIN,
+                                     return_edge=True)
+        loss = loss + ((ref - torch.tensor(tgt)).abs() * wt).sum()
+        r = ref.detach().numpy()
+        d = np.abs(imgs[vi] - r)
+        away = edge.numpy() > 1e-4
+        psnr = 10 * np.log10(1.0 / max(float(np.mean(d * d)), 1e-30))
+        print(f"view {vi}: max abs {d[away].max():.2e} (edge pixels {int((~away).sum())}), PSNR {psnr:.1f}",
+              flush=True)
+        assert d[away].max() <= 1.0 / 255.0 and psnr >= 50.0
+        assert (~away).mean() < 0.01
+    loss.backward()
+    for si in range(2):
+        for name, t64 in zip(NAMES, p64[si]):
+            g = params[si][name].grad.detach().cpu().double()
+            gr = t64.grad.detach()
+            rel = float(torch.linalg.norm(g - gr) / torch.linalg.norm(gr))
+            print(f"scene {si} grad {name}: rel err {rel:.2e}", flush=True)
+            assert rel <= 1e-3, (si, name, rel)
+
+
+def test_c4_full_size_fwd_bwd_step():
+    """The differentiation workload at full size: 500k splats, one 1024x1024 view, L1 loss against
+    the image of parameters perturbed by N(0, 0.01) (positions, log-scales) and N(0, 0.1) (SH,
+    opacity logits).  Gradients are finite and nonzero, a two-view batch of the same scene renders
+    each view bit for bit as one-view calls do, and a small gradient step lowers the loss."""
+    import bench
+
+    wl = S.WORKLOADS["c4_500k"]
+    sc = wl.make()
+    cam = S.orbit_camera(wl.width, wl.height, wl.radius)
+    cam2 = S.orbit_camera(wl.width, wl.height, wl.radius * 1.2, theta=1.2)
+    with torch.no_grad():
+        target = D.render(**D.scene_tensors(bench.c4_perturbed(sc), requires_grad=False), camera=cam)
+    p = D.scene_tensors(sc)
+    img = D.render(**p, camera=cam)
+    loss = (img - target).abs().mean()
+    loss.backward()
+    print("c4 visible", D.last_info["visible"], "pairs", D.last_info["pairs"], "loss", float(loss), flush=True)
+    for k in NAMES:
+        assert torch.isfinite(p[k].grad).all(), k
+        assert float(p[k].grad.abs().sum()) > 0, k
+    with torch.no_grad():
+        outs = D.render_batch([p], [(0, cam), (0, cam2)])
+        single2 = D.render(**p, camera=cam2)
+        np.testing.assert_array_equal(outs[0].cpu().numpy(), img.detach().cpu().numpy())
+        np.testing.assert_array_equal(outs[1].cpu().numpy(), single2.cpu().numpy())
+        # steps along -grad moving each group by eta of its norm: a gradient of the wrong sign or
+        # scale raises the loss at every small step
+        base = {k: p[k].detach().clone() for k in NAMES}
+        losses = []
+        for eta in (1e-3, 1e-4, 1e-5):
+            q = {k: base[k] - eta * float(torch.linalg.norm(base[k])) / float(torch.linalg.norm(p[k].grad))
+                 * p[k].grad for k in NAMES}
+            losses.append(float((D.render(**q, camera=cam) - target).abs().mean()))
+    print("c4 loss after steps", losses, flush=True)
+    assert min(losses) < float(loss)

@@ -10,6 +10,8 @@ import oracle  # noqa: E402
 from oracle import torch_ref  # noqa: E402
 from vkgs_amd import scene as S  # noqa: E402
 
+from _stack import held_rgb, stack_closed_form, stacked_scene  # noqa: E402
+
 
 def small_scene(n, seed, w, h, r=4.0):
     sc = S.gaussian_cloud(n, seed=seed, pos_std=1.0, logscale=(-2.5, 0.4))
@@ -93,3 +95,18 @@ def test_torch_ref_autograd_matches_central_differences():
                 assert abs(fd - an) <= 1e-5 + 1e-4 * abs(an), (gi, j, fd, an)
                 checked += 1
     assert checked >= 40
+
+
+@pytest.mark.parametrize("t_min,m_want", [(0.0, 30), (1.0 / 8192.0, 14), (2.0 ** -20, 21)])
+def test_torch_ref_t_min_stop_matches_closed_form(t_min, m_want):
+    # alpha 0.5 (logit 0): T_j = 2^-j hits t_min = 2^-13 and 2^-20 exactly, so the ">=" of the
+    # stop rule is exercised at equality
+    rgb = (0.8, 0.3, 0.55)
+    sc = stacked_scene(30, 0.0, rgb)
+    cam = S.orbit_camera(33, 33, 4.0)  # odd size: the origin projects onto the centre pixel's centre
+    order = np.arange(30)
+    P, V, eye = cam64(cam)
+    img = torch_ref.render(*params64(sc), P, V, eye, 33, 33, order, t_min=t_min).numpy()
+    want, m = stack_closed_form(0.5, held_rgb(sc), 30, t_min)
+    assert m == m_want
+    np.testing.assert_allclose(img[16, 16], want, rtol=0, atol=1e-9)

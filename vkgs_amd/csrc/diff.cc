@@ -81,12 +81,20 @@ struct vkgs_diff_ctx {
   ScanPlan scan{};
   bool have_forward = false;
   bool grow_pairs = false;
+  uint64_t need_pairs = 0;  // pair count of the last overflowing forward
 };
 
 namespace {
 
 enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscSlotPairs = 3 };
 constexpr uint32_t kPermRefresh = 32;
+// Smallest stop threshold of the differentiable raster: the backward recovers
+// the transmittance in front of each entry as T_i = T_{i+1} / (1 - alpha_i)
+// from the stored final T, which stays >= kDiffTMinFloor * (1 - kAlphaMax), a
+// normal float, only if compositing stops there (with t_min = 0, 16 saturated
+// splats take T to exactly 0 and every recovered T with it).  The omitted
+// contribution is below 2^-20 of the pixel's colour.
+constexpr float kDiffTMinFloor = 1.0f / 1048576.0f;
 constexpr uint32_t kSpatialViews = 4;  // views of a scene in one call for the spatial order to pay
 
 template <typename T>
@@ -236,7 +244,12 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   hipError_t e = hipSuccess;
   if (c->max_pairs == 0 || c->grow_pairs || c->max_pairs < (uint64_t)cap) {
     const uint64_t want = (uint64_t)cap * 4 + (1u << 20);
-    c->max_pairs = c->grow_pairs ? c->max_pairs * 2 : (want > c->max_pairs ? want : c->max_pairs);
+    if (c->grow_pairs) {  // room for the overflowing forward's whole list, plus an eighth
+      const uint64_t need = c->need_pairs + c->need_pairs / 8 + (1u << 20);
+      c->max_pairs = need > c->max_pairs * 2 ? need : c->max_pairs * 2;
+    } else {
+      c->max_pairs = want > c->max_pairs ? want : c->max_pairs;
+    }
     if (c->max_pairs > 0xffffffffull) c->max_pairs = 0xffffffffull;
     c->grow_pairs = false;
   }
@@ -334,7 +347,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   L.tile_h = rs0.tile_h;
   L.tile_shift_x = rs0.tile_shift_x;
   L.tile_shift_y = rs0.tile_shift_y;
-  L.t_min = cfg->t_min;
+  L.t_min = cfg->t_min > kDiffTMinFloor ? cfg->t_min : kDiffTMinFloor;
   c->cap_items = cap;
 
   // ---- processing order (when due) ----
@@ -460,6 +473,7 @@ int vkgs_diff_info(vkgs_diff_ctx* c, uint32_t* visible, uint64_t* pairs, void* s
   if (pairs) *pairs = m[kMiscPairs];
   if (m[kMiscStatus]) {
     c->grow_pairs = true;
+    c->need_pairs = m[kMiscPairs];
     return VKGS_ERR_OVERFLOW;
   }
   return VKGS_OK;

@@ -162,7 +162,9 @@ class _RenderBatch(torch.autograd.Function):
         c = _pool.get()
         sarr = _scene_structs(groups)
         varr = _view_structs(views, outs)
-        for _attempt in range(4):
+        # an overflowing forward reports its whole pair count, and the next forward sizes the pair
+        # storage for it, so one retry suffices; a result is never returned from a truncated list
+        for attempt in range(2):
             check(_L().vkgs_diff_forward(c.h, sarr, n_scenes, varr, len(views), C.byref(cfg), _stream()),
                   "vkgs_diff_forward")
             vis, pairs = C.c_uint32(), C.c_uint64()
@@ -170,7 +172,8 @@ class _RenderBatch(torch.autograd.Function):
             if st == 0:
                 last_info.update(visible=int(vis.value), pairs=int(pairs.value), views=len(views))
                 break
-            if st != 6:  # VKGS_ERR_OVERFLOW: the next forward grows the pair storage
+            if st != 6 or attempt == 1:  # VKGS_ERR_OVERFLOW twice: raise, never a truncated image
+                _pool.put(c)
                 check(st, "vkgs_diff_info")
         ctx.c = c
         ctx.meta = meta
