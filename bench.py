@@ -27,10 +27,9 @@ if HERE not in sys.path:
 METRIC = "FPS @6M splats 1600x900; batched vs plain fwd+bwd views/s; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# per (pixel, splat) evaluation, an FMA counted as 2 flops (as the peak counts it) and exp2 as 1:
-# dx, dy 2; the two rows of the inverse rot_scale 3 + 3; q 3; falloff 2; alpha 1; T * alpha 1;
-# colour and alpha FMAs 4 x 2; T update 1
-RASTER_FLOPS_PER_EVAL = 24
+# FP32 operations per (pixel, splat) evaluation of the reference fragment and blend, exp counted as
+# 1: eigenframe coordinates, falloff, alpha, and the blend of colour and alpha
+RASTER_FLOPS_PER_EVAL = 20
 
 
 def parse_args(argv=None):
@@ -52,6 +51,10 @@ def parse_args(argv=None):
                     help="also run one device-to-device copy of MIB MiB (a known-size kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
     return ap.parse_args(argv)
+
+
+TRAFFIC_NOTE = ("HBM bytes per launch from the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary "
+                "profiles/traffic_<workload>.json (not measured in this run; its header names the commit)")
 
 
 def load_traffic(kernel: str, workload: str, tile: str):
@@ -95,6 +98,54 @@ def batch_workload(scale: float = 1.0, seed: int = 5):
         for _ in range(2):
             views.append((1 + i, cam(*res[rng.integers(0, 4)])))
     return scenes, views
+
+
+def c4_perturbed(sc, seed: int = 9):
+    """Target parameters of the differentiation workload: positions and log-scales perturbed by
+    N(0, 0.01), SH and opacity logits by N(0, 0.1)."""
+    import numpy as np
+    from vkgs_amd import scene as S
+
+    rng = np.random.default_rng(seed)
+    rows = sc.rows.copy()
+    c = S.COL
+    n = rows.shape[0]
+    rows[:, 0:3] += rng.normal(0.0, 0.01, (n, 3)).astype(np.float32)
+    rows[:, c["scale_0"]:c["scale_0"] + 3] += rng.normal(0.0, 0.01, (n, 3)).astype(np.float32)
+    rows[:, c["opacity"]] += rng.normal(0.0, 0.1, n).astype(np.float32)
+    rows[:, c["f_dc_0"]:c["f_rest_0"] + 45] += rng.normal(0.0, 0.1, (n, 48)).astype(np.float32)
+    return S.Scene(rows)
+
+
+def bench_c4(steps: int):
+    """ms/view of fwd+bwd on the differentiation workload: 500k splats, one 1024x1024 view, L1 loss
+    against the image of the perturbed parameters (one view per launch set)."""
+    import torch
+    from vkgs_amd import diff as D
+    from vkgs_amd import scene as S
+
+    wl = S.WORKLOADS["c4_500k"]
+    sc = wl.make()
+    cam = S.orbit_camera(wl.width, wl.height, wl.radius)
+    params = D.scene_tensors(sc)
+    with torch.no_grad():
+        target = D.render(**D.scene_tensors(c4_perturbed(sc), requires_grad=False), camera=cam)
+
+    def step():
+        for t in params.values():
+            t.grad = None
+        img = D.render(**params, camera=cam)
+        (img - target).abs().mean().backward()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {"c4_ms_per_view": round(dt * 1e3, 3), "c4_views_per_s": round(1.0 / dt, 2),
+            "c4_visible": D.last_info["visible"], "c4_pairs": D.last_info["pairs"]}
 
 
 def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
@@ -237,6 +288,7 @@ def main(argv=None) -> int:
         for k, v in r.stage_ms().items():
             if k in stages:
                 stages[k].append(v)
+    wave_entries = r.raster_work()  # of the last timed frame: the same camera, so every frame's
     r.set_timing(False)
     stage_ms = {k: round(median(v), 5) for k, v in stages.items()}
     log("stage ms", stage_ms)
@@ -259,11 +311,12 @@ def main(argv=None) -> int:
     tx = np.arange(ranges.shape[0]) % tiles_x
     ty = np.arange(ranges.shape[0]) // tiles_x
     pix = (np.minimum(tw, wl.width - tx * tw) * np.minimum(th, wl.height - ty * th)).astype(np.float64)
-    p_eval = float((pix * lens).sum())
+    p_eval = float((pix * lens).sum())  # whole tile lists
+    p_exec = 64.0 * wave_entries  # evaluations the raster executed (lists up to each tile's early exit)
     n, V, K = sc.n, visible, pairs
     b_proj = 12.0 * n + 172.0 * V  # positions of all + cov3d/opacity/SH read + 48-B record per visible
     t_proj = stage_ms["project"] * 1e-3
-    f_raster = RASTER_FLOPS_PER_EVAL * p_eval
+    f_raster = RASTER_FLOPS_PER_EVAL * p_exec
     t_raster = stage_ms["raster"] * 1e-3
     proj_gbs = b_proj / t_proj / 1e9
     raster_tflops = f_raster / t_raster / 1e12
@@ -286,7 +339,9 @@ def main(argv=None) -> int:
         roof = {"kernel": raster_name, "bound": "valu", "achieved": round(raster_tflops, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(raster_tflops / FP32_PEAK_TFLOPS, 4),
                 "traffic": load_traffic(raster_name, wl.name, f"{tw}x{th}"), "avg_ms": stage_ms["raster"],
-                "work": f"{RASTER_FLOPS_PER_EVAL} flop x {p_eval:.4g} (pixel, splat) evaluations"}
+                "work": f"{RASTER_FLOPS_PER_EVAL} FP32 ops (exp = 1) x {p_exec:.4g} executed (pixel, splat) "
+                        f"evaluations (64 x {wave_entries} wave steps; whole lists: {p_eval:.4g})",
+                "traffic_from": TRAFFIC_NOTE}
     else:
         roof = {"kernel": "fused_write_kernel", "bound": "hbm", "achieved": round(proj_gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4),
@@ -295,7 +350,7 @@ def main(argv=None) -> int:
     roof_proj = {"kernel": "fused_write_kernel", "bound": "hbm", "achieved": round(proj_gbs, 1),
                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(proj_gbs / HBM_PEAK_GBS, 4),
                  "traffic": load_traffic("fused_write_kernel", wl.name, f"{tw}x{th}"), "avg_ms": stage_ms["project"],
-                 "work": f"12N + 172V = {b_proj:.4g} B"}
+                 "work": f"12N + 172V = {b_proj:.4g} B", "traffic_from": TRAFFIC_NOTE}
 
     result = {
         "metric": METRIC,
@@ -338,6 +393,8 @@ def main(argv=None) -> int:
     if args.fwd_bwd_steps > 0:
         try:
             result["fwd_bwd"] = bench_fwd_bwd(args.fwd_bwd_steps, args.batch_scale, args.fwd_bwd_mode)
+            if args.fwd_bwd_mode == "both":
+                result["fwd_bwd"].update(bench_c4(max(args.fwd_bwd_steps, 3)))
             log("fwd+bwd", result["fwd_bwd"])
         except Exception as e:  # reported, never hides the frame figure
             result["fwd_bwd"] = {"error": repr(e)}

@@ -93,7 +93,7 @@ uint64_t vkgs_ply_buffer_bytes(const vkgs_ply_info* info);
 int vkgs_ply_read(const char* path, const vkgs_ply_info* info, void* dst, uint64_t dst_bytes);
 
 /* ---------------------------------------------------------------------------
- * Stage 0: activation + layout (parse_ply.comp:94-157, dispatch
+ * Stage 0: activation + layout (parse_ply.comp:34-98, dispatch
  * engine.cc:1137-1161).  d_ply is the device copy of the ply buffer above.
  * Writes the vkgs splat storage (engine.cc:485-492):
  *   position f32 (N,3), cov3d f32 (N,6) = (xx,xy,xz,yy,yz,zz), opacity f32 (N),
@@ -144,7 +144,7 @@ int vkgs_sort_kv_noalloc(uint32_t max_n, const uint32_t* d_count, uint32_t* d_ke
                          uint32_t key_bits, void* stream);
 
 /* ---------------------------------------------------------------------------
- * Stage 3: inverse index (inverse_index.comp:55-60 after the -1 fill of
+ * Stage 3: inverse index (inverse_index.comp:13-18 after the -1 fill of
  * engine.cc:1226): d_inverse[d_index[k]] = k for k < *d_count, -1 elsewhere.
  * ------------------------------------------------------------------------- */
 int vkgs_inverse_index(const uint32_t* d_count, const uint32_t* d_index, uint32_t n, int32_t* d_inverse,
@@ -260,19 +260,28 @@ int vkgs_renderer_set_timing(vkgs_renderer* r, int enable);
 /* ms of the last timed frame: [visible count + scan, project + compact,
  * depth sort, bin, tile sort + ranges, raster, total, 0] */
 int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[8]);
+/* Raster work of the last timed frame (timing on): the list entries the
+ * raster's waves composited, summed over waves; each is one 64-lane
+ * (pixel, splat) evaluation step, so executed evaluations = 64 x this.
+ * Entries a pixel skipped after its early exit are not counted.
+ * Synchronous. */
+int vkgs_renderer_raster_work(vkgs_renderer* r, uint64_t* wave_entries);
 
 /* ---------------------------------------------------------------------------
  * Differentiable, batched path (no reference counterpart; the reference only
  * renders).  Parameters are the raw 3DGS ones in the PLY's conventions:
  * means (N,3), log-scales (N,3) (scale_0..2), quaternions (N,4) in PLY order
  * (rot_0..rot_3 = w,x,y,z), opacity logits (N), SH (N,48) in the vkgs
- * channel-major order; activation as parse_ply.comp:106-156 (SH kept in fp32),
+ * channel-major order; activation as parse_ply.comp:46-96 (SH kept in fp32),
  * projection as projection.comp:80-179, blend as engine.cc:279-299 with alpha
  * clamped to 0.999.  Many views of many scenes render in one set of launches:
  * tiles of all views share one id space, so the tile sort key carries the view
  * in its high bits.  A context keeps one forward's state until its backward,
  * which re-walks each tile list back to front from the stored per-pixel
- * transmittance and processed-entry count.
+ * transmittance and processed-entry count.  The raster stops a pixel once its
+ * transmittance is below max(cfg->t_min, 2^-20), so the stored transmittance
+ * stays a normal float (t_min = 0 therefore omits at most 2^-20 of a pixel's
+ * colour on this path).
  * ------------------------------------------------------------------------- */
 /* sh and grad_sh must be 16-byte aligned (VKGS_ERR_INVALID_ARG otherwise). */
 typedef struct vkgs_diff_scene {
@@ -308,7 +317,8 @@ int vkgs_diff_forward(vkgs_diff_ctx* ctx, const vkgs_diff_scene* scenes, uint32_
 int vkgs_diff_backward(vkgs_diff_ctx* ctx, const vkgs_diff_scene* scenes, uint32_t n_scenes,
                        const vkgs_diff_view* views, uint32_t n_views, void* stream);
 /* Synchronous: visible items and pairs of the last forward; VKGS_ERR_OVERFLOW
- * if its pair list overflowed (the next forward doubles the pair storage). */
+ * if its pair list overflowed, in which case *pairs is the whole list's length
+ * and the next forward sizes the pair storage for it (a retry then fits). */
 int vkgs_diff_info(vkgs_diff_ctx* ctx, uint32_t* visible, uint64_t* pairs, void* stream);
 
 #ifdef __cplusplus

@@ -13,7 +13,7 @@
  *  - A product C = A*B sums k = 0..R-1 left to right: ((a0*b0 + a1*b1) + a2*b2) ...
  *  - GLSL expressions are evaluated left-associatively as written in the shader.
  *
- * Parity status (what pins each function; tests/test_oracle_*.py):
+ * Parity status (what pins each function; tests/test_oracle.py, tests/test_golden.py):
  *  - oracle_parse_ply: rotation vs scipy Rotation.from_quat, closed forms for
  *    identity quaternions, fp16 against numpy float16, sigmoid closed form.
  *  - oracle_rank: NDC of the camera frustum corners/centre (closed forms),
@@ -185,13 +185,13 @@ void oracle_parse_ply(const float* ply, const uint32_t offsets[60], uint32_t n, 
   for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
     const uint32_t id = (uint32_t)ii;
     const float* row = ply + (size_t)stride * id;
-    /* parse_ply.comp:106-108 */
+    /* parse_ply.comp:46-48 */
     float s[3] = {expf(row[offsets[3]]), expf(row[offsets[4]]), expf(row[offsets[5]])};
-    /* :110-112, q = (rot_1, rot_2, rot_3, rot_0) = (x, y, z, w) */
+    /* :50-52, q = (rot_1, rot_2, rot_3, rot_0) = (x, y, z, w) */
     float q[4] = {row[offsets[6]], row[offsets[7]], row[offsets[8]], row[offsets[9]]};
     float len = sqrtf(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
     for (int k = 0; k < 4; ++k) q[k] = q[k] / len;
-    /* :114-132 */
+    /* :54-72 */
     float xx = q[0] * q[0], yy = q[1] * q[1], zz = q[2] * q[2];
     float xy = q[0] * q[1], xz = q[0] * q[2], yz = q[1] * q[2];
     float wx = q[3] * q[0], wy = q[3] * q[1], wz = q[3] * q[2];
@@ -205,7 +205,7 @@ void oracle_parse_ply(const float* ply, const uint32_t offsets[60], uint32_t n, 
     rot[2 * 3 + 0] = 2.f * (xz + wy);
     rot[2 * 3 + 1] = 2.f * (yz - wx);
     rot[2 * 3 + 2] = 1.f - 2.f * (xx + yy);
-    /* :134-138, cov3d = rot * ss * transpose(rot) */
+    /* :74-78, cov3d = rot * ss * transpose(rot) */
     float ss[9] = {0};
     ss[0] = s[0] * s[0];
     ss[4] = s[1] * s[1];
@@ -214,7 +214,7 @@ void oracle_parse_ply(const float* ply, const uint32_t offsets[60], uint32_t n, 
     m3_mul(rot, ss, rs);
     m3_transpose(rot, rt);
     m3_mul(rs, rt, cov);
-    /* :140-145 */
+    /* :80-85 */
     float* c = cov3d + 6 * (size_t)id;
     c[0] = cov[0 * 3 + 0];
     c[1] = cov[1 * 3 + 0];
@@ -222,13 +222,13 @@ void oracle_parse_ply(const float* ply, const uint32_t offsets[60], uint32_t n, 
     c[3] = cov[1 * 3 + 1];
     c[4] = cov[2 * 3 + 1];
     c[5] = cov[2 * 3 + 2];
-    /* :147-149 */
+    /* :87-89 */
     position[3 * (size_t)id + 0] = row[offsets[0]];
     position[3 * (size_t)id + 1] = row[offsets[1]];
     position[3 * (size_t)id + 2] = row[offsets[2]];
-    /* :151-154 */
+    /* :91-94 */
     for (int i = 0; i < 48; ++i) sh[48 * (size_t)id + i] = oracle_f32_to_f16(row[offsets[10 + i]]);
-    /* :90, :156 */
+    /* :30, :96 */
     opacity[id] = 1.f / (1.f + expf(-row[offsets[58]]));
   }
 }
@@ -502,15 +502,30 @@ int oracle_setup_splat(const float* inst, uint32_t width, uint32_t height, oracl
   return 1;
 }
 
+/* Coverage of pixel (px, py) by the +-3 sigma square (splat.vert:21-25) in the
+ * eigenframe scaled by ORACLE_FRAME_SCALE = sqrt(log2(e) / 2), u' = s u, so
+ * the square is max(|u'_x|, |u'_y|) <= fl(3 s) and exp(-|u|^2 / 2) =
+ * exp2(-|u'|^2).  u' is formed from the corner (X0, Y0) of the pixel's 64 x 64
+ * block with the operations of the HIP raster, so both decide coverage
+ * identically:
+ *   i' = s * (row of the inverse rot_scale), o' = fma(i'0, X0 - cx, i'1 * (Y0 - cy)),
+ *   u' = fma(i'0, px + 0.5 - X0, fma(i'1, py + 0.5 - Y0, o')). */
+#define ORACLE_FRAME_SCALE 0.849321800288019f
+#define ORACLE_COVER_SCALED (3.0f * ORACLE_FRAME_SCALE)
+
 int oracle_eval_pixel(const oracle_splat2d* s, int px, int py, float* alpha) {
-  const float dx = ((float)px + 0.5f) - s->cx;
-  const float dy = ((float)py + 0.5f) - s->cy;
-  const float ux = fmaf(s->i00, dx, s->i01 * dy);
-  const float uy = fmaf(s->i10, dx, s->i11 * dy);
-  if (!(fabsf(ux) <= 3.f && fabsf(uy) <= 3.f)) return 0;
-  const float q = fmaf(ux, ux, uy * uy);
-  /* splat.frag:9-10 */
-  *alpha = s->o * expf(-0.5f * q);
+  const int bx = px & ~63, by = py & ~63;
+  const float X0 = (float)bx, Y0 = (float)by;
+  const float i00 = s->i00 * ORACLE_FRAME_SCALE, i01 = s->i01 * ORACLE_FRAME_SCALE;
+  const float i10 = s->i10 * ORACLE_FRAME_SCALE, i11 = s->i11 * ORACLE_FRAME_SCALE;
+  const float dx = X0 - s->cx, dy = Y0 - s->cy;
+  const float ox = fmaf(i00, dx, i01 * dy), oy = fmaf(i10, dx, i11 * dy);
+  const float lx = (float)(px - bx) + 0.5f, ly = (float)(py - by) + 0.5f;
+  const float ux = fmaf(i00, lx, fmaf(i01, ly, ox));
+  const float uy = fmaf(i10, lx, fmaf(i11, ly, oy));
+  if (!(fmaxf(fabsf(ux), fabsf(uy)) <= ORACLE_COVER_SCALED)) return 0;
+  /* splat.frag:9-10: alpha = opacity * exp(-|u|^2 / 2) */
+  *alpha = s->o * exp2f(-fmaf(ux, ux, uy * uy));
   return 1;
 }
 

@@ -12,11 +12,11 @@
  * keys, visible counts, tile ranges and sorted order bit-exact between the two.
  *
  * Reference stages restated (file:line in jaesung-cs/vkgs):
- *   oracle_parse_ply    src/shader/parse_ply.comp:94-157
+ *   oracle_parse_ply    src/shader/parse_ply.comp:34-98
  *   oracle_rank         src/shader/rank.comp:27-42
  *   oracle_sort_kv      vrdxCmdSortKeyValueIndirect (engine.cc:1218-1219): stable
  *                       ascending sort of (u32 key, u32 value)
- *   oracle_inverse      src/shader/inverse_index.comp:55-60 + fill -1 (engine.cc:1226)
+ *   oracle_inverse      src/shader/inverse_index.comp:13-18 + fill -1 (engine.cc:1226)
  *   oracle_projection   src/shader/projection.comp:60-180
  *   oracle_draw         splat.vert:10-26 + splat.frag:8-12 + blend state
  *                       engine.cc:279-299 + clear engine.cc:1382-1387
@@ -24,7 +24,7 @@
  *                       the reference uses the hardware rasterizer)
  *
  * Parity status of each function is recorded in the header comment of
- * vkgs_oracle.c and pinned by tests/test_oracle_*.py.
+ * vkgs_oracle.c and pinned by tests/test_oracle.py and tests/test_golden.py.
  */
 #ifndef VKGS_ORACLE_H
 #define VKGS_ORACLE_H

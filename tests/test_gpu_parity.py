@@ -325,13 +325,27 @@ def test_stage_and_fused_paths_match_golden_fixture():
     r.close()
 
 
-def test_renderer_full_size_matches_oracle():
-    """The 6M-splat 1600x900 workload the frame rate is quoted on (5.4M visible, 10.1M pairs): visible
-    count, sorted keys and ids, and every tile's list bit-exact against the oracle at full size; the
-    image within 1/255 and PSNR >= 50 dB of the oracle's back-to-front composite."""
-    wl = S.WORKLOADS["c3_6m"]
-    sc = wl.make()
-    cam = S.orbit_camera(wl.width, wl.height, wl.radius)
+_SCENES = {}
+
+
+def _workload_scene(name):
+    if name not in _SCENES:
+        _SCENES.clear()  # one full-size scene held at a time
+        _SCENES[name] = S.WORKLOADS[name].make()
+    return _SCENES[name]
+
+
+@pytest.mark.parametrize("name,width,height", [("c2_1m", 1600, 900), ("c3_6m", 1920, 1080), ("c3_6m", 1600, 900)])
+def test_renderer_full_size_matches_oracle(name, width, height):
+    """Full-size plain forward workloads: the 1M-splat outdoor scene (64 clusters and a ground plane,
+    orbit r = 15 m) at 1600x900, and the 6M-splat scene the frame rate is quoted on at 1920x1080 and
+    1600x900 (5.4M visible, 10.1M pairs): visible count, sorted keys and ids, and every tile's list
+    bit-exact against the oracle; the image within 1/255 and PSNR >= 50 dB of the oracle's
+    back-to-front composite."""
+    wl = S.WORKLOADS[name]
+    sc = _workload_scene(name)
+    cam = S.orbit_camera(width, height, wl.radius)
+    print("full size", name, width, height, flush=True)
     r = P.Renderer(sc.n)
     r.load_buffer(sc.ply_buffer(), sc.n)
     for _ in range(3):  # the first frame may grow the pair storage
@@ -358,12 +372,12 @@ def test_renderer_full_size_matches_oracle():
     np.testing.assert_array_equal(ids, sidx)
     inv = oracle.inverse(sidx, sc.n)
     inst, _ = oracle.projection(cam, pos, cov, opa, sh, inv, ov)
-    oranges, olst = oracle.bin_tiles(inst, wl.width, wl.height, *DEFAULT_TILE)
+    oranges, olst = oracle.bin_tiles(inst, width, height, *DEFAULT_TILE)
     assert k == olst.size
     print("empty tiles", int((oranges[:, 1] == oranges[:, 0]).sum()), flush=True)
     np.testing.assert_array_equal(ranges, canonical_ranges(oranges))
     np.testing.assert_array_equal(lst_ids, sidx[olst])
-    ref_img = oracle.draw(inst, wl.width, wl.height)
+    ref_img = oracle.draw(inst, width, height)
     diff = np.abs(img - ref_img).max()
     print("full size image max abs diff", diff, "psnr", psnr(img, ref_img), flush=True)
     assert diff <= 1.0 / 255.0

@@ -292,8 +292,13 @@ def test_draw_single_splat_closed_form():
     np.testing.assert_allclose(img[16, 17, 0], 0.2 * a1, rtol=1e-6)
     # outside the +-3 sigma square (7 px away): untouched
     assert img[16, 23, 3] == 1.0 and img[16, 23, 0] == 0.0
-    # the square is 3 sigma = 6 px: pixel 16+6 is on the edge (covered), 16+7 is not
-    assert img[16, 22, 0] > 0
+    # the square's edge at 3 sigma: with sigma = 2.05 px pixel 16+6 (u = 2.93) is covered, 16+7
+    # (u = 3.41) is not (a pixel centre exactly on the edge is a tie the reference settles with its
+    # rasterizer's fill rule; the analytic test decides it by the rounding of the scaled frame)
+    s2 = 2.0 / W * 2.05
+    img2 = oracle.draw(_inst(cx, cx, [[s2, 0], [0, s2]], (0.2, 0.4, 0.8), 0.7)[None], W, H)
+    assert img2[16, 22, 0] > 0 and img2[22, 16, 0] > 0 and img2[16, 10, 0] > 0
+    assert img2[16, 23, 3] == 1.0 and img2[23, 16, 3] == 1.0 and img2[16, 9, 3] == 1.0
     # the footprint is a square in the eigenframe (splat.vert:17-25), not a disc:
     # p = (2.5, 2.5) has |p| > 3 and is still covered
     a_corner = 0.7 * math.exp(-0.5 * 12.5)

@@ -110,6 +110,7 @@ SIGNATURES = {
     "vkgs_renderer_tile_lists": (_I, [_P, _P, _P, _P, _P]),
     "vkgs_renderer_set_timing": (_I, [_P, _I]),
     "vkgs_renderer_stage_ms": (_I, [_P, _P]),
+    "vkgs_renderer_raster_work": (_I, [_P, _P]),
     "vkgs_renderer_autotune": (_I, [_P, _P, _P, _P, _U32, _P, _P, _P]),
     "vkgs_diff_create": (_I, [_P]),
     "vkgs_diff_destroy": (_I, [_P]),

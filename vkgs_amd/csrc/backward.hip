@@ -1,6 +1,6 @@
 // Differentiable, batched splat path for gfx950 (forward and backward).
 //
-// Forward: per (view, splat) item the activation of parse_ply.comp:106-156
+// Forward: per (view, splat) item the activation of parse_ply.comp:46-96
 // (fp32 SH, no fp16 rounding) and the projection of projection.comp:80-179,
 // ordered compaction with the raster record written at the item's slot, depth
 // sort, binning into global tiles, tile sort, and a raster that also stores,
@@ -43,7 +43,7 @@ __device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo
 // ------------------------------------------------------ forward projection ---
 
 // Rotation (row-major) of the normalized quaternion q = (x, y, z, w), as built
-// by parse_ply.comp:114-132 (the shader stores it column-major).
+// by parse_ply.comp:54-72 (the shader stores it column-major).
 __device__ __forceinline__ void quat_rot(const float q[4], float R[9]) {
   const float x = q[0], y = q[1], z = q[2], w = q[3];
   const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y,
@@ -71,7 +71,7 @@ __device__ __forceinline__ float load_quat(const DiffScene& sc, uint32_t id, flo
   return len;
 }
 
-// cov3d in the shader's column-major layout (parse_ply.comp:106-138).
+// cov3d in the shader's column-major layout (parse_ply.comp:46-78).
 __device__ __forceinline__ void activate_cov(const DiffScene& sc, uint32_t id, float S[9]) {
   float q[4];
   load_quat(sc, id, q);
@@ -113,7 +113,7 @@ __device__ __forceinline__ void sh_basis(float x, float y, float z, float b[16])
 }
 
 // View-independent inputs of the projection of one splat: mean, activated
-// cov3d (parse_ply.comp:106-138, column-major), fp32 SH and opacity.
+// cov3d (parse_ply.comp:46-78, column-major), fp32 SH and opacity.
 struct SplatInputs {
   float p[3];
   float S[9];

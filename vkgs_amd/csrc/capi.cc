@@ -110,7 +110,7 @@ static hipError_t raster_scheduled(const DrawBuffers& b, const RasterShape& rs, 
     if (e != hipSuccess) return e;
   }
   if (mark) (void)hipEventRecord(mark, s);
-  return launch_rasterize(b.rec, b.vals, b.ranges, rs, b.tile_order, out, s);
+  return launch_rasterize(b.rec, b.vals, b.ranges, rs, b.tile_order, out, s, b.work);
 }
 
 // Shared by vkgs_draw and the renderer: bin, sort by tile, ranges, raster.

@@ -147,6 +147,22 @@ VKHD TileRect unpack_rect(uint32_t p, RectBits rb, uint32_t* count) {
   return r;
 }
 
+// The raster tests coverage and evaluates the falloff in the eigenframe
+// scaled by kFrameScale = sqrt(log2(e) / 2): with u' = kFrameScale * u the
+// reference's alpha = opacity * exp(-|u|^2 / 2) (splat.frag:9-10) is
+// exp2(log2(opacity) - |u'|^2), and the square |u_x|, |u_y| <= 3 of the quad
+// (splat.vert:21-25) is max(|u'_x|, |u'_y|) <= kCoverScaled.  u' of a pixel is
+// formed from the corner (X0, Y0) of its 64 x 64 pixel block:
+//   i' = kFrameScale * (row of the inverse rot_scale),
+//   o' = fma(i'_0, X0 - cx, i'_1 * (Y0 - cy)),
+//   u' = fma(i'_0, px + 0.5 - X0, fma(i'_1, py + 0.5 - Y0, o')),
+// so that per pixel it is two FMAs, the terms stay below 64 |i'| in magnitude,
+// and the result does not depend on the tile shape (every tile lies in one
+// block).  The oracle evaluates the same operations.
+constexpr float kFrameScale = 0.849321800288019f;
+constexpr float kCoverScaled = 3.0f * kFrameScale;
+constexpr int kRasterBlock = 64;
+
 // Raster set-up of one instance (splat.vert:10-26 restated in framebuffer
 // pixels): centre, inverse of the pixel-space rot_scale, colour, and the
 // conservative pixel rectangle of the +-3 sigma quad.  Returns false when the

@@ -3,7 +3,7 @@
 // Parameters are the raw 3DGS ones (fp32): means (N,3), log-scales (N,3),
 // quaternions (N,4) in PLY order (w,x,y,z) = (rot_0..rot_3), opacity logits
 // (N), SH (N,48) in the vkgs channel-major order.  Activation follows
-// parse_ply.comp:106-156 (without the fp16 rounding of the SH), projection
+// parse_ply.comp:46-96 (without the fp16 rounding of the SH), projection
 // follows projection.comp:80-179 and the raster the reference's blend.
 //
 // Every (view, splat) pair is one item; items are grouped per view in blocks

@@ -67,6 +67,7 @@ struct vkgs_renderer {
   DevBuf rec, rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges, tile_keys, tile_order, tile_sort;
   SortPlan tile_plan{};
   DevBuf sorted_ids, list_ids;
+  DevBuf work;  // list entries composited per raster wave (timed frames)
   SortPlan depth_plan{}, pair_plan{};
   ScanPlan scan{};
   RasterShape rs{};
@@ -83,6 +84,7 @@ struct vkgs_renderer {
   } graph_key{};
   bool graph_valid = false;
   bool grow_pairs = false;
+  uint64_t need_pairs = 0;  // pair count of the last overflowing frame
 
   bool timing = false;
   hipEvent_t ev[kEvents] = {};
@@ -105,8 +107,9 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   RasterShape rs;
   if (!make_raster_shape(width, height, &r->cfg, &rs)) return hipErrorInvalidValue;
   if (r->max_pairs == 0) r->max_pairs = (uint64_t)(n ? n : 1) * 4 + (1u << 20);
-  if (r->grow_pairs) {
-    r->max_pairs *= 2;
+  if (r->grow_pairs) {  // room for the overflowing frame's whole list, plus an eighth
+    const uint64_t need = r->need_pairs + r->need_pairs / 8 + (1u << 20);
+    r->max_pairs = need > r->max_pairs * 2 ? need : r->max_pairs * 2;
     r->grow_pairs = false;
   }
   if (r->max_pairs > 0xffffffffull) r->max_pairs = 0xffffffffull;
@@ -144,6 +147,7 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   if ((e = r->tile_keys.ensure((size_t)tiles * 4)) != hipSuccess) return e;
   if ((e = r->tile_order.ensure((size_t)tiles * 4)) != hipSuccess) return e;
   if ((e = r->tile_sort.ensure(r->tile_plan.bytes)) != hipSuccess) return e;
+  if ((e = r->work.ensure((size_t)tiles * (rs.tile_w * rs.tile_h / kWave) * 4)) != hipSuccess) return e;
   r->frame_n = n;
   return hipSuccess;
 }
@@ -169,6 +173,7 @@ DrawBuffers draw_buffers(vkgs_renderer* r) {
   b.tile_order = r->tile_order.as<uint32_t>();
   b.tile_sort = r->tile_plan;
   b.tile_sort_storage = r->tile_sort.p;
+  b.work = r->timing ? r->work.as<uint32_t>() : nullptr;
   return b;
 }
 
@@ -245,7 +250,8 @@ int vkgs_renderer_destroy(vkgs_renderer* r) {
   if (!r) return VKGS_ERR_INVALID_ARG;
   (void)hipDeviceSynchronize();
   drop_graph(r);
-  DevBuf* bufs[] = {&r->position,   &r->cov3d,      &r->opacity,  &r->sh,         &r->fc,        &r->misc,
+  DevBuf* bufs[] = {&r->work,       &r->position,   &r->cov3d,    &r->opacity,    &r->sh,        &r->fc,
+                    &r->misc,
                     &r->counts,     &r->rc,         &r->key,      &r->val,        &r->val64,     &r->ids,
                     &r->sort_storage, &r->rec,      &r->rect,     &r->cnt,        &r->offs,      &r->scan_storage,
                     &r->pkeys,      &r->pvals,      &r->psort,    &r->ranges,     &r->tile_keys, &r->tile_order,
@@ -385,7 +391,8 @@ int vkgs_renderer_frame_info(vkgs_renderer* r, uint32_t* visible, uint64_t* pair
   if (visible) *visible = m[kMiscVisible];
   if (pairs) *pairs = m[kMiscPairs];
   if (m[kMiscStatus]) {
-    r->grow_pairs = true;  // the next render reallocates with twice the pair capacity
+    r->grow_pairs = true;  // the next render reallocates with room for the whole list
+    r->need_pairs = m[kMiscPairs];
     return VKGS_ERR_OVERFLOW;
   }
   return VKGS_OK;
@@ -506,6 +513,18 @@ int vkgs_renderer_set_timing(vkgs_renderer* r, int enable) {
 int vkgs_renderer_stage_ms(vkgs_renderer* r, float ms[8]) {
   if (!r || !ms) return VKGS_ERR_INVALID_ARG;
   memcpy(ms, r->stage_ms, sizeof(r->stage_ms));
+  return VKGS_OK;
+}
+
+int vkgs_renderer_raster_work(vkgs_renderer* r, uint64_t* wave_entries) {
+  if (!r || !wave_entries || !r->work.p || !r->timing) return VKGS_ERR_INVALID_ARG;
+  const size_t words = (size_t)r->rs.tiles_x * r->rs.tiles_y * (r->rs.tile_w * r->rs.tile_h / kWave);
+  std::vector<uint32_t> w(words);
+  VKGS_HIP_CHECK(hipDeviceSynchronize());
+  VKGS_HIP_CHECK(hipMemcpy(w.data(), r->work.p, words * 4, hipMemcpyDeviceToHost));
+  uint64_t sum = 0;
+  for (uint32_t x : w) sum += x;
+  *wave_entries = sum;
   return VKGS_OK;
 }
 

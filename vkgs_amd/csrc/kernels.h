@@ -97,8 +97,10 @@ hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t
 hipError_t launch_tile_order(const uint32_t* ranges, uint32_t tiles, uint32_t* keys, uint32_t* order,
                              const SortPlan& plan, void* sort_storage, hipStream_t s);
 // tile_order (optional): workgroup i rasterizes tile tile_order[i].
+// work (optional): work[i * waves + w] = list entries wave w of workgroup i
+// composited (waves = tile pixels / 64), a count of executed evaluations.
 hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
-                            const uint32_t* tile_order, float4* out, hipStream_t s);
+                            const uint32_t* tile_order, float4* out, hipStream_t s, uint32_t* work = nullptr);
 
 // Device buffers of one draw (bin, tile sort, ranges, raster).
 struct DrawBuffers {
@@ -121,6 +123,7 @@ struct DrawBuffers {
   uint32_t* tile_order;
   SortPlan tile_sort;
   void* tile_sort_storage;
+  uint32_t* work = nullptr;  // optional: list entries composited per raster wave
 };
 
 // Bins instances order[k] (k < *d_count), sorts the pairs by tile, marks the

@@ -42,12 +42,12 @@ __global__ __launch_bounds__(256) void parse_ply_kernel(const uint32_t* __restri
   const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= n) return;
   const float* row = reinterpret_cast<const float*>(ply + 60) + (size_t)off[59] * id;
-  // parse_ply.comp:106-112
+  // parse_ply.comp:46-52
   const float s0 = expf(row[off[3]]), s1 = expf(row[off[4]]), s2 = expf(row[off[5]]);
   float q[4] = {row[off[6]], row[off[7]], row[off[8]], row[off[9]]};
   const float len = sqrtf(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
   for (int k = 0; k < 4; ++k) q[k] = q[k] / len;
-  // :114-132
+  // :54-72
   const float xx = q[0] * q[0], yy = q[1] * q[1], zz = q[2] * q[2];
   const float xy = q[0] * q[1], xz = q[0] * q[2], yz = q[1] * q[2];
   const float wx = q[3] * q[0], wy = q[3] * q[1], wz = q[3] * q[2];
@@ -325,7 +325,7 @@ __global__ void inverse_scatter_kernel(const uint32_t* __restrict__ d_count, con
   const uint32_t v = *d_count;
   if (k < v && k < n) {
     const uint32_t id = index[k];
-    if (id < n) inverse[id] = (int32_t)k;  // inverse_index.comp:59
+    if (id < n) inverse[id] = (int32_t)k;  // inverse_index.comp:17
   }
 }
 

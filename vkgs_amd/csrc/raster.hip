@@ -206,15 +206,62 @@ __global__ void tile_order_keys_kernel(const uint32_t* __restrict__ ranges, uint
   vals[t] = t;
 }
 
+// A raster record restaged for one tile (see kFrameScale): the scaled inverse
+// rot_scale rows with their offsets at the corner (X0, Y0) of the tile's
+// 64 x 64 pixel block, -log2(opacity), the colour and the pixel rectangle.  A
+// lane with no record stages one that covers no pixel (offsets +inf).
+__device__ __forceinline__ void stage_record(const float4& p0, const float4& p1, const float4& p2, bool valid,
+                                             float X0, float Y0, float4* s0, float4* s1, float4* s2) {
+  if (valid) {
+    const float i00 = p0.z * kFrameScale, i01 = p0.w * kFrameScale;
+    const float i10 = p1.x * kFrameScale, i11 = p1.y * kFrameScale;
+    const float dx = X0 - p0.x, dy = Y0 - p0.y;
+    *s0 = make_float4(i00, i01, fmaf(i00, dx, i01 * dy), -__builtin_amdgcn_logf(p1.z));
+    *s1 = make_float4(i10, i11, fmaf(i10, dx, i11 * dy), p1.w);
+    *s2 = p2;
+  } else {
+    const float inf = __builtin_inff(), empty = __uint_as_float(pack16(1, 0));
+    *s0 = make_float4(0.f, 0.f, inf, inf);
+    *s1 = make_float4(0.f, 0.f, inf, empty);
+    *s2 = make_float4(0.f, 0.f, 0.f, empty);
+  }
+}
+
+// One (pixel, record) step of front-to-back compositing; (lx, ly) is the pixel
+// centre relative to its block corner.  Per step: 2 FMAs per eigenframe
+// coordinate, one max for the square, 2 FMAs and exp2 for alpha (the opacity
+// is inside the exponent), a compare and select for coverage, T * alpha, four
+// accumulating FMAs and the transmittance update.
+__device__ __forceinline__ void composite(const float4& a, const float4& b, const float4& c, float lx, float ly,
+                                          float& T, float& cr, float& cg, float& cb, float& aa) {
+  const float ux = fmaf(a.x, lx, fmaf(a.y, ly, a.z));
+  const float uy = fmaf(b.x, lx, fmaf(b.y, ly, b.z));
+  const float m = fmaxf(fabsf(ux), fabsf(uy));
+  const float e = __builtin_amdgcn_exp2f(-fmaf(ux, ux, fmaf(uy, uy, a.w)));
+  const float alpha = m <= kCoverScaled ? e : 0.f;
+  const float ta = T * alpha;
+  cr = fmaf(c.x, ta, cr);
+  cg = fmaf(c.y, ta, cg);
+  cb = fmaf(c.z, ta, cb);
+  aa = fmaf(alpha, ta, aa);
+  T = T - ta;
+}
+
+// List entries a one-wave tile composites between two exit tests: a pixel may
+// composite up to three entries past T < t_min, which only adds terms of the
+// exact composite.
+constexpr int kStepGroup = 4;
+
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restrict__ rec,
                                                            const uint32_t* __restrict__ vals,
                                                            const uint32_t* __restrict__ ranges, RasterShape rs,
                                                            const uint32_t* __restrict__ tile_order,
-                                                           float4* __restrict__ out) {
+                                                           float4* __restrict__ out, uint32_t* __restrict__ work) {
   constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
   constexpr bool kOneWave = kN == kWave;  // the wave covers the tile: every binned record touches it
+  static_assert(kN % kStepGroup == 0, "batches hold whole step groups");
   __shared__ float4 s_rec[2][3][kN];
 
   const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
@@ -227,10 +274,12 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
   const int col0 = __builtin_amdgcn_readfirstlane((int)tx * TW), col1 = col0 + TW - 1;
   const int row0 = __builtin_amdgcn_readfirstlane((int)ty * TH + (t / kWave) * kRowsPerWave);
   const int row1 = row0 + kRowsPerWave - 1;
+  const int bx0 = col0 & ~(kRasterBlock - 1), by0 = __builtin_amdgcn_readfirstlane(((int)ty * TH) & ~(kRasterBlock - 1));
+  const float X0 = (float)bx0, Y0 = (float)by0;
+  const float fx = (float)(px - bx0) + 0.5f, fy = (float)(py - by0) + 0.5f;
   const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
   const uint32_t items = end - start;
   const uint32_t batches = (items + kN - 1) / kN;
-  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   const float t_min = rs.t_min;
 
   // batch b holds list positions end-1-b*kN-t (nearest first)
@@ -246,11 +295,10 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
   // and pixels outside the image start at T = 0 (their colour is never
   // stored).
   float T = inside ? 1.f : 0.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
+  uint32_t walked = 0;
   for (uint32_t b = 0; b < batches; ++b) {
     const int buf = b & 1;
-    s_rec[buf][0][t] = p0;
-    s_rec[buf][1][t] = p1;
-    s_rec[buf][2][t] = p2;
+    stage_record(p0, p1, p2, b * kN + t < items, X0, Y0, &s_rec[buf][0][t], &s_rec[buf][1][t], &s_rec[buf][2][t]);
     if (__syncthreads_count(T >= t_min) == 0) break;
     // gather the next batch while this one is composited
     const uint32_t next = (b + 1) * kN + t;
@@ -261,53 +309,49 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
       p2 = rec[3 * (size_t)k + 2];
     }
     const int cnt = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kN, items - b * kN));
-    if (__ballot(T >= t_min) != 0) {
-      const float4* s0 = s_rec[buf][0];
-      const float4* s1 = s_rec[buf][1];
-      const float4* s2 = s_rec[buf][2];
-      bool alive = T >= t_min;
+    if (__ballot(T >= t_min) == 0) continue;
+    const float4* s0 = s_rec[buf][0];
+    const float4* s1 = s_rec[buf][1];
+    const float4* s2 = s_rec[buf][2];
+    if (kOneWave) {
+      // step groups of kStepGroup entries (the batch's tail is padded with
+      // records that cover nothing), one exit test per group
+      for (int e = 0; e < cnt; e += kStepGroup) {
+#pragma unroll
+        for (int j = 0; j < kStepGroup; ++j) composite(s0[e + j], s1[e + j], s2[e + j], fx, fy, T, cr, cg, cb, aa);
+        walked += (uint32_t)min(kStepGroup, cnt - e);
+        if (__ballot(T >= t_min) == 0) break;
+      }
+    } else {
       for (int e = 0; e < cnt; ++e) {
-        const float4 a = s0[e], bb = s1[e], c = s2[e];
-        bool touches = true;
-        if (!kOneWave) {
-          const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
-          const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
-          const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
-          const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
-          touches = !(by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1);
-        }
-        if (touches) {
-          const float dx = fx - a.x, dy = fy - a.y;
-          const float ux = fmaf(a.z, dx, a.w * dy);
-          const float uy = fmaf(bb.x, dx, bb.y * dy);
-          const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && alive;
-          const float q = fmaf(ux, ux, uy * uy);
-          const float alpha = in ? bb.z * gauss_falloff(q) : 0.f;
-          const float ta = T * alpha;
-          cr = fmaf(c.x, ta, cr);
-          cg = fmaf(c.y, ta, cg);
-          cb = fmaf(c.z, ta, cb);
-          aa = fmaf(alpha, ta, aa);
-          T = T - ta;
-          alive = T >= t_min;
-          if (__ballot(alive) == 0) break;
-        }
+        const uint32_t rx = __builtin_amdgcn_readfirstlane(__float_as_uint(s1[e].w));
+        const uint32_t ry = __builtin_amdgcn_readfirstlane(__float_as_uint(s2[e].w));
+        const int rx0 = (int)(rx & 0xffffu), rx1 = (int)(rx >> 16);
+        const int ry0 = (int)(ry & 0xffffu), ry1 = (int)(ry >> 16);
+        if (ry1 < row0 || ry0 > row1 || rx1 < col0 || rx0 > col1) continue;
+        composite(s0[e], s1[e], s2[e], fx, fy, T, cr, cg, cb, aa);
+        ++walked;
+        if (__ballot(T >= t_min) == 0) break;
       }
     }
   }
   if (inside) out[(size_t)py * rs.width + px] = make_float4(cr, cg, cb, aa + T);
+  if (work && t % kWave == 0) work[(size_t)blockIdx.x * (kN / kWave) + t / kWave] = walked;
 }
 
 // Scalar-broadcast raster: every wave walks its tile's list on its own.  The
 // list entry and its 48-byte record have wave-uniform addresses, so they are
 // fetched with scalar loads into SGPRs once per wave (no LDS traffic, no
 // barrier); the next record is fetched while the current one is composited.
-// Waves of one tile share records through the scalar cache.
+// Waves of one tile share records through the scalar cache.  Each lane
+// restages the record itself (the same operations as stage_record).
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void rasterize_wave_kernel(const float4* __restrict__ rec,
                                                                 const uint32_t* __restrict__ vals,
                                                                 const uint32_t* __restrict__ ranges, RasterShape rs,
-                                                                float4* __restrict__ out) {
+                                                                float4* __restrict__ out,
+                                                                uint32_t* __restrict__ work) {
+  constexpr int kN = TW * TH;
   constexpr int kRowsPerWave = kWave / TW;
   const uint32_t tile = blockIdx.x;
   const uint32_t tx = tile % rs.tiles_x, ty = tile / rs.tiles_x;
@@ -317,13 +361,15 @@ __global__ __launch_bounds__(TW* TH) void rasterize_wave_kernel(const float4* __
   const int col0 = __builtin_amdgcn_readfirstlane((int)tx * TW), col1 = col0 + TW - 1;
   const int row0 = __builtin_amdgcn_readfirstlane((int)ty * TH + (t / kWave) * kRowsPerWave);
   const int row1 = row0 + kRowsPerWave - 1;
+  const int bx0 = col0 & ~(kRasterBlock - 1), by0 = __builtin_amdgcn_readfirstlane(((int)ty * TH) & ~(kRasterBlock - 1));
+  const float X0 = (float)bx0, Y0 = (float)by0;
+  const float fx = (float)(px - bx0) + 0.5f, fy = (float)(py - by0) + 0.5f;
   const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
-  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   const float t_min = rs.t_min;
 
-  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
-  bool done = !inside;
-  if (end > start && __ballot(!done) != 0) {
+  float T = inside ? 1.f : 0.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
+  uint32_t walked = 0;
+  if (end > start && __ballot(T >= t_min) != 0) {
     uint32_t j = end - 1;
     uint32_t k = __builtin_amdgcn_readfirstlane(vals[j]);
     float4 a = rec[3 * (size_t)k], bb = rec[3 * (size_t)k + 1], c = rec[3 * (size_t)k + 2];
@@ -331,24 +377,15 @@ __global__ __launch_bounds__(TW* TH) void rasterize_wave_kernel(const float4* __
       const bool more = j > start;
       const uint32_t kn = more ? __builtin_amdgcn_readfirstlane(vals[j - 1]) : k;
       const float4 na = rec[3 * (size_t)kn], nb = rec[3 * (size_t)kn + 1], nc = rec[3 * (size_t)kn + 2];
-      const uint32_t bx = __float_as_uint(bb.w), by = __float_as_uint(c.w);
-      const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
-      const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
-      if (!(by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1)) {
-        const float dx = fx - a.x, dy = fy - a.y;
-        const float ux = fmaf(a.z, dx, a.w * dy);
-        const float uy = fmaf(bb.x, dx, bb.y * dy);
-        const bool in = fabsf(ux) <= 3.f && fabsf(uy) <= 3.f && !done;
-        const float q = fmaf(ux, ux, uy * uy);
-        const float alpha = in ? bb.z * gauss_falloff(q) : 0.f;
-        const float ta = T * alpha;
-        cr = fmaf(c.x, ta, cr);
-        cg = fmaf(c.y, ta, cg);
-        cb = fmaf(c.z, ta, cb);
-        aa = fmaf(alpha, ta, aa);
-        T = T - ta;
-        done = done || T < t_min;
-        if (__ballot(!done) == 0) break;
+      const uint32_t rx = __float_as_uint(bb.w), ry = __float_as_uint(c.w);
+      const int rx0 = (int)(rx & 0xffffu), rx1 = (int)(rx >> 16);
+      const int ry0 = (int)(ry & 0xffffu), ry1 = (int)(ry >> 16);
+      if (!(ry1 < row0 || ry0 > row1 || rx1 < col0 || rx0 > col1)) {
+        float4 s0, s1, s2;
+        stage_record(a, bb, c, true, X0, Y0, &s0, &s1, &s2);
+        composite(s0, s1, s2, fx, fy, T, cr, cg, cb, aa);
+        ++walked;
+        if (__ballot(T >= t_min) == 0) break;
       }
       if (!more) break;
       --j;
@@ -358,6 +395,7 @@ __global__ __launch_bounds__(TW* TH) void rasterize_wave_kernel(const float4* __
     }
   }
   if (inside) out[(size_t)py * rs.width + px] = make_float4(cr, cg, cb, aa + T);
+  if (work && t % kWave == 0) work[(size_t)blockIdx.x * (kN / kWave) + t / kWave] = walked;
 }
 
 __global__ void gather_u32_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx,
@@ -468,15 +506,15 @@ hipError_t launch_tile_order(const uint32_t* ranges, uint32_t tiles, uint32_t* k
 }
 
 hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint32_t* ranges, const RasterShape& rs,
-                            const uint32_t* tile_order, float4* out, hipStream_t s) {
+                            const uint32_t* tile_order, float4* out, hipStream_t s, uint32_t* work) {
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tw = rs.tile_w, th = rs.tile_h;
 #define VKGS_RASTER_CASE(W_, H_)                                                                   \
   if (tw == W_ && th == H_) {                                                                      \
     if (rs.variant == 0)                                                                           \
-      rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, tile_order, out);  \
+      rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, tile_order, out, work); \
     else                                                                                           \
-      rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out);         \
+      rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out, work);   \
     return hipGetLastError();                                                                      \
   }
   VKGS_RASTER_CASE(16, 16)

@@ -234,7 +234,8 @@ class Renderer:
     def sorted_ids(self):
         ip, kp = C.c_void_p(), C.c_void_p()
         check(lib().vkgs_renderer_sorted(self._h, C.byref(ip), C.byref(kp)), "vkgs_renderer_sorted")
-        _, v, _ = self.frame_info()
+        st, v, _ = self.frame_info()
+        check(st, "vkgs_renderer_frame_info")
         ids = _download_u32(ip.value, v)
         keys = _download_u32(kp.value, v)
         return ids, keys
@@ -244,7 +245,9 @@ class Renderer:
         tx, ty = C.c_uint32(), C.c_uint32()
         check(lib().vkgs_renderer_tile_lists(self._h, C.byref(rp), C.byref(lp), C.byref(tx), C.byref(ty)),
               "vkgs_renderer_tile_lists")
-        _, _, k = self.frame_info()
+        # an overflowing frame's pair count exceeds the list storage: its lists are incomplete
+        st, _, k = self.frame_info()
+        check(st, "vkgs_renderer_frame_info")
         tiles = tx.value * ty.value
         ranges = _download_u32(rp.value, 2 * tiles).reshape(tiles, 2).astype(np.int64)
         return ranges, _download_u32(lp.value, k)
@@ -268,6 +271,12 @@ class Renderer:
         a = (C.c_float * 8)()
         check(lib().vkgs_renderer_stage_ms(self._h, a), "vkgs_renderer_stage_ms")
         return dict(zip(STAGES, list(a)))
+
+    def raster_work(self) -> int:
+        """List entries the raster's waves composited in the last timed frame (64 lanes each)."""
+        w = C.c_uint64()
+        check(lib().vkgs_renderer_raster_work(self._h, C.byref(w)), "vkgs_renderer_raster_work")
+        return int(w.value)
 
 
 def _copy_device(dst: int, src: int, nbytes: int) -> None:
