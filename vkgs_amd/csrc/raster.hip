@@ -339,6 +339,97 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
   if (work && t % kWave == 0) work[(size_t)blockIdx.x * (kN / kWave) + t / kWave] = walked;
 }
 
+// Two 8x8 tiles per wave: lanes 0-31 composite tile order[2i], lanes 32-63
+// tile order[2i + 1], each lane two pixels of its tile (rows r and r + 4).  A
+// record read from LDS then serves two evaluations: an 8x8 tile with one pixel
+// per lane reads 40 B of LDS per lane and list entry, which bounds it (the LDS
+// returns 128 B per clock to the CU's four SIMDs) before its 16 VALU
+// instructions do.  Consecutive tiles of the longest-first schedule have list
+// lengths within a bucket of each other, so little of the wave idles.  Both
+// halves step through their lists together; a half whose list ended reads
+// records that cover nothing.
+__global__ __launch_bounds__(kWave) void rasterize_pair_kernel(const float4* __restrict__ rec,
+                                                               const uint32_t* __restrict__ vals,
+                                                               const uint32_t* __restrict__ ranges, RasterShape rs,
+                                                               const uint32_t* __restrict__ tile_order,
+                                                               float4* __restrict__ out,
+                                                               uint32_t* __restrict__ work) {
+  constexpr int kHalf = kWave / 2;   // lanes per tile, list entries per batch
+  constexpr int kStride = kHalf + 1; // the second tile's records start one float4 later: other LDS banks
+  static_assert(kHalf % kStepGroup == 0, "batches hold whole step groups");
+  __shared__ float4 s_rec[2][3][2 * kStride];
+
+  const int t = threadIdx.x, h = t / kHalf, l = t % kHalf;
+  const uint32_t tiles = rs.tiles_x * rs.tiles_y;
+  const uint32_t slot = 2 * blockIdx.x + h;
+  const bool has_tile = slot < tiles;
+  const uint32_t tile = has_tile ? (tile_order ? tile_order[slot] : slot) : 0u;
+  const uint32_t tx = tile % rs.tiles_x, ty = tile / rs.tiles_x;
+  const int px = (int)tx * 8 + l % 8, py0 = (int)ty * 8 + l / 8, py1 = py0 + 4;
+  const bool in0 = has_tile && px < (int)rs.width && py0 < (int)rs.height;
+  const bool in1 = has_tile && px < (int)rs.width && py1 < (int)rs.height;
+  const int bx0 = ((int)tx * 8) & ~(kRasterBlock - 1), by0 = ((int)ty * 8) & ~(kRasterBlock - 1);
+  const float X0 = (float)bx0, Y0 = (float)by0;
+  const float fx = (float)(px - bx0) + 0.5f, fy0 = (float)(py0 - by0) + 0.5f, fy1 = fy0 + 4.f;
+  uint32_t end = 0, items = 0;
+  if (has_tile) {
+    end = ranges[2 * tile + 1];
+    items = end - ranges[2 * tile];
+  }
+  const uint32_t items_a = __builtin_amdgcn_readlane(items, 0), items_b = __builtin_amdgcn_readlane(items, kHalf);
+  const uint32_t longest = max(items_a, items_b);
+  const uint32_t batches = (longest + kHalf - 1) / kHalf;
+  const float t_min = rs.t_min;
+
+  // batch b holds list positions end-1-b*kHalf-l of each half's tile (nearest first)
+  float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
+  if ((uint32_t)l < items) {
+    const uint32_t k = vals[end - 1 - l];
+    p0 = rec[3 * (size_t)k];
+    p1 = rec[3 * (size_t)k + 1];
+    p2 = rec[3 * (size_t)k + 2];
+  }
+  float T0 = in0 ? 1.f : 0.f, r0 = 0.f, g0 = 0.f, b0 = 0.f, a0 = 0.f;
+  float T1 = in1 ? 1.f : 0.f, r1 = 0.f, g1 = 0.f, b1 = 0.f, a1 = 0.f;
+  uint32_t walked_a = 0, walked_b = 0;
+  for (uint32_t b = 0; b < batches; ++b) {
+    const int buf = b & 1;
+    const int at = h * kStride + l;
+    stage_record(p0, p1, p2, b * kHalf + l < items, X0, Y0, &s_rec[buf][0][at], &s_rec[buf][1][at],
+                 &s_rec[buf][2][at]);
+    if (__syncthreads_count((T0 >= t_min) | (T1 >= t_min)) == 0) break;
+    const uint32_t next = (b + 1) * kHalf + l;
+    if (next < items) {
+      const uint32_t k = vals[end - 1 - next];
+      p0 = rec[3 * (size_t)k];
+      p1 = rec[3 * (size_t)k + 1];
+      p2 = rec[3 * (size_t)k + 2];
+    }
+    const uint32_t done = b * kHalf;
+    const int cnt_a = items_a > done ? (int)min((uint32_t)kHalf, items_a - done) : 0;
+    const int cnt_b = items_b > done ? (int)min((uint32_t)kHalf, items_b - done) : 0;
+    const int cnt = max(cnt_a, cnt_b);
+    if (__ballot((T0 >= t_min) | (T1 >= t_min)) == 0) continue;
+    const float4* s0 = s_rec[buf][0] + h * kStride;
+    const float4* s1 = s_rec[buf][1] + h * kStride;
+    const float4* s2 = s_rec[buf][2] + h * kStride;
+    for (int e = 0; e < cnt; e += kStepGroup) {
+#pragma unroll
+      for (int j = 0; j < kStepGroup; ++j) {
+        const float4 ra = s0[e + j], rb = s1[e + j], rc = s2[e + j];
+        composite(ra, rb, rc, fx, fy0, T0, r0, g0, b0, a0);
+        composite(ra, rb, rc, fx, fy1, T1, r1, g1, b1, a1);
+      }
+      walked_a += (uint32_t)max(0, min(kStepGroup, cnt_a - e));
+      walked_b += (uint32_t)max(0, min(kStepGroup, cnt_b - e));
+      if (__ballot((T0 >= t_min) | (T1 >= t_min)) == 0) break;
+    }
+  }
+  if (in0) out[(size_t)py0 * rs.width + px] = make_float4(r0, g0, b0, a0 + T0);
+  if (in1) out[(size_t)py1 * rs.width + px] = make_float4(r1, g1, b1, a1 + T1);
+  if (work && l == 0 && has_tile) work[slot] = h ? walked_b : walked_a;
+}
+
 // Scalar-broadcast raster: every wave walks its tile's list on its own.  The
 // list entry and its 48-byte record have wave-uniform addresses, so they are
 // fetched with scalar loads into SGPRs once per wave (no LDS traffic, no
@@ -516,6 +607,10 @@ hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint3
     else                                                                                           \
       rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out, work);   \
     return hipGetLastError();                                                                      \
+  }
+  if (tw == 8 && th == 8 && rs.variant == 0) {
+    rasterize_pair_kernel<<<(tiles + 1) / 2, kWave, 0, s>>>(rec, vals, ranges, rs, tile_order, out, work);
+    return hipGetLastError();
   }
   VKGS_RASTER_CASE(16, 16)
   VKGS_RASTER_CASE(8, 8)
