@@ -389,6 +389,14 @@ void oracle_projection(const oracle_camera* cam, const float model[16], const fl
       sin_t = NAN;
     }
     float rs00 = s0 * cos_t, rs01 = s0 * sin_t, rs10 = (-s1) * sin_t, rs11 = s1 * cos_t;
+    /* A NaN anywhere in the eigendecomposition (D = 0 for an isotropic cov2d,
+     * or a + b < D by rounding for a needle) makes the whole rot_scale the
+     * canonical quiet NaN, so its bits do not depend on how a platform
+     * propagates NaN signs; the splat draws nothing either way. */
+    if (rs00 != rs00 || rs01 != rs01 || rs10 != rs10 || rs11 != rs11) {
+      union { uint32_t u; float f; } qnan = {0x7fc00000u};
+      rs00 = rs01 = rs10 = rs11 = qnan.f;
+    }
     /* :136-137 */
     m4_mulv(P, p4, t4);
     float ndc[3] = {t4[0] / t4[3], t4[1] / t4[3], t4[2] / t4[3]};

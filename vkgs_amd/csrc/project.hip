@@ -195,6 +195,14 @@ __device__ __forceinline__ void project_splat(const FrameConst& fc, const float*
   // :177-179
   out[0] = make_float4(nx, ny, nz, 0.f);
   out[1] = make_float4(s0 * cos_t, s0 * sin_t, (-s1) * sin_t, s1 * cos_t);
+  // a NaN anywhere in the eigendecomposition (D = 0 for an isotropic cov2d, or
+  // a + b < D by rounding for a needle) makes the whole rot_scale the canonical
+  // quiet NaN, whose bits do not depend on how NaN signs propagate; such a
+  // splat draws nothing either way
+  if (!(out[1].x == out[1].x && out[1].y == out[1].y && out[1].z == out[1].z && out[1].w == out[1].w)) {
+    const float q = __uint_as_float(0x7fc00000u);
+    out[1] = make_float4(q, q, q, q);
+  }
   out[2] = make_float4(color[0], color[1], color[2], opacity[id]);
 }
 

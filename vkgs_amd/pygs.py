@@ -22,7 +22,7 @@ from . import scene as _scene
 
 _lock = threading.Lock()
 _state = {"renderer": None, "thread": None, "stop": False, "pending": None, "frame": None, "camera": None,
-          "size": (1600, 900)}
+          "size": (1600, 900), "error": None}
 
 rasterize = _diff.render
 rasterize_batch = _diff.render_batch
@@ -48,6 +48,20 @@ def _load_now(path: str) -> None:
     r.load_ply(path)
 
 
+def _render_checked(r, cam):
+    """One frame; a frame whose tile lists outgrew the renderer's pair storage is rendered again
+    (the renderer then sizes the storage for the whole list), so no incomplete frame is returned."""
+    for attempt in range(2):
+        img = r.render(cam)
+        st, _, _ = r.frame_info()  # synchronous
+        if st == 0:
+            return img
+        if st != 6 or attempt == 1:  # VKGS_ERR_OVERFLOW twice, or another error
+            from ._lib import check
+            check(st, "vkgs_renderer_frame_info")
+    return img
+
+
 def _loop() -> None:
     import torch
     while True:
@@ -62,8 +76,14 @@ def _loop() -> None:
             threading.Event().wait(0.01)
             continue
         cam = _state["camera"] or _scene.orbit_camera(*_state["size"], 2.0)
-        img = r.render(cam)
-        torch.cuda.synchronize()
+        try:
+            img = _render_checked(r, cam)
+            torch.cuda.synchronize()
+        except Exception as e:  # reported through latest_error(); the loop stops
+            with _lock:
+                _state["error"] = e
+                _state["stop"] = True
+            break
         with _lock:
             _state["frame"] = img
 
@@ -76,6 +96,7 @@ def show() -> None:
             print("[pygs] viewer is already running")
             return
         _state["stop"] = False
+        _state["error"] = None
         t = threading.Thread(target=_loop, name="vkgs-render", daemon=True)
         _state["thread"] = t
     t.start()
@@ -118,13 +139,20 @@ def latest_frame():
         return _state["frame"]
 
 
+def latest_error():
+    """The exception that stopped the render loop, if any."""
+    with _lock:
+        return _state["error"]
+
+
 def render(camera=None, width: int = 1600, height: int = 900, r: float = 2.0):
     """One frame of the loaded splats, synchronously."""
     rnd = _state["renderer"]
     if rnd is None or rnd.n == 0:
         raise RuntimeError("no splats loaded: call pygs.load(path) first")
     cam = camera or _scene.orbit_camera(width, height, r)
-    return rnd.render(cam)
+    return _render_checked(rnd, cam)
 
 
-__all__ = ["show", "load", "close", "render", "set_camera", "latest_frame", "rasterize", "rasterize_batch"]
+__all__ = ["show", "load", "close", "render", "set_camera", "latest_frame", "latest_error", "rasterize",
+           "rasterize_batch"]
