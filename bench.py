@@ -45,7 +45,7 @@ def parse_args(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="timed fwd+bwd steps of the batch workload (0 = skip)")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scales every scene size of the batch workload")
-    ap.add_argument("--fwd-bwd-mode", choices=("both", "plain", "batched"), default="both",
+    ap.add_argument("--fwd-bwd-mode", choices=("both", "plain", "batched", "batched_eager"), default="both",
                     help="which fwd+bwd forms to time (one form alone is for profiling it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a known-size kernel for scaling the "
@@ -150,9 +150,10 @@ def bench_c4(steps: int):
 
 def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
     """views/s of fwd+bwd (L1 loss, the mean over each view summed over views) for the batch
-    workload: one launch set and one loss per view (plain), and one launch set for all views
-    with the loss formed over their flat (P, 4) output and the 32-view scene processed in
-    spatial order (batched)."""
+    workload: one launch set and one loss per view (plain); one launch set for all views with the
+    loss formed over their flat (P, 4) output and the 32-view scene processed in spatial order
+    (batched_eager); and that step captured once into a hipGraph and replayed (batched: one graph
+    launch per step, its loss gradient formed elementwise inside the graph)."""
     import torch
     from vkgs_amd import diff as D
 
@@ -177,16 +178,28 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
             img = D.render(**params[si], camera=c)
             (img - tgt).abs().mean().backward()
 
-    def batched_step():
+    def batched_eager_step():
         zero_grad()
         flat, _ = D.render_batch_flat(params, views, spatial_order=True)
         ((flat - tgt_flat).abs() * w_flat).sum().backward()
 
+    def l1_grad(flat):  # d/d(flat) of the summed per-view L1 means: elementwise, no reduction
+        return torch.sign(flat - tgt_flat) * w_flat
+
+    graphed = {}
+
+    def batched_step():  # the whole fwd + bwd step over all views as one hipGraph launch
+        if "step" not in graphed:
+            zero_grad()
+            graphed["step"] = D.GraphedStep(params, views, l1_grad, spatial_order=True)
+        graphed["step"].step()
+
     out = {}
-    for name, fn in (("plain", plain_step), ("batched", batched_step)):
+    forms = (("plain", plain_step), ("batched_eager", batched_eager_step), ("batched", batched_step))
+    for name, fn in forms:
         if mode not in ("both", name):
             continue
-        fn()  # warm-up: sizes every buffer
+        fn()  # warm-up: sizes every buffer (and captures the graphed step)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -195,11 +208,14 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
         dt = (time.perf_counter() - t0) / steps
         out[f"{name}_views_per_s"] = round(len(views) / dt, 2)
         out[f"{name}_ms_per_view"] = round(dt * 1e3 / len(views), 3)
-        if name == "batched":
+        if name == "batched_eager":
             out["batched_visible_items"] = D.last_info["visible"]
             out["batched_pairs"] = D.last_info["pairs"]
+    if "step" in graphed:
+        graphed["step"].check()  # no replay overflowed its pair list
     if mode == "both":
         out["batched_over_plain"] = round(out["batched_views_per_s"] / out["plain_views_per_s"], 3)
+        out["batched_eager_over_plain"] = round(out["batched_eager_views_per_s"] / out["plain_views_per_s"], 3)
     out["views"] = len(views)
     out["splats"] = [s.n for s in scenes]
     return out

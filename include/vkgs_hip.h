@@ -320,6 +320,15 @@ int vkgs_diff_backward(vkgs_diff_ctx* ctx, const vkgs_diff_scene* scenes, uint32
  * if its pair list overflowed, in which case *pairs is the whole list's length
  * and the next forward sizes the pair storage for it (a retry then fits). */
 int vkgs_diff_info(vkgs_diff_ctx* ctx, uint32_t* visible, uint64_t* pairs, void* stream);
+/* Synchronous: VKGS_ERR_OVERFLOW if any forward since the previous check
+ * overflowed its pair list (the results of that forward and its backward are
+ * incomplete), with *pairs_needed the longest such list; the next forward then
+ * sizes the pair storage for it.  Clears the record.  vkgs_diff_forward and
+ * vkgs_diff_backward themselves never wait for the device or read results
+ * back, so a caller may capture them (with its own work between them) into a
+ * hipGraph on one stream, after an eager call has sized every buffer for the
+ * same scenes and views, and check this after replays. */
+int vkgs_diff_check(vkgs_diff_ctx* ctx, uint64_t* pairs_needed, void* stream);
 
 #ifdef __cplusplus
 }

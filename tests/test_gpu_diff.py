@@ -488,3 +488,46 @@ def test_c4_full_size_fwd_bwd_step():
             losses.append(float((D.render(**q, camera=cam) - target).abs().mean()))
     print("c4 loss after steps", losses, flush=True)
     assert min(losses) < float(loss)
+
+
+def test_graphed_step_replays_match_eager_steps():
+    """diff.GraphedStep: forward, elementwise loss gradient and backward captured into one hipGraph.
+    Four replays, with eager steps of other views and host synchronizations between them, each give
+    the gradients of the same step run eagerly, bit for bit (the passes never wait for the device, so
+    nothing of them is left out of the graph)."""
+    sa, cam = small_scene(3000, 71, 80, 64)
+    sb, _ = small_scene(1200, 72, 80, 64)
+    cams = [cam] + [S.orbit_camera(80, 64, 4.0 + 0.3 * i, theta=0.5 * i) for i in range(1, 5)]
+    views = [(0, c) for c in cams] + [(1, S.orbit_camera(64, 48, 3.6, phi=1.1))]
+    pa, pb = D.scene_tensors(sa), D.scene_tensors(sb)
+    spans, total = D._spans([(v[0], v[1], None) for v in views])
+    tgt = torch.rand((total, 4), device="cuda")
+    w = torch.rand((total, 1), device="cuda") + 0.5
+
+    def grad_fn(flat):
+        return torch.sign(flat - tgt) * w
+
+    def eager():
+        for p in (pa, pb):
+            for t in p.values():
+                t.grad = None
+        flat, _ = D.render_batch_flat([pa, pb], views, spatial_order=True)
+        flat.backward(grad_fn(flat.detach()))
+        return flat.detach().clone(), {k: (pa[k].grad.clone(), pb[k].grad.clone()) for k in NAMES}
+
+    ref_img, ref_g = eager()
+    step = D.GraphedStep([pa, pb], views, grad_fn, spatial_order=True)
+    other = [(0, S.orbit_camera(96, 72, 5.0, theta=2.0))]
+    for rep in range(4):
+        flat = step.step()
+        torch.cuda.synchronize()
+        print("replay", rep, flush=True)
+        np.testing.assert_array_equal(flat.cpu().numpy(), ref_img.cpu().numpy())
+        for k in NAMES:
+            np.testing.assert_array_equal(pa[k].grad.cpu().numpy(), ref_g[k][0].cpu().numpy())
+            np.testing.assert_array_equal(pb[k].grad.cpu().numpy(), ref_g[k][1].cpu().numpy())
+        # eager work of the program between replays (another view set through the pooled contexts)
+        o, _ = D.render_batch_flat([pa], other)
+        o.sum().backward()
+        torch.cuda.synchronize()
+    step.check()

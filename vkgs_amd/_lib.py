@@ -117,6 +117,7 @@ SIGNATURES = {
     "vkgs_diff_forward": (_I, [_P, _P, _U32, _P, _U32, _P, _P]),
     "vkgs_diff_backward": (_I, [_P, _P, _U32, _P, _U32, _P]),
     "vkgs_diff_info": (_I, [_P, _P, _P, _P]),
+    "vkgs_diff_check": (_I, [_P, _P, _P]),
 }
 
 _lib = None

@@ -86,7 +86,17 @@ struct vkgs_diff_ctx {
 
 namespace {
 
-enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscSlotPairs = 3 };
+// misc words: per forward (cleared by it) visible, pairs, status, slot-order
+// pairs; across forwards (cleared by vkgs_diff_check) an overflow flag and the
+// largest overflowing pair count
+enum : uint32_t {
+  kMiscVisible = 0,
+  kMiscPairs = 1,
+  kMiscStatus = 2,
+  kMiscSlotPairs = 3,
+  kMiscSticky = 4,
+  kMiscStickyPairs = 5
+};
 constexpr uint32_t kPermRefresh = 32;
 // Smallest stop threshold of the differentiable raster: the backward recovers
 // the transmittance in front of each entry as T_i = T_{i+1} / (1 - alpha_i)
@@ -105,9 +115,9 @@ size_t pack(std::vector<unsigned char>& out, const std::vector<T>& v) {
   return off;
 }
 
-// Packs the host tables into one buffer, uploads it (synchronously with
-// respect to the host, so the host vectors can change at once) and points the
-// launch's table pointers into it.
+// Packs the host tables into one buffer, uploads it through kernel arguments
+// (stream-ordered: no host sync, and a captured graph replays the same
+// tables) and points the launch's table pointers into it.
 hipError_t upload_tables(vkgs_diff_ctx* c, hipStream_t s) {
   std::vector<unsigned char>& h = c->h_tables;
   h.clear();
@@ -116,9 +126,8 @@ hipError_t upload_tables(vkgs_diff_ctx* c, hipStream_t s) {
   const size_t o_sb = pack(h, c->h_scene_blocks), o_sv = pack(h, c->h_scene_views);
   hipError_t e = c->tables.ensure(h.size());
   if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(c->tables.p, h.data(), h.size(), hipMemcpyHostToDevice, s);
-  if (e != hipSuccess) return e;
-  e = hipStreamSynchronize(s);
+  h.resize(align_up(h.size(), 4));
+  e = upload_launch(c->tables.p, h.data(), h.size(), s);
   if (e != hipSuccess) return e;
   unsigned char* base = c->tables.as<unsigned char>();
   DiffLaunch& L = c->launch;
@@ -279,7 +288,9 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   ENSURE(key, (size_t)cap * 4);
   ENSURE(masks, (size_t)(blocks ? blocks : 1) * kMaskWords * 8);
   ENSURE(g9, (size_t)cap * 36);
+  const void* old_misc = c->misc.p;
   ENSURE(misc, 256);
+  if (e == hipSuccess && c->misc.p != old_misc) e = fill_u32_launch(c->misc.as<uint32_t>(), 0u, 64, s);
   ENSURE(offs, (size_t)cap * 4);
   ENSURE(scnt, (size_t)cap * 4);
   ENSURE(soffs, (size_t)cap * 4);
@@ -288,6 +299,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   ENSURE(pkeys, (size_t)c->max_pairs * 4);
   ENSURE(pvals, (size_t)c->max_pairs * 4);
   ENSURE(pslot, (size_t)c->max_pairs * 4);
+  ENSURE(gpart, (size_t)c->max_pairs * 36);  // per-pair partials of the backward: the device-side bound
   ENSURE(pair_sort, c->pair_plan.bytes);
   ENSURE(ranges, (size_t)(tiles ? tiles : 1) * 8);
   ENSURE(tile_keys, (size_t)(tiles ? tiles : 1) * 4);
@@ -414,6 +426,9 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   else
     VKGS_HIP_CHECK(sort_kv_launch(c->pair_plan, misc + kMiscPairs, c->pkeys.as<uint32_t>(),
                                   c->pvals.as<uint32_t>(), tile_bits, c->pair_sort.p, s));
+  // (status, pairs) -> the sticky record read by vkgs_diff_check
+  VKGS_HIP_CHECK(fold_status_launch(misc + kMiscStatus, misc + kMiscPairs, misc + kMiscSticky,
+                                    misc + kMiscStickyPairs, s));
   VKGS_HIP_CHECK(launch_ranges(c->pkeys.as<uint32_t>(), misc + kMiscPairs, c->max_pairs, (uint32_t)tiles,
                                c->ranges.as<uint32_t>(), s));
   VKGS_HIP_CHECK(launch_tile_order(c->ranges.as<uint32_t>(), (uint32_t)tiles, c->tile_keys.as<uint32_t>(),
@@ -449,19 +464,31 @@ int vkgs_diff_backward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t
   VKGS_HIP_CHECK(upload_tables(c, s));
   const DiffLaunch& L = c->launch;
   uint32_t* misc = c->misc.as<uint32_t>();
-  // per-pair partials: sized by the live pair count of the forward
-  uint32_t m[4];
-  VKGS_HIP_CHECK(hipMemcpy(m, misc, sizeof(m), hipMemcpyDeviceToHost));
-  const uint64_t pairs = m[kMiscPairs] < c->max_pairs ? m[kMiscPairs] : c->max_pairs;
-  if (c->gpart.ensure((size_t)(pairs ? pairs : 1) * 36) != hipSuccess) return VKGS_ERR_HIP;
+  // per-pair partials: gpart holds max_pairs, the bound of the live count the
+  // kernels read on the device (no host read-back, so a graph can capture this)
   VKGS_HIP_CHECK(launch_diff_raster_backward(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(),
                                              c->ranges.as<uint32_t>(), c->pslot.as<uint32_t>(),
-                                             c->tile_order.as<uint32_t>(), pairs, c->soffs.as<uint32_t>(),
+                                             c->tile_order.as<uint32_t>(), c->max_pairs, c->soffs.as<uint32_t>(),
                                              c->scnt.as<uint32_t>(), misc + kMiscVisible, c->cap_items,
                                              c->gpart.as<float>(), c->g9.as<float>(), s));
   VKGS_HIP_CHECK(launch_diff_project_backward(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(),
                                               c->g9.as<float>(), s));
   return VKGS_OK;
+}
+
+int vkgs_diff_check(vkgs_diff_ctx* c, uint64_t* pairs_needed, void* stream) {
+  if (!c || !c->misc.p) return VKGS_ERR_INVALID_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t m[8];
+  VKGS_HIP_CHECK(hipStreamSynchronize(s));
+  VKGS_HIP_CHECK(hipMemcpy(m, c->misc.p, sizeof(m), hipMemcpyDeviceToHost));
+  if (pairs_needed) *pairs_needed = m[kMiscStickyPairs];
+  if (!m[kMiscSticky]) return VKGS_OK;
+  c->grow_pairs = true;
+  c->need_pairs = m[kMiscStickyPairs] > c->need_pairs ? m[kMiscStickyPairs] : c->need_pairs;
+  VKGS_HIP_CHECK(fill_u32_launch(c->misc.as<uint32_t>() + kMiscSticky, 0u, 2, s));
+  VKGS_HIP_CHECK(hipStreamSynchronize(s));
+  return VKGS_ERR_OVERFLOW;
 }
 
 int vkgs_diff_info(vkgs_diff_ctx* c, uint32_t* visible, uint64_t* pairs, void* stream) {

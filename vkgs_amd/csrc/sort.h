@@ -78,4 +78,16 @@ struct Words16 {
 };
 hipError_t set_words_launch(uint32_t* dst, const Words16& words, uint32_t n, hipStream_t stream);
 
+// Stream-ordered host-to-device copy of `bytes` (a multiple of 4) carried in
+// the arguments of kUploadChunkWords-word kernel launches: the host buffer can
+// change as soon as the call returns, no host sync and no memcpy node is
+// needed, and a captured hipGraph replays the same bytes.  For small tables.
+constexpr uint32_t kUploadChunkWords = 768;
+hipError_t upload_launch(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
+// Sticky overflow record: if *status (a list's overflow flag) is set,
+// *sticky = 1 and *sticky_count = max(*sticky_count, *count) (the list's full length).
+hipError_t fold_status_launch(const uint32_t* status, const uint32_t* count, uint32_t* sticky,
+                              uint32_t* sticky_count, hipStream_t stream);
+
 }  // namespace vkgs

@@ -386,6 +386,22 @@ __global__ void set_words(uint32_t* __restrict__ dst, Words16 words, uint32_t n)
   if (threadIdx.x < n) dst[threadIdx.x] = words.w[threadIdx.x];
 }
 
+__global__ void fold_status_kernel(const uint32_t* __restrict__ status, const uint32_t* __restrict__ count,
+                                   uint32_t* __restrict__ sticky, uint32_t* __restrict__ sticky_count) {
+  if (threadIdx.x == 0 && *status != 0u) {
+    *sticky = 1u;
+    *sticky_count = *count > *sticky_count ? *count : *sticky_count;
+  }
+}
+
+struct WordChunk {
+  uint32_t w[kUploadChunkWords];
+};
+
+__global__ __launch_bounds__(256) void set_chunk(uint32_t* __restrict__ dst, WordChunk chunk, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = chunk.w[i];
+}
+
 }  // namespace
 
 SortPlan sort_plan(uint32_t max_n, uint32_t val_bytes) {
@@ -531,6 +547,27 @@ hipError_t fill_u32_launch(uint32_t* dst, uint32_t value, size_t n, hipStream_t 
   if (blocks > 2048) blocks = 2048;
   fill_u32<<<(uint32_t)blocks, 256, 0, stream>>>(dst, value, n);
   return hipGetLastError();
+}
+
+hipError_t fold_status_launch(const uint32_t* status, const uint32_t* count, uint32_t* sticky,
+                              uint32_t* sticky_count, hipStream_t stream) {
+  fold_status_kernel<<<1, 64, 0, stream>>>(status, count, sticky, sticky_count);
+  return hipGetLastError();
+}
+
+hipError_t upload_launch(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+  if (bytes % 4 != 0) return hipErrorInvalidValue;
+  const uint32_t* s = static_cast<const uint32_t*>(src);
+  uint32_t* d = static_cast<uint32_t*>(dst);
+  for (size_t off = 0; off < bytes / 4; off += kUploadChunkWords) {
+    const uint32_t n = (uint32_t)(bytes / 4 - off < kUploadChunkWords ? bytes / 4 - off : kUploadChunkWords);
+    WordChunk c;
+    for (uint32_t i = 0; i < n; ++i) c.w[i] = s[off + i];
+    set_chunk<<<1, 256, 0, stream>>>(d + off, c, n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t set_words_launch(uint32_t* dst, const Words16& words, uint32_t n, hipStream_t stream) {

@@ -39,6 +39,7 @@ _SIGS = {
     "vkgs_diff_forward": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P]),
     "vkgs_diff_backward": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P]),
     "vkgs_diff_info": (C.c_int, [_P, _P, _P, _P]),
+    "vkgs_diff_check": (C.c_int, [_P, _P, _P]),
 }
 
 
@@ -143,12 +144,16 @@ def _spans(views):
 
 
 class _RenderBatch(torch.autograd.Function):
-    """meta = (views, n_scenes, cfg, flat): with flat the views' images are
-    slices of one (P, 4) tensor, which is the single output."""
+    """meta = (views, n_scenes, cfg, flat, own): with flat the views' images are
+    slices of one (P, 4) tensor, which is the single output.  own = None: a
+    pooled context, and each forward waits for its pair count (an overflowing
+    forward is run again with room for it); own = a context kept by the caller,
+    which checks for overflow later (vkgs_diff_check), so neither pass waits
+    for the device."""
 
     @staticmethod
     def forward(ctx, meta, *params):
-        views, n_scenes, cfg, flat = meta
+        views, n_scenes, cfg, flat, own = meta
         groups = [tuple(_aligned(p.detach().contiguous().float()) for p in params[5 * i:5 * i + 5])
                   for i in range(n_scenes)]
         dev = groups[0][0].device
@@ -159,12 +164,15 @@ class _RenderBatch(torch.autograd.Function):
         else:
             outs = [torch.empty((cam.screen_size[1], cam.screen_size[0], 4), dtype=torch.float32, device=dev)
                     for _, cam, _ in views]
-        c = _pool.get()
+        c = own if own is not None else _pool.get()
         sarr = _scene_structs(groups)
         varr = _view_structs(views, outs)
+        if own is not None:
+            check(_L().vkgs_diff_forward(c.h, sarr, n_scenes, varr, len(views), C.byref(cfg), _stream()),
+                  "vkgs_diff_forward")
         # an overflowing forward reports its whole pair count, and the next forward sizes the pair
         # storage for it, so one retry suffices; a result is never returned from a truncated list
-        for attempt in range(2):
+        for attempt in range(2 if own is None else 0):
             check(_L().vkgs_diff_forward(c.h, sarr, n_scenes, varr, len(views), C.byref(cfg), _stream()),
                   "vkgs_diff_forward")
             vis, pairs = C.c_uint32(), C.c_uint64()
@@ -183,7 +191,7 @@ class _RenderBatch(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *grad_outs):
-        views, n_scenes, cfg, flat = ctx.meta
+        views, n_scenes, cfg, flat, own = ctx.meta
         groups = ctx.groups
         if flat:
             spans, total = _spans(views)
@@ -203,7 +211,8 @@ class _RenderBatch(torch.autograd.Function):
         sarr = _scene_structs(groups, grads)
         varr = _view_structs(views, outs_dummy, gouts)
         check(_L().vkgs_diff_backward(ctx.c.h, sarr, n_scenes, varr, len(views), _stream()), "vkgs_diff_backward")
-        _pool.put(ctx.c)
+        if own is None:
+            _pool.put(ctx.c)
         ctx.c = None
         flat = []
         for g, p in zip([t for gg in grads for t in gg], ctx.saved_tensors):
@@ -225,7 +234,7 @@ def render_batch(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=F
     spatial_order: process the splats of scenes with four or more views in this call in a spatial
     (Morton) order (faster; equal depth keys then blend in that order instead of ascending id)."""
     vs, params = _batch_args(scenes, views)
-    return list(_RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order), False), *params))
+    return list(_RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order), False, None), *params))
 
 
 def render_batch_flat(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False):
@@ -234,8 +243,73 @@ def render_batch_flat(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_or
     (see view_images).  A loss over all views can then be formed with a few whole-batch tensor
     operations instead of a few per view."""
     vs, params = _batch_args(scenes, views)
-    flat = _RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order), True), *params)
+    flat = _RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order), True, None), *params)
     return flat, _spans(vs)[0]
+
+
+class GraphedStep:
+    """A forward + backward step over a fixed batch of views, captured once into a hipGraph and
+    replayed: every launch of both passes, and the elementwise gradient of the loss between them,
+    go to the GPU as one graph launch with no host synchronization.
+
+        step = GraphedStep([params_a, params_b], views, lambda flat: torch.sign(flat - target) * w)
+        step.step()     # replays; p.grad of every parameter tensor holds the step's gradients
+        step.check()    # raises VkgsError if a replay since the last check overflowed its pair list
+
+    grad_fn(flat) -> dL/d(flat) for render_batch_flat's (P, 4) output, made of elementwise tensor
+    operations (no reductions: a reduction's internal memsets are not safe to replay).  The
+    construction runs `warmup` eager steps (sizing every buffer of the step's own context, with the
+    overflow retry of the eager path), then captures.  Each replay overwrites the .grad tensors
+    (they are the graph's outputs, not accumulated).  The views, parameter tensors and the spatial
+    processing order are those of the capture; after check() reports an overflow, build a new step
+    (its warm-up sizes the pair storage for the longer list)."""
+
+    def __init__(self, scenes, views, grad_fn, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False, warmup=2):
+        self.vs, self.params = _batch_args(scenes, views)
+        self.n_scenes = len(scenes)
+        self.cfg = _config(tile, t_min, spatial_order)
+        self.grad_fn = grad_fn
+        self.ctx = _Ctx()
+        self.spans = _spans(self.vs)[0]
+        if True:  # eager warm-up on the current stream, as the captured step will run on it
+            for _ in range(max(warmup, 1)):
+                self._zero_grad()
+                flat = _RenderBatch.apply((self.vs, self.n_scenes, self.cfg, True, None), *self.params)
+                flat.backward(grad_fn(flat.detach()))
+            # the step's own context, sized by eager steps of its own (an overflowing one sizes the
+            # pair storage of the next for its whole list)
+            for attempt in range(2):
+                self._zero_grad()
+                self._run()
+                st = _L().vkgs_diff_check(self.ctx.h, None, _stream())
+                if st == 0:
+                    break
+                if st != 6 or attempt == 1:
+                    check(st, "vkgs_diff_check")
+        torch.cuda.synchronize()
+        self._zero_grad()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.flat = self._run()
+
+    def _zero_grad(self):
+        for p in self.params:
+            p.grad = None
+
+    def _run(self):
+        flat = _RenderBatch.apply((self.vs, self.n_scenes, self.cfg, True, self.ctx), *self.params)
+        flat.backward(self.grad_fn(flat.detach()))
+        return flat
+
+    def step(self):
+        """Replays the captured step; returns the (P, 4) rendered output (overwritten by the next replay)."""
+        self.graph.replay()
+        return self.flat.detach()
+
+    def check(self) -> None:
+        """Synchronous: raises VkgsError (VKGS_ERR_OVERFLOW) if any replay since the last check
+        overflowed its pair list."""
+        check(_L().vkgs_diff_check(self.ctx.h, None, _stream()), "vkgs_diff_check")
 
 
 def view_images(flat, spans):
