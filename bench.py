@@ -40,6 +40,8 @@ def parse_args(argv=None):
     ap.add_argument("--workload", default="c3_6m")
     ap.add_argument("--tile", default="auto", help="WxH, or auto (vkgs_renderer_autotune)")
     ap.add_argument("--t-min", type=float, default=1.0 / 8192.0)
+    ap.add_argument("--triangles", action="store_true",
+                    help="triangle coverage (VKGS_RASTER_TRIANGLES), the reference's rasterization, for its cost")
     ap.add_argument("--timed-frames", type=int, default=20, help="eager frames with per-stage HIP events")
     ap.add_argument("--cpu-frames", type=int, default=2, help="oracle frames for cpu_baseline (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -256,12 +258,12 @@ def main(argv=None) -> int:
     r.load_buffer(sc.ply_buffer(), sc.n)
     out = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device="cuda")
     if args.tile == "auto":
-        r.set_config(8, 8, args.t_min)
+        r.set_config(8, 8, args.t_min, triangles=args.triangles)
         tw, th, tuned_ms = r.autotune(cam, out, frames=10)
         log(f"autotuned tile {tw}x{th} ({tuned_ms:.3f} ms/frame)")
     else:
         tw, th = (int(x) for x in args.tile.split("x"))
-        r.set_config(tw, th, args.t_min)
+        r.set_config(tw, th, args.t_min, triangles=args.triangles)
     for _ in range(3):  # grows the pair storage if the first frame overflowed it
         r.render(cam, out)
         status, visible, pairs = r.frame_info()

@@ -193,6 +193,17 @@ typedef struct vkgs_raster_config {
  * the same and refreshed every 32 forwards. */
 #define VKGS_DIFF_SPATIAL_ORDER 1u
 
+/* vkgs_raster_config.flags bit: rasterize each splat as the reference's
+ * fixed-function pipeline does (two triangles per quad, index pattern
+ * 4i + {0,1,2,2,1,3} (engine.cc:607-616), vertices snapped to 1/256 pixel,
+ * the top-left tie rule, CCW front faces with back faces culled
+ * (graphics_pipeline.cc:67-81), the varying p interpolated from the snapped
+ * vertices (splat.vert:17-25)) instead of the analytic +-3 sigma square.
+ * Slower; a parity option of the plain renderer and vkgs_draw.  Splats with
+ * a quad vertex more than 10^6 pixels from the image origin are not drawn in
+ * this mode. */
+#define VKGS_RASTER_TRIANGLES 2u
+
 /* default: 8x8 tiles, t_min = 1/8192 */
 void vkgs_raster_config_default(vkgs_raster_config* cfg);
 

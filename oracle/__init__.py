@@ -114,6 +114,15 @@ def draw(inst: np.ndarray, width: int, height: int) -> np.ndarray:
     return img
 
 
+def draw_triangles(inst: np.ndarray, width: int, height: int, subpixel_bits: int = 8) -> np.ndarray:
+    """The draw as two rasterized triangles per instance (snapped vertices, top-left rule)."""
+    inst = _f32(inst).reshape(-1, 12)
+    img = np.empty((height, width, 4), np.float32)
+    lib().oracle_draw_triangles(_p(inst), C.c_uint32(inst.shape[0]), C.c_uint32(width), C.c_uint32(height),
+                                C.c_uint32(subpixel_bits), _p(img))
+    return img
+
+
 def draw_bruteforce(inst: np.ndarray, width: int, height: int) -> np.ndarray:
     inst = _f32(inst).reshape(-1, 12)
     img = np.empty((height, width, 4), np.float32)

@@ -603,6 +603,136 @@ void oracle_draw_bruteforce(const float* instances, uint32_t count, uint32_t wid
   }
 }
 
+/* ------------------------------------------------- triangle rasterization --- */
+
+/* The reference's draw as the fixed-function pipeline executes it: each
+ * instance is a quad of two triangles, index pattern 4i + {0,1,2, 2,1,3}
+ * (engine.cc:607-616); vertex k sits at ndc + (rot_scale * q_k) * 3 with
+ * q_k = (2 (k / 2) - 1, 2 (k % 2) - 1) and carries the varying p = 3 q_k
+ * (splat.vert:12-25); the viewport maps x_fb = W/2 x_ndc + W/2 (engine.cc:
+ * 1419-1426); vertices are snapped to 1 / 2^subpixel_bits of a pixel (round
+ * to nearest, Vulkan's subPixelPrecisionBits, 8 on common implementations);
+ * back faces are culled with CCW front faces (graphics_pipeline.cc:72-73),
+ * using Vulkan's signed area a = -1/2 sum(x_i y_i+1 - x_i+1 y_i) in
+ * framebuffer coordinates (y down); a pixel centre is covered when it is
+ * inside all three edges, a centre exactly on an edge only for a top or left
+ * edge (the tie rule), so the shared diagonal is covered once; p is
+ * interpolated linearly from the snapped vertices (w = 1, so perspective
+ * correction is the identity) and alpha = opacity * exp(-p.p / 2)
+ * (splat.frag:9-11); the depth test is LESS against 1.0 with z_fb = z_ndc,
+ * and z < 0 is clipped.  The driver's exact fixed-point and interpolation
+ * arithmetic are not public; this restatement uses exact integer edge
+ * functions and double-precision barycentrics. */
+static int64_t snap_coord(float v, int bits) { return (int64_t)llrint((double)v * (double)(1 << bits)); }
+
+static void draw_triangle(const int64_t X[3], const int64_t Y[3], const float P[3][2], const float* inst,
+                          uint32_t width, uint32_t height, int bits, float* rgba) {
+  /* Vulkan's signed area in framebuffer coordinates (y down): a > 0 is CCW, front */
+  const int64_t cross = (X[1] - X[0]) * (Y[2] - Y[0]) - (Y[1] - Y[0]) * (X[2] - X[0]);
+  /* a = -1/2 (sum x_i y_i+1 - x_i+1 y_i) = -cross / 2 */
+  if (!(-cross > 0)) return; /* back-facing (culled) or degenerate */
+  /* interior-positive edge functions: with cross < 0, E_i(P) = -((B - A) x (P - A)) > 0 inside */
+  int64_t xmin = X[0], xmax = X[0], ymin = Y[0], ymax = Y[0];
+  for (int i = 1; i < 3; ++i) {
+    xmin = X[i] < xmin ? X[i] : xmin;
+    xmax = X[i] > xmax ? X[i] : xmax;
+    ymin = Y[i] < ymin ? Y[i] : ymin;
+    ymax = Y[i] > ymax ? Y[i] : ymax;
+  }
+  const int64_t one = (int64_t)1 << bits, half = one >> 1;
+  int64_t px0 = (xmin - half + one - 1) >> bits, px1 = (xmax - half) >> bits;
+  int64_t py0 = (ymin - half + one - 1) >> bits, py1 = (ymax - half) >> bits;
+  if (px0 < 0) px0 = 0;
+  if (py0 < 0) py0 = 0;
+  if (px1 > (int64_t)width - 1) px1 = (int64_t)width - 1;
+  if (py1 > (int64_t)height - 1) py1 = (int64_t)height - 1;
+  const double area = (double)(-cross);
+  for (int64_t py = py0; py <= py1; ++py)
+    for (int64_t px = px0; px <= px1; ++px) {
+      const int64_t sx = px * one + half, sy = py * one + half;
+      int64_t e[3];
+      int inside = 1;
+      for (int i = 0; i < 3 && inside; ++i) {
+        const int a = i, b = (i + 1) % 3;
+        const int64_t dx = X[b] - X[a], dy = Y[b] - Y[a];
+        e[i] = -(dx * (sy - Y[a]) - dy * (sx - X[a]));
+        /* interior-positive orientation: a left edge has dy > 0 here, a top edge dy == 0, dx < 0
+         * (the negation above mirrors the orientation of the y-down cross product) */
+        const int top_left = (dy > 0) || (dy == 0 && dx < 0);
+        if (e[i] < 0 || (e[i] == 0 && !top_left)) inside = 0;
+      }
+      if (!inside) continue;
+      /* barycentric weight of vertex c is the edge function of the edge opposite c */
+      const double l0 = (double)e[1] / area, l1 = (double)e[2] / area, l2 = (double)e[0] / area;
+      const double p0 = l0 * P[0][0] + l1 * P[1][0] + l2 * P[2][0];
+      const double p1 = l0 * P[0][1] + l1 * P[1][1] + l2 * P[2][1];
+      const float g = expf(-0.5f * (float)(p0 * p0 + p1 * p1));
+      const float a = inst[11] * g;
+      float* dst = rgba + 4 * ((size_t)py * width + (size_t)px);
+      const float om = 1.f - a;
+      dst[0] = inst[8] * a + dst[0] * om;
+      dst[1] = inst[9] * a + dst[1] * om;
+      dst[2] = inst[10] * a + dst[2] * om;
+      dst[3] = a * a + dst[3] * om;
+    }
+}
+
+void oracle_draw_triangles(const float* instances, uint32_t count, uint32_t width, uint32_t height,
+                           uint32_t subpixel_bits, float* rgba) {
+  clear_image(rgba, (size_t)width * height);
+  const int bits = (int)subpixel_bits;
+  const float hw = 0.5f * (float)width, hh = 0.5f * (float)height;
+#pragma omp parallel
+  {
+    int nt = 1, t = 0;
+#ifdef _OPENMP
+    nt = omp_get_num_threads();
+    t = omp_get_thread_num();
+#endif
+    /* each thread draws every splat into its own band of rows, in draw order */
+    const int band = ((int)height + nt - 1) / nt;
+    const int r0 = t * band, r1 = r0 + band < (int)height ? r0 + band : (int)height;
+    for (uint32_t k = 0; k < count && r0 < r1; ++k) {
+      const float* inst = instances + 12 * (size_t)k;
+      const float z = inst[2];
+      if (!(z >= 0.f && z < 1.f)) continue;
+      const float m00 = inst[4], m10 = inst[5], m01 = inst[6], m11 = inst[7];
+      int64_t X[4], Y[4];
+      float P[4][2];
+      int ok = 1;
+      for (int v = 0; v < 4 && ok; ++v) {
+        const float qx = (float)(v / 2) * 2.f - 1.f, qy = (float)(v % 2) * 2.f - 1.f;
+        /* (rot_scale * position) * confidence_radius, splat.vert:19-23 */
+        const float rx = (m00 * qx + m01 * qy) * 3.f, ry = (m10 * qx + m11 * qy) * 3.f;
+        const float xn = inst[0] + rx, yn = inst[1] + ry;
+        const float xf = xn * hw + hw, yf = yn * hh + hh;
+        if (!(isfinite(xf) && isfinite(yf)) || fabsf(xf) > 1e6f || fabsf(yf) > 1e6f) ok = 0;
+        X[v] = ok ? snap_coord(xf, bits) : 0;
+        Y[v] = ok ? snap_coord(yf, bits) : 0;
+        P[v][0] = qx * 3.f;
+        P[v][1] = qy * 3.f;
+      }
+      if (!ok) continue;
+      /* restrict to this thread's rows: the triangle loop clamps to the image, so clamp the band here */
+      const int64_t tri[2][3] = {{0, 1, 2}, {2, 1, 3}};
+      for (int ti = 0; ti < 2; ++ti) {
+        int64_t tx[3], ty[3];
+        float tp[3][2];
+        for (int j = 0; j < 3; ++j) {
+          tx[j] = X[tri[ti][j]];
+          ty[j] = Y[tri[ti][j]];
+          tp[j][0] = P[tri[ti][j]][0];
+          tp[j][1] = P[tri[ti][j]][1];
+        }
+        /* draw into rows [r0, r1) only: offset the image and its height */
+        const int64_t off = (int64_t)r0 << bits;
+        int64_t ty_b[3] = {ty[0] - off, ty[1] - off, ty[2] - off};
+        draw_triangle(tx, ty_b, tp, inst, width, (uint32_t)(r1 - r0), bits, rgba + 4 * (size_t)r0 * width);
+      }
+    }
+  }
+}
+
 uint64_t oracle_bin(const float* instances, uint32_t count, uint32_t width, uint32_t height,
                     uint32_t tile_w, uint32_t tile_h, uint32_t* ranges, uint32_t* list) {
   const uint32_t tx_n = (width + tile_w - 1) / tile_w, ty_n = (height + tile_h - 1) / tile_h;

@@ -98,6 +98,15 @@ void oracle_draw_bruteforce(const float* instances, uint32_t count, uint32_t wid
 uint64_t oracle_bin(const float* instances, uint32_t count, uint32_t width, uint32_t height,
                     uint32_t tile_w, uint32_t tile_h, uint32_t* ranges, uint32_t* list);
 
+/* The draw as the reference's fixed-function pipeline rasterizes it: two
+ * triangles per instance (indices 4i + {0,1,2,2,1,3}), vertices snapped to
+ * 1/2^subpixel_bits of a pixel, the top-left tie rule, CCW front faces with
+ * back faces culled, the varying p interpolated from the snapped vertices;
+ * back-to-front "over" compositing as oracle_draw.  For measuring how far the
+ * analytic coverage of oracle_draw and the HIP raster sits from it. */
+void oracle_draw_triangles(const float* instances, uint32_t count, uint32_t width, uint32_t height,
+                           uint32_t subpixel_bits, float* rgba);
+
 /* Per-pixel evaluation used by every raster path: returns 1 and alpha when the
  * pixel centre (px+0.5, py+0.5) lies in the +-3 sigma square. */
 int oracle_eval_pixel(const oracle_splat2d* s, int px, int py, float* alpha);

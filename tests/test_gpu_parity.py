@@ -382,3 +382,59 @@ def test_renderer_full_size_matches_oracle(name, width, height):
     print("full size image max abs diff", diff, "psnr", psnr(img, ref_img), flush=True)
     assert diff <= 1.0 / 255.0
     assert psnr(img, ref_img) >= 50.0
+
+
+@pytest.mark.parametrize("tile", [(8, 8), (16, 16)])
+def test_triangle_raster_matches_triangle_oracle(c1, tile):
+    """VKGS_RASTER_TRIANGLES (the reference's two snapped triangles per quad, top-left rule) on the 100k
+    workload at 1600x900: coverage decided by exact integer edge functions on both sides, so with every
+    splat composited (t_min = 0) the image is within 1e-5 of the oracle's triangle rasterization; the
+    stage path (vkgs_draw) gives the renderer's image bit for bit."""
+    sc, cam, st, host = c1
+    W, H = 1600, 900
+    r = P.Renderer(sc.n)
+    r.load_buffer(sc.ply_buffer(), sc.n)
+    r.set_config(*tile, t_min=0.0, triangles=True)
+    img = r.render(cam).cpu().numpy()
+    assert r.frame_info()[0] == 0
+    r.close()
+    v, key, index = oracle.rank(cam, host["pos"])
+    _, order = oracle.sort_kv(key, index)
+    inst, _ = oracle.projection(cam, host["pos"], host["cov"], host["opa"], host["sh"], oracle.inverse(order, sc.n), v)
+    ref = oracle.draw_triangles(inst[:v], W, H)
+    d = np.abs(img - ref).max()
+    print("triangle raster vs oracle: max abs", d, flush=True)
+    assert d <= 1e-5
+    cfg = P._lib.RasterConfig()
+    cfg.tile_w, cfg.tile_h, cfg.t_min, cfg.flags = tile[0], tile[1], 0.0, P.RASTER_TRIANGLES
+    stage = P.reference_frame(cam, st, W, H, cfg=cfg)
+    np.testing.assert_array_equal(stage["image"].cpu().numpy(), img)
+
+
+def test_triangle_raster_full_size():
+    """The 6M workload at 1600x900 with triangle coverage at the default early exit: within 1/255 and
+    PSNR >= 50 dB of the oracle's triangle rasterization."""
+    wl = S.WORKLOADS["c3_6m"]
+    sc = _workload_scene("c3_6m")
+    cam = S.orbit_camera(wl.width, wl.height, wl.radius)
+    r = P.Renderer(sc.n)
+    r.load_buffer(sc.ply_buffer(), sc.n)
+    r.set_config(8, 8, triangles=True)
+    for _ in range(3):
+        img = r.render(cam)
+        if r.frame_info()[0] == 0:
+            break
+    img = img.cpu().numpy()
+    r.close()
+    st = P.parse_ply(sc.ply_buffer(), sc.n)
+    torch.cuda.synchronize()
+    pos, cov, opa = st.position.cpu().numpy(), st.cov3d.cpu().numpy(), st.opacity.cpu().numpy()
+    sh = st.sh.cpu().numpy().view(np.uint16)
+    del st
+    v, key, index = oracle.rank(cam, pos)
+    _, order = oracle.sort_kv(key, index)
+    inst, _ = oracle.projection(cam, pos, cov, opa, sh, oracle.inverse(order, sc.n), v)
+    ref = oracle.draw_triangles(inst[:v], wl.width, wl.height)
+    d = np.abs(img - ref).max()
+    print("full size triangle raster: max abs", d, "psnr", psnr(img, ref), flush=True)
+    assert d <= 1.0 / 255.0 and psnr(img, ref) >= 50.0

@@ -353,3 +353,69 @@ def test_draw_rectangle_is_conservative_and_bins_match_bruteforce():
             sub = oracle.draw_bruteforce(inst[lst[ranges[t, 0]:ranges[t, 1]]], 64, 48)
             img[y0:y0 + th, x0:x0 + tw] = sub[y0:y0 + th, x0:x0 + tw]
         np.testing.assert_array_equal(img, a)
+
+
+# ----------------------------------------------------------- triangle-exact draw ---
+
+def test_triangle_draw_square_edges_follow_the_top_left_rule():
+    """An axis-aligned quad whose edges fall on pixel centres (centre (32.5, 32.5), half extent 6 px, all
+    vertices on the 1/256 grid): the left and top edges' pixels are covered, the right and bottom edges'
+    are not (the tie rule); inside, the interpolated p is the exact affine map, so alpha matches
+    o exp(-|p|^2 / 2) with p = (x - c) / sigma, sigma = 2 px."""
+    W = H = 64
+    c = 2 * 32.5 / W - 1
+    s = 0.0625  # 3 s W / 2 = 6 px
+    inst = _inst(c, c, [[s, 0], [0, s]], (0.2, 0.4, 0.8), 0.7)[None]
+    img = oracle.draw_triangles(inst, W, H)
+    cov = img[..., 3] != 1.0
+    rows, cols = np.nonzero(cov)
+    assert (cols.min(), cols.max(), rows.min(), rows.max()) == (26, 37, 26, 37)
+    for y, x in [(32, 32), (30, 35), (26, 26), (37, 29)]:
+        p = np.array([x + 0.5 - 32.5, y + 0.5 - 32.5]) / 2.0
+        a = 0.7 * math.exp(-0.5 * float(p @ p))
+        np.testing.assert_allclose(img[y, x, 0], 0.2 * a, rtol=2e-6)
+
+
+def test_triangle_draw_covers_the_shared_diagonal_once():
+    """Pixel centres on the diagonal the two triangles share are covered by exactly one of them: with a
+    uniform alpha the diagonal matches its neighbours' single blend, not a double one."""
+    W = H = 64
+    c = 2 * 32.5 / W - 1
+    s = 0.0625
+    inst = _inst(c, c, [[s, 0], [0, s]], (1.0, 1.0, 1.0), 0.5)[None]
+    img = oracle.draw_triangles(inst, W, H)
+    # the vertices 1 = (-1, +1) and 2 = (+1, -1) in q: the diagonal x - 32.5 = -(y - 32.5)
+    for d in range(-5, 6):
+        x, y = 32 + d, 32 - d
+        p = np.array([x + 0.5 - 32.5, y + 0.5 - 32.5]) / 2.0
+        a = 0.5 * math.exp(-0.5 * float(p @ p))
+        np.testing.assert_allclose(img[y, x, 0], a, rtol=2e-6)
+        np.testing.assert_allclose(img[y, x, 3], a * a + (1 - a), rtol=2e-6)
+
+
+def test_triangle_draw_rejects_depth_and_nan_cases():
+    W = H = 32
+    c = 2 * 16.5 / W - 1
+    s = 0.2
+    for inst in (_inst(c, c, [[s, 0], [0, s]], (1, 1, 1), 0.7, z=1.0),
+                 _inst(c, c, [[s, 0], [0, s]], (1, 1, 1), 0.7, z=-1e-6),
+                 _inst(c, c, [[float("nan")] * 2] * 2, (1, 1, 1), 0.7)):
+        img = oracle.draw_triangles(inst[None], W, H)
+        assert np.all(img[..., 3] == 1.0) and np.all(img[..., :3] == 0.0)
+
+
+def test_triangle_draw_agrees_with_the_analytic_draw_away_from_edges():
+    """On a small random scene the two rasterizations differ only where the snapped, interpolated quad
+    differs from the analytic square: pixels within a small margin of a quad edge, and the last bits of
+    alpha from the 1/256-pixel vertex snap."""
+    from vkgs_amd import scene as S
+    sc = S.gaussian_cloud(300, seed=23, pos_std=1.0, logscale=(-2.0, 0.2))
+    cam = S.orbit_camera(64, 48, 4.0)
+    pos, cov, opa, sh = oracle.parse_ply(sc.rows, S.standard_offsets())
+    v, key, index = oracle.rank(cam, pos)
+    _, order = oracle.sort_kv(key, index)
+    inst, _ = oracle.projection(cam, pos, cov, opa, sh, oracle.inverse(order, sc.n), v)
+    a = oracle.draw(inst[:v], 64, 48)
+    t = oracle.draw_triangles(inst[:v], 64, 48)
+    d = np.abs(a - t).max(-1)
+    assert (d > 1e-3).mean() < 0.05 and d.max() < 0.05

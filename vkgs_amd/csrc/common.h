@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "vkgs_hip.h"
 
@@ -29,6 +30,7 @@ struct FrameConst {
   float inv_w2, inv_h2; // (1/W)/W and (1/H)/H, the low-pass terms (projection.comp:115-116)
   uint32_t width, height;
   uint32_t tile_w, tile_h, tiles_x, tiles_y;
+  uint32_t triangles;  // records in the triangle-raster layout (VKGS_RASTER_TRIANGLES)
 };
 
 VKHD void m4_mul(const float* a, const float* b, float* c) {
@@ -211,6 +213,27 @@ VKHD bool setup_splat(const float* inst, uint32_t width, uint32_t height, Splat2
   s->y0 = (int)loy;
   s->y1 = (int)hiy;
   return true;
+}
+
+// The 48-byte raster record of one instance (f = its 12 floats), set up as s
+// (setup_splat; an empty rectangle when nothing is drawn).  Analytic layout:
+// (cx, cy, i00, i01), (i10, i11, opacity, x0 | x1 << 16), (r, g, b, y0 | y1 << 16)
+// in framebuffer pixels.  Triangle layout (the quad is rebuilt per tile from
+// the instance's own numbers): (ndc x, ndc y, rot_scale m00, m10),
+// (m01, m11, opacity, x rect), (r, g, b, y rect).
+VKHD void raster_record(const float* f, const Splat2D& s, bool triangles, float4* out) {
+  const uint32_t rx = (uint32_t)s.x0 | ((uint32_t)s.x1 << 16), ry = (uint32_t)s.y0 | ((uint32_t)s.y1 << 16);
+  float fx, fy;
+  memcpy(&fx, &rx, 4);
+  memcpy(&fy, &ry, 4);
+  if (triangles) {
+    out[0] = make_float4(f[0], f[1], f[4], f[5]);
+    out[1] = make_float4(f[6], f[7], s.o, fx);
+  } else {
+    out[0] = make_float4(s.cx, s.cy, s.i00, s.i01);
+    out[1] = make_float4(s.i10, s.i11, s.o, fx);
+  }
+  out[2] = make_float4(s.r, s.g, s.b, fy);
 }
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }

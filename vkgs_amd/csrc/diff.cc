@@ -206,6 +206,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   c->h_scene_blocks[n_scenes] = (uint32_t)sblocks;
   RasterShape rs0;
   if (!make_raster_shape(16, 16, cfg, &rs0)) return VKGS_ERR_INVALID_ARG;
+  if (rs0.triangles) return VKGS_ERR_INVALID_ARG;  // the differentiable raster is analytic only
   c->h_views.assign(n_views, DiffView{});
   c->h_view_blocks.assign(n_views + 1, 0u);
   c->h_view_tiles.assign(n_views + 1, 0u);

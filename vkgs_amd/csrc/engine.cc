@@ -345,6 +345,7 @@ int vkgs_renderer_render(vkgs_renderer* r, const vkgs_camera* camera, const floa
   // never waits for the previous frame before writing them
   FrameConst fc;
   make_frame_const(camera, model, &fc);
+  fc.triangles = r->rs.triangles;
   fc.tile_w = r->rs.tile_w;
   fc.tile_h = r->rs.tile_h;
   fc.tiles_x = r->rs.tiles_x;

@@ -62,7 +62,8 @@ struct RasterShape {
   uint32_t width, height, tile_w, tile_h, tiles_x, tiles_y;
   uint32_t tile_shift_x, tile_shift_y;
   float t_min;
-  uint32_t variant;  // 0: LDS batches, 1: scalar-load waves
+  uint32_t variant;    // 0: LDS batches, 1: scalar-load waves
+  uint32_t triangles;  // 1: triangle coverage (VKGS_RASTER_TRIANGLES)
 };
 
 bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config* cfg, RasterShape* rs);

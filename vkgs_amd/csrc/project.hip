@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
     project_splat(fc, position, cov3d, opacity, sh, id, o);
     const float f[12] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w,
                          o[2].x, o[2].y, o[2].z, o[2].w};
-    Splat2D s;
+    Splat2D s = {};
     uint4 r = make_uint4(0u, 0u, 0u, 0u);
     RectBits rb;
     rect_bits(fc.tiles_x, fc.tiles_y, &rb);
@@ -306,9 +306,7 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
       s.y0 = 1;
       s.y1 = 0;
     }
-    rec[3 * (size_t)slot + 0] = make_float4(s.cx, s.cy, s.i00, s.i01);
-    rec[3 * (size_t)slot + 1] = make_float4(s.i10, s.i11, s.o, __uint_as_float(pack16(s.x0, s.x1)));
-    rec[3 * (size_t)slot + 2] = make_float4(s.r, s.g, s.b, __uint_as_float(pack16(s.y0, s.y1)));
+    raster_record(f, s, fc.triangles != 0, rec + 3 * (size_t)slot);
     if (kPacked) {
       val64[slot] = (uint64_t)slot | ((uint64_t)packed << 32);
     } else {

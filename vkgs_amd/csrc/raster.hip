@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const float4* __restrict
     const uint32_t idx = order ? order[k] : k;
     const float4 a = inst[3 * (size_t)idx], b = inst[3 * (size_t)idx + 1], c = inst[3 * (size_t)idx + 2];
     const float f[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
-    Splat2D s;
+    Splat2D s = {};
     uint32_t count = 0;
     TileRect r = {0u, 0u};
     if (setup_splat(f, rs.width, rs.height, &s)) {
@@ -62,9 +62,7 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const float4* __restrict
       s.y0 = 1;
       s.y1 = 0;
     }
-    rec[3 * (size_t)k + 0] = make_float4(s.cx, s.cy, s.i00, s.i01);
-    rec[3 * (size_t)k + 1] = make_float4(s.i10, s.i11, s.o, __uint_as_float(pack16(s.x0, s.x1)));
-    rec[3 * (size_t)k + 2] = make_float4(s.r, s.g, s.b, __uint_as_float(pack16(s.y0, s.y1)));
+    raster_record(f, s, rs.triangles != 0, rec + 3 * (size_t)k);
     rect[k] = r;
     cnt[k] = count;
   }
@@ -430,6 +428,160 @@ __global__ __launch_bounds__(kWave) void rasterize_pair_kernel(const float4* __r
   if (work && l == 0 && has_tile) work[slot] = h ? walked_b : walked_a;
 }
 
+// ------------------------------------------------- triangle-exact raster ---
+
+constexpr int kSnapBits = 8;             // vertices snapped to 1/256 pixel
+constexpr float kMaxVertexPixels = 1e6f; // quads with a vertex farther out are not drawn
+
+__device__ __forceinline__ int64_t snap(float v) { return (int64_t)rintf(v * (float)(1 << kSnapBits)); }
+
+// One splat's quad restaged for one tile: for each of its two triangles
+// (vertices 0,1,2 and 2,1,3) the three edge functions at the centre of the
+// tile's first pixel (exact, in 1/256-pixel fixed point, as doubles: every
+// edge value that can decide coverage is an integer below 2^53) and their
+// steps per pixel, a tie bit per edge (top or left edge: a centre exactly on
+// it is covered), and 1 / area for the barycentric weights.
+struct TriStage {
+  double e0[2][3], ex[2][3], ey[2][3], inv_area[2];
+  uint32_t ties;  // bit 3 t + i: edge i of triangle t takes ties; bit 6 + t: triangle t is front-facing
+};
+
+__device__ __forceinline__ void stage_triangles(const float4& p0, const float4& p1, bool valid, int px0, int py0,
+                                                float hw, float hh, TriStage* st) {
+  st->ties = 0;
+  if (!valid) return;
+  const float cx = p0.x, cy = p0.y, m00 = p0.z, m10 = p0.w, m01 = p1.x, m11 = p1.y;
+  int64_t X[4], Y[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const float qx = (float)(v / 2) * 2.f - 1.f, qy = (float)(v % 2) * 2.f - 1.f;
+    const float rx = (m00 * qx + m01 * qy) * 3.f, ry = (m10 * qx + m11 * qy) * 3.f;  // splat.vert:19-23
+    const float xf = (cx + rx) * hw + hw, yf = (cy + ry) * hh + hh;                   // viewport
+    if (!(fabsf(xf) <= kMaxVertexPixels && fabsf(yf) <= kMaxVertexPixels)) return;
+    X[v] = snap(xf);
+    Y[v] = snap(yf);
+  }
+  const int64_t one = (int64_t)1 << kSnapBits, half = one >> 1;
+  const int64_t sx = (int64_t)px0 * one + half, sy = (int64_t)py0 * one + half;
+  const int tri[2][3] = {{0, 1, 2}, {2, 1, 3}};
+  uint32_t ties = 0;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int a = tri[t][0], b = tri[t][1], c = tri[t][2];
+    const int64_t cross = (X[b] - X[a]) * (Y[c] - Y[a]) - (Y[b] - Y[a]) * (X[c] - X[a]);
+    if (!(-cross > 0)) continue;  // back-facing (culled) or degenerate: Vulkan area -cross / 2
+    ties |= 1u << (6 + t);
+    st->inv_area[t] = 1.0 / (double)(-cross);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int va = tri[t][i], vb = tri[t][(i + 1) % 3];
+      const int64_t dx = X[vb] - X[va], dy = Y[vb] - Y[va];
+      st->e0[t][i] = (double)(-dx * (sy - Y[va]) + dy * (sx - X[va]));
+      st->ex[t][i] = (double)(dy * one);
+      st->ey[t][i] = (double)(-dx * one);
+      if (dy > 0 || (dy == 0 && dx < 0)) ties |= 1u << (3 * t + i);
+    }
+  }
+  st->ties = ties;
+}
+
+// Coverage and alpha of pixel (lx, ly) of the tile (offsets from its first
+// pixel) by a staged quad: at most one triangle covers a centre (the shared
+// edge goes to one of them by the tie rule).  p is interpolated from the
+// vertices' 3 q and alpha = opacity * exp(-p.p / 2) (splat.frag:9-11).
+__device__ __forceinline__ float triangle_alpha(const TriStage& st, float opacity, double lx, double ly) {
+  // varying p = 3 q of the vertices of each triangle, in its vertex order
+  const float P[2][3][2] = {{{-3.f, -3.f}, {-3.f, 3.f}, {3.f, -3.f}}, {{3.f, -3.f}, {-3.f, 3.f}, {3.f, 3.f}}};
+  float alpha = 0.f;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    if (!((st.ties >> (6 + t)) & 1u)) continue;
+    double e[3];
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      e[i] = fma(st.ex[t][i], lx, fma(st.ey[t][i], ly, st.e0[t][i]));
+      const bool tie = (st.ties >> (3 * t + i)) & 1u;
+      in = in && (e[i] > 0.0 || (e[i] == 0.0 && tie));
+    }
+    if (in) {
+      // the weight of a vertex is the edge function of the edge opposite it
+      const double l0 = e[1] * st.inv_area[t], l1 = e[2] * st.inv_area[t], l2 = e[0] * st.inv_area[t];
+      const double q0 = l0 * P[t][0][0] + l1 * P[t][1][0] + l2 * P[t][2][0];
+      const double q1 = l0 * P[t][0][1] + l1 * P[t][1][1] + l2 * P[t][2][1];
+      alpha = opacity * gauss_falloff((float)(q0 * q0 + q1 * q1));
+    }
+  }
+  return alpha;
+}
+
+// The raster with triangle coverage: one workgroup per tile, one lane per
+// pixel, records staged per tile in LDS, front-to-back compositing with the
+// same early exit as rasterize_kernel.
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void rasterize_tri_kernel(const float4* __restrict__ rec,
+                                                               const uint32_t* __restrict__ vals,
+                                                               const uint32_t* __restrict__ ranges, RasterShape rs,
+                                                               const uint32_t* __restrict__ tile_order,
+                                                               float4* __restrict__ out,
+                                                               uint32_t* __restrict__ work) {
+  constexpr int kN = TW * TH;
+  constexpr int kB = 32;  // records per batch
+  constexpr int kRowsPerWave = kWave / TW;
+  __shared__ TriStage s_tri[kB];
+  __shared__ float4 s_col[kB];
+  __shared__ uint32_t s_rect[kB][2];
+
+  const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
+  const uint32_t tx = tile % rs.tiles_x, ty = tile / rs.tiles_x;
+  const int t = threadIdx.x;
+  const int lx = t % TW, ly = t / TW;
+  const int px0 = (int)tx * TW, py0 = (int)ty * TH;
+  const int px = px0 + lx, py = py0 + ly;
+  const bool inside = px < (int)rs.width && py < (int)rs.height;
+  const int col0 = __builtin_amdgcn_readfirstlane(px0), col1 = col0 + TW - 1;
+  const int row0 = __builtin_amdgcn_readfirstlane(py0 + (t / kWave) * kRowsPerWave);
+  const int row1 = row0 + kRowsPerWave - 1;
+  const float hw = 0.5f * (float)rs.width, hh = 0.5f * (float)rs.height;
+  const uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
+  const uint32_t items = end - start;
+  const float t_min = rs.t_min;
+  float T = inside ? 1.f : 0.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
+  uint32_t walked = 0;
+  for (uint32_t b0 = 0; b0 < items; b0 += kB) {
+    const int cnt = (int)min((uint32_t)kB, items - b0);
+    __syncthreads();
+    if (t < cnt) {
+      const uint32_t k = vals[end - 1 - (b0 + t)];
+      const float4 p0 = rec[3 * (size_t)k], p1 = rec[3 * (size_t)k + 1], p2 = rec[3 * (size_t)k + 2];
+      stage_triangles(p0, p1, true, px0, py0, hw, hh, &s_tri[t]);
+      s_col[t] = make_float4(p2.x, p2.y, p2.z, p1.z);
+      s_rect[t][0] = __float_as_uint(p1.w);
+      s_rect[t][1] = __float_as_uint(p2.w);
+    }
+    if (__syncthreads_count(T >= t_min) == 0) break;
+    for (int e = 0; e < cnt; ++e) {
+      const uint32_t rx = __builtin_amdgcn_readfirstlane(s_rect[e][0]);
+      const uint32_t ry = __builtin_amdgcn_readfirstlane(s_rect[e][1]);
+      if ((int)(ry >> 16) < row0 || (int)(ry & 0xffffu) > row1 || (int)(rx >> 16) < col0 ||
+          (int)(rx & 0xffffu) > col1)
+        continue;
+      const float4 c = s_col[e];
+      const float alpha = triangle_alpha(s_tri[e], c.w, (double)lx, (double)ly);
+      const float ta = T * alpha;
+      cr = fmaf(c.x, ta, cr);
+      cg = fmaf(c.y, ta, cg);
+      cb = fmaf(c.z, ta, cb);
+      aa = fmaf(alpha, ta, aa);
+      T = T - ta;
+      ++walked;
+      if (__ballot(T >= t_min) == 0) break;
+    }
+  }
+  if (inside) out[(size_t)py * rs.width + px] = make_float4(cr, cg, cb, aa + T);
+  if (work && t % kWave == 0) work[(size_t)blockIdx.x * (kN / kWave) + t / kWave] = walked;
+}
+
 // Scalar-broadcast raster: every wave walks its tile's list on its own.  The
 // list entry and its 48-byte record have wave-uniform addresses, so they are
 // fetched with scalar loads into SGPRs once per wave (no LDS traffic, no
@@ -527,6 +679,7 @@ bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config
   if (!(cfg->t_min >= 0.f && cfg->t_min < 1.f)) return false;
   if (cfg->variant > 1) return false;
   rs->variant = cfg->variant;
+  rs->triangles = (cfg->flags & VKGS_RASTER_TRIANGLES) ? 1u : 0u;
   rs->width = width;
   rs->height = height;
   rs->tile_w = tw;
@@ -607,6 +760,23 @@ hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint3
     else                                                                                           \
       rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out, work);   \
     return hipGetLastError();                                                                      \
+  }
+  if (rs.triangles) {
+#define VKGS_TRI_CASE(W_, H_)                                                                        \
+  if (tw == W_ && th == H_) {                                                                         \
+    rasterize_tri_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, tile_order, out, work); \
+    return hipGetLastError();                                                                         \
+  }
+    VKGS_TRI_CASE(8, 8)
+    VKGS_TRI_CASE(16, 16)
+    VKGS_TRI_CASE(32, 8)
+    VKGS_TRI_CASE(16, 8)
+    VKGS_TRI_CASE(8, 16)
+    VKGS_TRI_CASE(16, 4)
+    VKGS_TRI_CASE(32, 4)
+    VKGS_TRI_CASE(64, 4)
+#undef VKGS_TRI_CASE
+    return hipErrorInvalidValue;
   }
   if (tw == 8 && th == 8 && rs.variant == 0) {
     rasterize_pair_kernel<<<(tiles + 1) / 2, kWave, 0, s>>>(rec, vals, ranges, rs, tile_order, out, work);

@@ -17,6 +17,7 @@ from ._lib import check, lib, ptr
 
 IDENTITY = np.eye(4, dtype=np.float32).ravel()
 STAGES = ["count", "project", "depth_sort", "bin", "tile_sort", "raster", "total"]
+RASTER_TRIANGLES = 2  # VKGS_RASTER_TRIANGLES
 
 
 def _stream() -> int:
@@ -212,9 +213,12 @@ class Renderer:
             _copy_device(t.data_ptr(), p.value, t.numel() * t.element_size())
         return out
 
-    def set_config(self, tile_w=16, tile_h=16, t_min=1.0 / 8192.0) -> None:
+    def set_config(self, tile_w=16, tile_h=16, t_min=1.0 / 8192.0, triangles=False) -> None:
+        """Tile shape, early-exit threshold, and triangles=True for the reference's triangle coverage
+        (VKGS_RASTER_TRIANGLES) instead of the analytic square."""
         cfg = _lib.RasterConfig()
         cfg.tile_w, cfg.tile_h, cfg.t_min = tile_w, tile_h, t_min
+        cfg.flags = RASTER_TRIANGLES if triangles else 0
         check(lib().vkgs_renderer_set_config(self._h, C.byref(cfg)), "vkgs_renderer_set_config")
 
     def render(self, cam, out: torch.Tensor | None = None, model=None) -> torch.Tensor:
