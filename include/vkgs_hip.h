@@ -239,6 +239,25 @@ int vkgs_renderer_destroy(vkgs_renderer* r);
 /* Engine::LoadSplats (engine.cc:541-544) + parse (engine.cc:1137-1161):
  * synchronous load of a PLY file into the splat storage. */
 int vkgs_renderer_load_ply(vkgs_renderer* r, const char* path);
+/* Asynchronous load (SplatLoadThread, splat_load_thread.cc:33-204, and
+ * Engine::LoadSplatsAsync / LoadSplats, engine.cc:541-549): a loader thread
+ * reads the file in 65536-row chunks into pinned memory and copies it to the
+ * device; the next vkgs_renderer_render after it finishes parses it into the
+ * splat storage on the render stream (frames before that show the previous
+ * splats).  A new call cancels a load in progress; vkgs_renderer_cancel_load
+ * cancels too (a canceled or failed load changes nothing). */
+typedef enum vkgs_load_state {
+  VKGS_LOAD_IDLE = 0,
+  VKGS_LOAD_READING = 1,  /* header, rows or upload in progress */
+  VKGS_LOAD_READY = 2,    /* on the device, taken by the next render */
+  VKGS_LOAD_DONE = 3,     /* parsed into the splat storage */
+  VKGS_LOAD_CANCELED = 4,
+  VKGS_LOAD_FAILED = 5    /* vkgs_renderer_load_progress returns the error */
+} vkgs_load_state;
+int vkgs_renderer_load_ply_async(vkgs_renderer* r, const char* path);
+int vkgs_renderer_cancel_load(vkgs_renderer* r);
+/* Rows in the file and rows read so far, and the vkgs_load_state. */
+int vkgs_renderer_load_progress(vkgs_renderer* r, uint64_t* total, uint64_t* loaded, int* state);
 /* Load from a host ply buffer (60 offsets + rows), as produced by vkgs_ply_read. */
 int vkgs_renderer_load_ply_buffer(vkgs_renderer* r, const void* host_ply, uint32_t n);
 uint32_t vkgs_renderer_splat_count(const vkgs_renderer* r);

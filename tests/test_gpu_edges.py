@@ -245,3 +245,71 @@ def test_stage_timing_sums_to_total_and_graph_replay_resumes():
     print("graph replay", replay, "ms/frame; timed eager", timed, flush=True)
     assert replay < timed
     r.close()
+
+
+def test_async_load_then_render_and_cancel(tmp_path):
+    """vkgs_renderer_load_ply_async (SplatLoadThread + Engine::LoadSplatsAsync): the load runs on the
+    renderer's loader thread while frames keep the previous splats; the first frame after it is ready
+    equals a synchronous load's frame.  A new request cancels the one in progress (the later file wins);
+    cancel_load leaves the splats as they were; a missing file fails with VKGS_ERR_IO."""
+    import time
+    a = S.gaussian_cloud(300_000, seed=66, pos_std=1.0, logscale=(-3.0, 0.3))
+    b = S.gaussian_cloud(20_000, seed=67, pos_std=1.0, logscale=(-2.5, 0.3))
+    pa, pb = os.path.join(tmp_path, "a.ply"), os.path.join(tmp_path, "b.ply")
+    a.write_ply(pa)
+    b.write_ply(pb)
+    cam = S.orbit_camera(128, 96, 4.0)
+
+    def frame(rr):
+        for _ in range(2):  # a frame that outgrew the pair storage is rendered again
+            img = rr.render(cam)
+            if rr.frame_info()[0] == 0:
+                return img.cpu().numpy()
+        raise AssertionError("pair overflow twice")
+
+    def wait(r, want):
+        t0 = time.time()
+        while time.time() - t0 < 60:
+            img = r.render(cam)  # takes a finished load
+            total, loaded, state = r.load_progress()
+            if state == want:
+                return img, total, loaded
+            time.sleep(0.005)
+        raise AssertionError(f"load state {state}, wanted {want}")
+
+    r = P.Renderer(a.n)
+    ref = P.Renderer(a.n)
+    ref.load_buffer(b.ply_buffer(), b.n)
+    want_b = frame(ref)
+    ref.load_buffer(a.ply_buffer(), a.n)
+    want_a = frame(ref)
+    ref.close()
+    r.load_ply_async(pa)
+    r.load_ply_async(pb)  # supersedes a
+    img, total, loaded = wait(r, "done")
+    assert r.n == b.n and total == b.n and loaded == b.n
+    np.testing.assert_array_equal(frame(r), want_b)
+    r.load_ply_async(pa)
+    img, total, loaded = wait(r, "done")
+    assert r.n == a.n
+    np.testing.assert_array_equal(frame(r), want_a)
+    r.load_ply_async(pb)
+    r.cancel_load()
+    for _ in range(20):
+        r.render(cam)
+    assert r.load_progress()[2] in ("canceled", "done")
+    if r.load_progress()[2] == "canceled":
+        assert r.n == a.n
+    r.load_ply_async(os.path.join(tmp_path, "missing.ply"))
+    t0 = time.time()
+    while time.time() - t0 < 30:
+        try:
+            st = r.load_progress()[2]
+        except _lib.VkgsError as e:
+            assert e.status == 4  # VKGS_ERR_IO
+            break
+        time.sleep(0.005)
+    else:
+        raise AssertionError("missing file did not fail")
+    assert r.n in (a.n, b.n)
+    r.close()

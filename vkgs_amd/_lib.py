@@ -101,6 +101,9 @@ SIGNATURES = {
     "vkgs_renderer_destroy": (_I, [_P]),
     "vkgs_renderer_load_ply": (_I, [_P, C.c_char_p]),
     "vkgs_renderer_load_ply_buffer": (_I, [_P, _P, _U32]),
+    "vkgs_renderer_load_ply_async": (_I, [_P, C.c_char_p]),
+    "vkgs_renderer_cancel_load": (_I, [_P]),
+    "vkgs_renderer_load_progress": (_I, [_P, _P, _P, _P]),
     "vkgs_renderer_splat_count": (_U32, [_P]),
     "vkgs_renderer_storage": (_I, [_P, _P, _P, _P, _P]),
     "vkgs_renderer_set_config": (_I, [_P, _P]),

@@ -10,8 +10,14 @@
 // counters), so no host sync happens in a frame.  When the tile grid allows
 // (rect_bits), the tile rectangle travels through the depth sort packed into
 // the value's high word instead of being gathered in depth order.
+#include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+#include <condition_variable>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -47,6 +53,31 @@ struct DevBuf {
   T* as() const {
     return static_cast<T*>(p);
   }
+};
+
+// Background PLY loader (the SplatLoadThread of splat_load_thread.cc:33-204
+// and Engine::LoadSplatsAsync, engine.cc:541-549): a thread reads the file in
+// 65536-row chunks into pinned host memory, checking for cancellation between
+// chunks, copies the ply buffer to the device on its own stream and publishes
+// it; the next render parses it into the splat storage on the render stream,
+// so the swap is ordered with the frames around it.  A canceled or failed load
+// publishes nothing and the renderer keeps its splats.
+struct Loader {
+  std::thread thread;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::string pending;
+  bool has_pending = false;
+  bool terminate = false;
+  std::atomic<bool> cancel{false};
+  hipStream_t stream = nullptr;
+  // progress (under mu)
+  uint64_t total = 0, loaded = 0;
+  int state = 0;  // vkgs_load_state
+  int error = 0;
+  // the published buffer (under mu): device ply buffer of ready_n rows
+  void* ready = nullptr;
+  uint32_t ready_n = 0;
 };
 
 }  // namespace
@@ -89,6 +120,9 @@ struct vkgs_renderer {
   bool timing = false;
   hipEvent_t ev[kEvents] = {};
   float stage_ms[kStages] = {};
+
+  Loader loader;
+  void* ply_in_use = nullptr;  // the last parsed async ply buffer (freed when replaced)
 };
 
 namespace {
@@ -106,7 +140,9 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
   const uint32_t n = r->n;
   RasterShape rs;
   if (!make_raster_shape(width, height, &r->cfg, &rs)) return hipErrorInvalidValue;
-  if (r->max_pairs == 0) r->max_pairs = (uint64_t)(n ? n : 1) * 4 + (1u << 20);
+  // at least 4 pairs per splat (more after a frame overflowed), also when a load made n larger
+  const uint64_t base_pairs = (uint64_t)(n ? n : 1) * 4 + (1u << 20);
+  if (r->max_pairs < base_pairs) r->max_pairs = base_pairs;
   if (r->grow_pairs) {  // room for the overflowing frame's whole list, plus an eighth
     const uint64_t need = r->need_pairs + r->need_pairs / 8 + (1u << 20);
     r->max_pairs = need > r->max_pairs * 2 ? need : r->max_pairs * 2;
@@ -175,6 +211,126 @@ DrawBuffers draw_buffers(vkgs_renderer* r) {
   b.tile_sort_storage = r->tile_sort.p;
   b.work = r->timing ? r->work.as<uint32_t>() : nullptr;
   return b;
+}
+
+void loader_main(vkgs_renderer* r) {
+  Loader& L = r->loader;
+  for (;;) {
+    std::string path;
+    {
+      std::unique_lock<std::mutex> g(L.mu);
+      L.cv.wait(g, [&] { return L.terminate || L.has_pending; });
+      if (L.terminate) return;
+      path = L.pending;
+      L.has_pending = false;
+      L.cancel = false;
+      L.total = L.loaded = 0;
+      L.state = VKGS_LOAD_READING;
+      L.error = 0;
+    }
+    // status: a vkgs_status error, or kCanceled
+    constexpr int kCanceled = -1;
+    auto fail = [&](int status) {
+      std::lock_guard<std::mutex> g(L.mu);
+      if (L.has_pending) return;  // superseded by a newer request, which the loop takes next
+      L.state = status == kCanceled ? VKGS_LOAD_CANCELED : VKGS_LOAD_FAILED;
+      L.error = status == kCanceled ? 0 : status;
+    };
+    vkgs_ply_info info;
+    int st = vkgs_ply_read_header(path.c_str(), &info);
+    if (st != VKGS_OK) {
+      fail(st);
+      continue;
+    }
+    if (info.point_count > r->capacity) {
+      fail(VKGS_ERR_CAPACITY);
+      continue;
+    }
+    const uint32_t n = (uint32_t)info.point_count;
+    {
+      std::lock_guard<std::mutex> g(L.mu);
+      L.total = n;
+    }
+    const size_t row_bytes = (size_t)info.stride * 4, bytes = 240 + (size_t)n * row_bytes;
+    void* host = nullptr;
+    if (hipHostMalloc(&host, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+      fail(VKGS_ERR_HIP);
+      continue;
+    }
+    memcpy(host, info.offsets, 240);
+    FILE* f = fopen(path.c_str(), "rb");
+    bool ok = f != nullptr && fseek(f, (long)info.header_bytes, SEEK_SET) == 0;
+    constexpr uint32_t kChunk = 65536;  // rows per read, as splat_load_thread.cc:142
+    for (uint32_t start = 0; ok && start < n; start += kChunk) {
+      if (L.cancel || L.terminate) break;
+      const uint32_t rows = n - start < kChunk ? n - start : kChunk;
+      ok = fread(static_cast<char*>(host) + 240 + (size_t)start * row_bytes, row_bytes, rows, f) == rows;
+      std::lock_guard<std::mutex> g(L.mu);
+      L.loaded = start + rows;
+    }
+    if (f) fclose(f);
+    if (L.cancel || L.terminate || !ok) {
+      (void)hipHostFree(host);
+      fail(!ok ? VKGS_ERR_IO : kCanceled);
+      if (L.terminate) return;
+      continue;
+    }
+    void* dev = nullptr;
+    hipError_t e = hipMalloc(&dev, bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, L.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
+    (void)hipHostFree(host);
+    if (e != hipSuccess || L.cancel) {
+      if (dev) (void)hipFree(dev);
+      fail(e != hipSuccess ? VKGS_ERR_HIP : kCanceled);
+      continue;
+    }
+    std::lock_guard<std::mutex> g(L.mu);
+    if (L.has_pending) {  // superseded while uploading
+      (void)hipFree(dev);
+      continue;
+    }
+    if (L.ready) (void)hipFree(L.ready);  // an earlier load nobody rendered yet
+    L.ready = dev;
+    L.ready_n = n;
+    L.state = VKGS_LOAD_READY;
+  }
+}
+
+// A synchronous load supersedes any asynchronous one: cancel it and drop a
+// published buffer nobody took.
+void supersede_async_load(vkgs_renderer* r) {
+  Loader& L = r->loader;
+  std::lock_guard<std::mutex> g(L.mu);
+  L.cancel = true;
+  L.has_pending = false;
+  if (L.ready) {
+    (void)hipFree(L.ready);
+    L.ready = nullptr;
+  }
+  if (L.state == VKGS_LOAD_READING || L.state == VKGS_LOAD_READY) L.state = VKGS_LOAD_CANCELED;
+}
+
+// Takes a published async load: parses it into the splat storage on stream s.
+hipError_t take_async_load(vkgs_renderer* r, hipStream_t s) {
+  Loader& L = r->loader;
+  void* dev = nullptr;
+  uint32_t n = 0;
+  {
+    std::lock_guard<std::mutex> g(L.mu);
+    if (!L.ready) return hipSuccess;
+    dev = L.ready;
+    n = L.ready_n;
+    L.ready = nullptr;
+    L.state = VKGS_LOAD_DONE;
+  }
+  if (r->ply_in_use) (void)hipFree(r->ply_in_use);  // parsed by an earlier frame
+  r->ply_in_use = dev;
+  hipError_t e = launch_parse_ply(dev, n, r->position.as<float>(), r->cov3d.as<float>(), r->opacity.as<float>(),
+                                  r->sh.as<uint16_t>(), s);
+  r->n = n;
+  drop_graph(r);
+  return e;
 }
 
 // Records the frame on stream s (eager, or under capture).
@@ -248,7 +404,19 @@ int vkgs_renderer_create(uint32_t capacity, vkgs_renderer** out) {
 
 int vkgs_renderer_destroy(vkgs_renderer* r) {
   if (!r) return VKGS_ERR_INVALID_ARG;
+  if (r->loader.thread.joinable()) {
+    {
+      std::lock_guard<std::mutex> g(r->loader.mu);
+      r->loader.terminate = true;
+      r->loader.cancel = true;
+    }
+    r->loader.cv.notify_one();
+    r->loader.thread.join();
+  }
   (void)hipDeviceSynchronize();
+  if (r->loader.ready) hipFree(r->loader.ready);
+  if (r->ply_in_use) hipFree(r->ply_in_use);
+  if (r->loader.stream) hipStreamDestroy(r->loader.stream);
   drop_graph(r);
   DevBuf* bufs[] = {&r->work,       &r->position,   &r->cov3d,    &r->opacity,    &r->sh,        &r->fc,
                     &r->misc,
@@ -267,6 +435,7 @@ int vkgs_renderer_destroy(vkgs_renderer* r) {
 int vkgs_renderer_load_ply_buffer(vkgs_renderer* r, const void* host_ply, uint32_t n) {
   if (!r || (!host_ply && n)) return VKGS_ERR_INVALID_ARG;
   if (n > r->capacity) return VKGS_ERR_CAPACITY;
+  supersede_async_load(r);
   if (n == 0) {
     r->n = 0;
     drop_graph(r);
@@ -290,6 +459,47 @@ int vkgs_renderer_load_ply_buffer(vkgs_renderer* r, const void* host_ply, uint32
   r->n = n;
   drop_graph(r);
   return VKGS_OK;
+}
+
+int vkgs_renderer_load_ply_async(vkgs_renderer* r, const char* path) {
+  if (!r || !path) return VKGS_ERR_INVALID_ARG;
+  Loader& L = r->loader;
+  if (!L.thread.joinable()) {
+    VKGS_HIP_CHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    L.thread = std::thread(loader_main, r);
+  }
+  {
+    std::lock_guard<std::mutex> g(L.mu);
+    L.cancel = true;  // a load in progress stops at its next chunk (Engine::LoadSplats, engine.cc:541-544)
+    L.pending = path;
+    L.has_pending = true;
+    L.state = VKGS_LOAD_READING;
+    L.total = L.loaded = 0;
+  }
+  L.cv.notify_one();
+  return VKGS_OK;
+}
+
+int vkgs_renderer_cancel_load(vkgs_renderer* r) {
+  if (!r) return VKGS_ERR_INVALID_ARG;
+  Loader& L = r->loader;
+  std::lock_guard<std::mutex> g(L.mu);
+  L.cancel = true;
+  if (L.has_pending) {
+    L.has_pending = false;
+    L.state = VKGS_LOAD_CANCELED;
+  }
+  return VKGS_OK;
+}
+
+int vkgs_renderer_load_progress(vkgs_renderer* r, uint64_t* total, uint64_t* loaded, int* state) {
+  if (!r) return VKGS_ERR_INVALID_ARG;
+  Loader& L = r->loader;
+  std::lock_guard<std::mutex> g(L.mu);
+  if (total) *total = L.total;
+  if (loaded) *loaded = L.loaded;
+  if (state) *state = L.state;
+  return L.state == VKGS_LOAD_FAILED ? L.error : VKGS_OK;
 }
 
 int vkgs_renderer_load_ply(vkgs_renderer* r, const char* path) {
@@ -333,6 +543,7 @@ int vkgs_renderer_render(vkgs_renderer* r, const vkgs_camera* camera, const floa
   RasterShape rs;
   if (!make_raster_shape(width, height, &r->cfg, &rs)) return VKGS_ERR_INVALID_ARG;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  VKGS_HIP_CHECK(take_async_load(r, s));
   const vkgs_renderer::Key key = {r->n, width, height, r->cfg.tile_w, r->cfg.tile_h, r->cfg.t_min,
                                   r->max_pairs, d_rgba};
   const bool same = r->graph_valid && memcmp(&key, &r->graph_key, sizeof(key)) == 0 && !r->grow_pairs;

@@ -18,6 +18,7 @@ from ._lib import check, lib, ptr
 IDENTITY = np.eye(4, dtype=np.float32).ravel()
 STAGES = ["count", "project", "depth_sort", "bin", "tile_sort", "raster", "total"]
 RASTER_TRIANGLES = 2  # VKGS_RASTER_TRIANGLES
+LOAD_STATES = ("idle", "reading", "ready", "done", "canceled", "failed")  # vkgs_load_state
 
 
 def _stream() -> int:
@@ -193,6 +194,21 @@ class Renderer:
 
     def load_ply(self, path: str) -> None:
         check(lib().vkgs_renderer_load_ply(self._h, path.encode()), "vkgs_renderer_load_ply")
+
+    def load_ply_async(self, path: str) -> None:
+        """Starts loading a PLY file on the renderer's loader thread (a load in progress is canceled);
+        the first render after it finishes shows it."""
+        check(lib().vkgs_renderer_load_ply_async(self._h, path.encode()), "vkgs_renderer_load_ply_async")
+
+    def cancel_load(self) -> None:
+        check(lib().vkgs_renderer_cancel_load(self._h), "vkgs_renderer_cancel_load")
+
+    def load_progress(self):
+        """(rows in the file, rows read, state), state one of LOAD_STATES; raises for a failed load."""
+        t, l, st = C.c_uint64(), C.c_uint64(), C.c_int()
+        rc = lib().vkgs_renderer_load_progress(self._h, C.byref(t), C.byref(l), C.byref(st))
+        check(rc, "vkgs_renderer_load_progress")
+        return int(t.value), int(l.value), LOAD_STATES[st.value]
 
     def load_buffer(self, ply_buffer: np.ndarray, n: int) -> None:
         buf = np.ascontiguousarray(ply_buffer)
