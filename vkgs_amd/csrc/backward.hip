@@ -870,6 +870,336 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   }
 }
 
+// ------------------------------------------- two 8x8 tiles per wave ---
+//
+// The 8x8 forward and backward rasters also run with two tiles per wave:
+// lanes 0-31 take tile order[2i], lanes 32-63 tile order[2i + 1], each lane
+// two pixels of its tile (rows r and r + 4).  A record read from LDS then
+// serves two pixels, and in the backward each of the nine per-entry sums
+// first adds the lane's two pixels and then runs the same three DPP steps
+// over half rows of 8 lanes, so the reduction costs half as much per pixel.
+// Consecutive tiles of the longest-first schedule have list lengths within a
+// bucket of each other; a half whose list is shorter walks records that
+// touch none of its pixels.  The per-pixel arithmetic is that of the one-tile
+// kernels, so images and gradients are the same bit for bit.
+
+constexpr int kPairHalf = kWave / 2;  // lanes per tile
+#ifndef VKGS_DIFF_PAIR_TILES
+#define VKGS_DIFF_PAIR_TILES 1
+#endif
+constexpr bool kPairTiles = VKGS_DIFF_PAIR_TILES != 0;  // 8x8 tiles two per wave (0: one per workgroup)
+// Two tiles per wave halve the launch's waves: below this many tiles (a few
+// waves per CU) the one-tile kernels keep more waves in flight and are faster.
+constexpr uint32_t kPairMinTiles = 8192;
+
+// Maximum of v over the 32 lanes of each half wave (every lane gets its half's).
+__device__ __forceinline__ uint32_t half_max(uint32_t v) {
+#pragma unroll
+  for (int d = 1; d < kPairHalf; d <<= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, kWave));
+  return v;
+}
+
+// The forward of diff_raster_kernel<8, 8> for two tiles per wave.
+__global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, const float4* __restrict__ rec,
+                                                                 const uint32_t* __restrict__ vals,
+                                                                 const uint32_t* __restrict__ ranges,
+                                                                 const uint32_t* __restrict__ pslot,
+                                                                 const uint32_t* __restrict__ tile_order,
+                                                                 int keep_state) {
+  constexpr int kStride = kPairHalf + 1;  // the second tile's records start one float4 later: other banks
+  __shared__ float4 s_rec[2][3][2 * kStride];
+  const int t = threadIdx.x, h = t / kPairHalf, l = t % kPairHalf;
+  const uint32_t slot = 2 * blockIdx.x + h;
+  const bool has_tile = slot < L.n_tiles;
+  const uint32_t gtile = has_tile ? (tile_order ? tile_order[slot] : slot) : 0u;
+  const DiffView& view = L.views[range_of(L.view_tiles, L.n_views, gtile)];
+  const uint32_t tile = gtile - view.tile_base;
+  const uint32_t W = view.fc.width, H = view.fc.height;
+  const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
+  const int px = (int)tx * 8 + l % 8, py0 = (int)ty * 8 + l / 8, py1 = py0 + 4;
+  const bool in0 = has_tile && px < (int)W && py0 < (int)H, in1 = has_tile && px < (int)W && py1 < (int)H;
+  uint32_t end = 0, items = 0;
+  if (has_tile) {
+    end = ranges[2 * gtile + 1];
+    items = end - ranges[2 * gtile];
+  }
+  const uint32_t items_a = __builtin_amdgcn_readlane(items, 0), items_b = __builtin_amdgcn_readlane(items, kPairHalf);
+  const uint32_t batches = (max(items_a, items_b) + kPairHalf - 1) / kPairHalf;
+  const float fx = (float)px + 0.5f, fy0 = (float)py0 + 0.5f, fy1 = (float)py1 + 0.5f;
+  const float t_min = L.t_min;
+
+  float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
+  if ((uint32_t)l < items) {
+    const uint32_t k = pslot[vals[end - 1 - l]];
+    p0 = rec[3 * (size_t)k];
+    p1 = rec[3 * (size_t)k + 1];
+    p2 = rec[3 * (size_t)k + 2];
+  }
+  float T0 = in0 ? 1.f : 0.f, r0 = 0.f, g0 = 0.f, b0 = 0.f, a0 = 0.f;
+  float T1 = in1 ? 1.f : 0.f, r1 = 0.f, g1 = 0.f, b1 = 0.f, a1 = 0.f;
+  uint32_t last0 = items, last1 = items;
+  bool alive0 = T0 >= t_min, alive1 = T1 >= t_min;
+  for (uint32_t b = 0; b < batches; ++b) {
+    const int buf = b & 1;
+    // entries past a half's list (the other tile's is longer) are staged with opacity 0: alpha is 0, so
+    // they change no pixel
+    const bool valid = b * kPairHalf + l < items;
+    s_rec[buf][0][h * kStride + l] = p0;
+    s_rec[buf][1][h * kStride + l] = valid ? p1 : make_float4(0.f, 0.f, 0.f, 0.f);
+    s_rec[buf][2][h * kStride + l] = p2;
+    if (__syncthreads_count(alive0 | alive1) == 0) break;
+    const uint32_t next = (b + 1) * kPairHalf + l;
+    if (next < items) {
+      const uint32_t k = pslot[vals[end - 1 - next]];
+      p0 = rec[3 * (size_t)k];
+      p1 = rec[3 * (size_t)k + 1];
+      p2 = rec[3 * (size_t)k + 2];
+    }
+    const uint32_t done = b * kPairHalf;
+    const int cnt_a = items_a > done ? (int)min((uint32_t)kPairHalf, items_a - done) : 0;
+    const int cnt_b = items_b > done ? (int)min((uint32_t)kPairHalf, items_b - done) : 0;
+    const int cnt = max(cnt_a, cnt_b);
+    if (__ballot(alive0 | alive1) == 0) continue;
+    const float4* s0 = s_rec[buf][0] + h * kStride;
+    const float4* s1 = s_rec[buf][1] + h * kStride;
+    const float4* s2 = s_rec[buf][2] + h * kStride;
+    for (int e = 0; e < cnt; ++e) {
+      const float4 a = s0[e], bb = s1[e], c = s2[e];
+      const uint32_t idx = b * kPairHalf + e + 1;  // entries processed once this one is
+      {
+        const float dx = fx - a.x, dy = fy0 - a.y;
+        const float ux = fmaf(a.z, dx, a.w * dy);
+        const float uy = fmaf(bb.x, dx, bb.y * dy);
+        const bool in = (fabsf(ux) <= 3.f) & (fabsf(uy) <= 3.f) & alive0;
+        const float q = fmaf(ux, ux, uy * uy);
+        const float alpha = in ? fminf(bb.z * gauss_falloff(q), kAlphaMax) : 0.f;
+        const float ta = T0 * alpha;
+        r0 = fmaf(c.x, ta, r0);
+        g0 = fmaf(c.y, ta, g0);
+        b0 = fmaf(c.z, ta, b0);
+        a0 = fmaf(alpha, ta, a0);
+        T0 = T0 - ta;
+        const bool was = alive0;
+        alive0 = T0 >= t_min;
+        if (was && !alive0) last0 = idx;
+      }
+      {
+        const float dx = fx - a.x, dy = fy1 - a.y;
+        const float ux = fmaf(a.z, dx, a.w * dy);
+        const float uy = fmaf(bb.x, dx, bb.y * dy);
+        const bool in = (fabsf(ux) <= 3.f) & (fabsf(uy) <= 3.f) & alive1;
+        const float q = fmaf(ux, ux, uy * uy);
+        const float alpha = in ? fminf(bb.z * gauss_falloff(q), kAlphaMax) : 0.f;
+        const float ta = T1 * alpha;
+        r1 = fmaf(c.x, ta, r1);
+        g1 = fmaf(c.y, ta, g1);
+        b1 = fmaf(c.z, ta, b1);
+        a1 = fmaf(alpha, ta, a1);
+        T1 = T1 - ta;
+        const bool was = alive1;
+        alive1 = T1 >= t_min;
+        if (was && !alive1) last1 = idx;
+      }
+      if (__ballot(alive0 | alive1) == 0) break;
+    }
+  }
+  if (in0) {
+    const size_t pix = (size_t)py0 * W + px;
+    view.out[pix] = make_float4(r0, g0, b0, a0 + T0);
+    if (keep_state) {
+      view.final_t[pix] = T0;
+      view.last[pix] = last0;
+    }
+  }
+  if (in1) {
+    const size_t pix = (size_t)py1 * W + px;
+    view.out[pix] = make_float4(r1, g1, b1, a1 + T1);
+    if (keep_state) {
+      view.final_t[pix] = T1;
+      view.last[pix] = last1;
+    }
+  }
+}
+
+// Backward of one (pixel, entry): the arithmetic of diff_raster_backward_kernel
+// for one pixel; adds its nine words to w[] (lanes outside the footprint or
+// past the pixel's last entry add zeros and keep T and acc).
+struct PixState {
+  float T, acc, gw2;
+  float4 g;
+  uint32_t last;
+};
+
+struct PixCover {
+  float dx, dy, ux, uy;
+  bool in;
+};
+
+__device__ __forceinline__ PixCover bwd_cover(const PixState& ps, const float4& a, const float4& bb, float fx, float fy,
+                                              uint32_t i) {
+  PixCover pc;
+  pc.dx = fx - a.x;
+  pc.dy = fy - a.y;
+  pc.ux = fmaf(a.z, pc.dx, a.w * pc.dy);
+  pc.uy = fmaf(bb.x, pc.dx, bb.y * pc.dy);
+  pc.in = (i < ps.last) & (fabsf(pc.ux) <= 3.f) & (fabsf(pc.uy) <= 3.f);
+  return pc;
+}
+
+__device__ __forceinline__ void bwd_pixel(PixState& ps, const PixCover& pc, const float4& bb, const float4& c,
+                                          float w[9]) {
+  const float dx = pc.dx, dy = pc.dy, ux = pc.ux, uy = pc.uy;
+  const bool in = pc.in;
+  const float q = fmaf(ux, ux, uy * uy);
+  const float G = in ? gauss_falloff(q) : 0.f;
+  const float raw = bb.z * G;
+  const float alpha = fminf(raw, kAlphaMax);
+  const float om = 1.f - alpha;
+  ps.T = ps.T * __builtin_amdgcn_rcpf(om);
+  const float ta = ps.T * alpha;
+  const float gc = fmaf(ps.g.z, c.z, fmaf(ps.g.y, c.y, ps.g.x * c.x));
+  const float dalpha = ps.T * (fmaf(ps.gw2, alpha, gc) - ps.acc);
+  ps.acc = fmaf(alpha, fmaf(ps.g.w, alpha, gc), om * ps.acc);
+  const float dop = raw > kAlphaMax ? 0.f : dalpha * G;
+  const float dpx = dop * dx, dpy = dop * dy;
+  w[0] += ps.g.x * ta;
+  w[1] += ps.g.y * ta;
+  w[2] += ps.g.z * ta;
+  w[3] += dop;
+  w[4] += dpx * dx;
+  w[5] += dpx * dy;
+  w[6] += dpy * dy;
+  w[7] += dpx;
+  w[8] += dpy;
+}
+
+// The backward of diff_raster_backward_kernel<8, 8> for two tiles per wave.
+__global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
+    DiffLaunch L, const float4* __restrict__ rec, const uint32_t* __restrict__ vals,
+    const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ pslot, const uint32_t* __restrict__ tile_order,
+    float* __restrict__ gpart) {
+  constexpr int kB = VKGS_BWD_BATCH;  // list entries per batch of each tile
+  constexpr int kSlots = kPairHalf / 8;  // half rows of 8 lanes per tile
+  __shared__ float4 s_rec[2][3][kB];
+  __shared__ float s_g[2][9][kSlots][kB + 1];
+  __shared__ uint32_t s_j[2][kB];
+
+  const int t = threadIdx.x, h = t / kPairHalf, l = t % kPairHalf;
+  const uint32_t slot = 2 * blockIdx.x + h;
+  const bool has_tile = slot < L.n_tiles;
+  const uint32_t gtile = has_tile ? (tile_order ? tile_order[slot] : slot) : 0u;
+  const DiffView& view = L.views[range_of(L.view_tiles, L.n_views, gtile)];
+  const uint32_t tile = gtile - view.tile_base;
+  const uint32_t W = view.fc.width, H = view.fc.height;
+  const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
+  const int px = (int)tx * 8 + l % 8, py0 = (int)ty * 8 + l / 8, py1 = py0 + 4;
+  const bool in0 = has_tile && px < (int)W && py0 < (int)H, in1 = has_tile && px < (int)W && py1 < (int)H;
+  uint32_t end = 0, items = 0;
+  if (has_tile) {
+    end = ranges[2 * gtile + 1];
+    items = end - ranges[2 * gtile];
+  }
+  const float fx = (float)px + 0.5f, fy0 = (float)py0 + 0.5f, fy1 = (float)py1 + 0.5f;
+  const bool half_last = (__lane_id() & 7u) == 7u;  // the last lane of each half row of 8
+  const uint32_t row = (uint32_t)l >> 3;           // half-row slot 0..3 of the tile
+
+  PixState s0 = {1.f, 0.f, 0.f, make_float4(0.f, 0.f, 0.f, 0.f), 0u}, s1 = s0;
+  if (in0) {
+    const size_t pix = (size_t)py0 * W + px;
+    s0.last = view.last[pix];
+    s0.T = view.final_t[pix];
+    s0.g = view.grad_out[pix];
+  }
+  if (in1) {
+    const size_t pix = (size_t)py1 * W + px;
+    s1.last = view.last[pix];
+    s1.T = view.final_t[pix];
+    s1.g = view.grad_out[pix];
+  }
+  // g . (colour, alpha) composited behind the current entry (as in the one-tile kernel)
+  s0.acc = s0.g.w;
+  s0.gw2 = 2.f * s0.g.w;
+  s1.acc = s1.g.w;
+  s1.gw2 = 2.f * s1.g.w;
+  const uint32_t max_half = half_max(max(s0.last, s1.last));
+  const uint32_t max_a = __builtin_amdgcn_readlane(max_half, 0), max_b = __builtin_amdgcn_readlane(max_half, kPairHalf);
+  const uint32_t max_all = max(max_a, max_b);
+
+  // traversal index i (0 = nearest) maps to list position end - 1 - i; walk i downwards
+  for (uint32_t hi = max_all; hi > 0;) {
+    const uint32_t lo = hi > (uint32_t)kB ? hi - kB : 0u;
+    const int cnt = (int)(hi - lo);
+    __syncthreads();
+    if (l < cnt) {
+      const uint32_t i = hi - 1 - l;
+      float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
+      uint32_t j = 0;
+      if (i < max_half) {  // an entry some pixel of this tile composited
+        j = vals[end - 1 - i];
+        const uint32_t k = pslot[j];
+        r0 = rec[3 * (size_t)k];
+        r1 = rec[3 * (size_t)k + 1];
+        r2 = rec[3 * (size_t)k + 2];
+      }
+      s_j[h][l] = j;
+      s_rec[h][0][l] = r0;
+      s_rec[h][1][l] = r1;
+      s_rec[h][2][l] = r2;
+#pragma unroll
+      for (int w = 0; w < 9; ++w)
+#pragma unroll
+        for (int r = 0; r < kSlots; ++r) s_g[h][w][r][l] = 0.f;
+    }
+    __syncthreads();
+    for (int e = 0; e < cnt; ++e) {
+      const uint32_t i = hi - 1 - e;
+      const float4 a = s_rec[h][0][e], bb = s_rec[h][1][e], c = s_rec[h][2][e];
+      const PixCover c0 = bwd_cover(s0, a, bb, fx, fy0, i), c1 = bwd_cover(s1, a, bb, fx, fy1, i);
+      if (__ballot(c0.in | c1.in) == 0) continue;  // no pixel of either tile: the slots stay zero
+      float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      bwd_pixel(s0, c0, bb, c, w);
+      bwd_pixel(s1, c1, bb, c, w);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) w[k] = half_row_sum(w[k]);
+      // keep the last DPP add ahead of the branch (every lane computes it)
+      __asm__ volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3]), "+v"(w[4]), "+v"(w[5]), "+v"(w[6]),
+                       "+v"(w[7]), "+v"(w[8]));
+      if (half_last) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s_g[h][k][row][e] = w[k];
+      }
+    }
+    __syncthreads();
+    if (l < cnt && hi - 1 - l < max_half) {
+      const float4 a = s_rec[h][0][l], bb = s_rec[h][1][l];
+      const float P = a.z * a.z + bb.x * bb.x, Q = a.z * a.w + bb.x * bb.y, R = a.w * a.w + bb.y * bb.y;
+      float u[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) u[k] = (s_g[h][k][0][l] + s_g[h][k][1][l]) + (s_g[h][k][2][l] + s_g[h][k][3][l]);
+      const float hq = -0.5f * bb.z;
+#pragma unroll
+      for (int k = 4; k < 9; ++k) u[k] *= hq;
+      float* out = gpart + 9 * (size_t)s_j[h][l];
+      out[0] = u[0];
+      out[1] = u[1];
+      out[2] = u[2];
+      out[3] = u[3];
+      out[4] = u[4];
+      out[5] = 2.f * u[5];
+      out[6] = u[6];
+      out[7] = -2.f * (P * u[7] + Q * u[8]);
+      out[8] = -2.f * (Q * u[7] + R * u[8]);
+    }
+    hi = lo;
+  }
+  // entries past every pixel's last one in this tile: zero partials (each pair written once)
+  for (uint32_t i = max_half + l; i < items; i += kPairHalf) {
+    float* out = gpart + 9 * (size_t)vals[end - 1 - i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[k] = 0.f;
+  }
+}
+
 // Sum of x over the 64 lanes of a wave (all lanes active); lane 63 gets it.
 __device__ __forceinline__ float wave_total_last(float x) {
   x = row_sum16(x);
@@ -1363,6 +1693,11 @@ hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const
 hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
                               const uint32_t* pslot, const uint32_t* tile_order, bool keep_state, hipStream_t s) {
   if (L.n_tiles == 0) return hipSuccess;
+  if (L.tile_w == 8 && L.tile_h == 8 && kPairTiles && L.n_tiles >= kPairMinTiles) {
+    diff_raster_pair_kernel<<<(L.n_tiles + 1) / 2, kWave, 0, s>>>(L, rec, vals, ranges, pslot, tile_order,
+                                                                  keep_state ? 1 : 0);
+    return hipGetLastError();
+  }
 #define VKGS_CASE(W_, H_)                                                                                  \
   if (L.tile_w == W_ && L.tile_h == H_) {                                                                  \
     diff_raster_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, pslot, tile_order,      \
@@ -1382,6 +1717,11 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
   // partials of pairs no pixel reached stay zero
   if (L.n_tiles > 0) {
     bool launched = false;
+    if (L.tile_w == 8 && L.tile_h == 8 && kPairTiles && L.n_tiles >= kPairMinTiles) {
+      diff_raster_backward_pair_kernel<<<(L.n_tiles + 1) / 2, kWave, 0, s>>>(L, rec, vals, ranges, pslot, tile_order,
+                                                                           gpart);
+      launched = true;
+    }
 #define VKGS_CASE(W_, H_)                                                                                  \
   if (!launched && L.tile_w == W_ && L.tile_h == H_) {                                                     \
     diff_raster_backward_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, pslot,         \

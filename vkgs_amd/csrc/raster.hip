@@ -778,7 +778,8 @@ hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint3
 #undef VKGS_TRI_CASE
     return hipErrorInvalidValue;
   }
-  if (tw == 8 && th == 8 && rs.variant == 0) {
+  // two tiles per wave halve the launch's waves: only where there are enough tiles to fill the GPU
+  if (tw == 8 && th == 8 && rs.variant == 0 && tiles >= 8192) {
     rasterize_pair_kernel<<<(tiles + 1) / 2, kWave, 0, s>>>(rec, vals, ranges, rs, tile_order, out, work);
     return hipGetLastError();
   }

@@ -271,22 +271,26 @@ class GraphedStep:
         self.grad_fn = grad_fn
         self.ctx = _Ctx()
         self.spans = _spans(self.vs)[0]
-        # eager warm-up on the current stream, on which the captured step will run
-        for _ in range(max(warmup, 1)):
-            self._zero_grad()
-            flat = _RenderBatch.apply((self.vs, self.n_scenes, self.cfg, True, None), *self.params)
-            flat.backward(grad_fn(flat.detach()))
-        # the step's own context, sized by eager steps of its own (an overflowing one sizes the
-        # pair storage of the next for its whole list)
-        for attempt in range(2):
-            self._zero_grad()
-            self._run()
-            st = _L().vkgs_diff_check(self.ctx.h, None, _stream())
-            if st == 0:
-                break
-            if st != 6 or attempt == 1:
-                check(st, "vkgs_diff_check")
-        torch.cuda.synchronize()
+        # eager warm-up on a side stream, as torch's graph capture requires (work of the warm-up on
+        # the default stream before a capture breaks it)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(max(warmup, 1)):
+                self._zero_grad()
+                flat = _RenderBatch.apply((self.vs, self.n_scenes, self.cfg, True, None), *self.params)
+                flat.backward(grad_fn(flat.detach()))
+            # the step's own context, sized by eager steps of its own (an overflowing one sizes the
+            # pair storage of the next for its whole list)
+            for attempt in range(2):
+                self._zero_grad()
+                self._run()
+                st = _L().vkgs_diff_check(self.ctx.h, None, _stream())
+                if st == 0:
+                    break
+                if st != 6 or attempt == 1:
+                    check(st, "vkgs_diff_check")
+        torch.cuda.current_stream().wait_stream(side)
         self._zero_grad()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
