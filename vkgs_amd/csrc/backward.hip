@@ -881,7 +881,9 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
 // Consecutive tiles of the longest-first schedule have list lengths within a
 // bucket of each other; a half whose list is shorter walks records that
 // touch none of its pixels.  The per-pixel arithmetic is that of the one-tile
-// kernels, so images and gradients are the same bit for bit.
+// kernels, so images are the same bit for bit; the backward adds a lane's two
+// pixels' gradient words with FMAs before the half-row sums, so its partials
+// agree with the one-tile kernel's to float rounding.
 
 constexpr int kPairHalf = kWave / 2;  // lanes per tile
 #ifndef VKGS_DIFF_PAIR_TILES
@@ -1021,9 +1023,9 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
   }
 }
 
-// Backward of one (pixel, entry): the arithmetic of diff_raster_backward_kernel
-// for one pixel; adds its nine words to w[] (lanes outside the footprint or
-// past the pixel's last entry add zeros and keep T and acc).
+// Backward of one (pixel, entry) with the arithmetic of
+// diff_raster_backward_kernel: a pixel outside the footprint or past its last
+// entry gets G = 0, so its terms are zero and T and acc stay.
 struct PixState {
   float T, acc, gw2;
   float4 g;
@@ -1046,31 +1048,28 @@ __device__ __forceinline__ PixCover bwd_cover(const PixState& ps, const float4& 
   return pc;
 }
 
-__device__ __forceinline__ void bwd_pixel(PixState& ps, const PixCover& pc, const float4& bb, const float4& c,
-                                          float w[9]) {
-  const float dx = pc.dx, dy = pc.dy, ux = pc.ux, uy = pc.uy;
-  const bool in = pc.in;
-  const float q = fmaf(ux, ux, uy * uy);
-  const float G = in ? gauss_falloff(q) : 0.f;
+// The per-pixel terms of one entry: ta = T alpha (the colour words' weight),
+// dop = dL/d(opacity term) and dop dx, dop dy; updates T and acc.
+struct PixTerms {
+  float ta, dop, dpx, dpy;
+};
+
+__device__ __forceinline__ PixTerms bwd_pixel(PixState& ps, const PixCover& pc, const float4& bb, const float4& c) {
+  const float q = fmaf(pc.ux, pc.ux, pc.uy * pc.uy);
+  const float G = pc.in ? gauss_falloff(q) : 0.f;
   const float raw = bb.z * G;
   const float alpha = fminf(raw, kAlphaMax);
   const float om = 1.f - alpha;
   ps.T = ps.T * __builtin_amdgcn_rcpf(om);
-  const float ta = ps.T * alpha;
+  PixTerms r;
+  r.ta = ps.T * alpha;
   const float gc = fmaf(ps.g.z, c.z, fmaf(ps.g.y, c.y, ps.g.x * c.x));
   const float dalpha = ps.T * (fmaf(ps.gw2, alpha, gc) - ps.acc);
   ps.acc = fmaf(alpha, fmaf(ps.g.w, alpha, gc), om * ps.acc);
-  const float dop = raw > kAlphaMax ? 0.f : dalpha * G;
-  const float dpx = dop * dx, dpy = dop * dy;
-  w[0] += ps.g.x * ta;
-  w[1] += ps.g.y * ta;
-  w[2] += ps.g.z * ta;
-  w[3] += dop;
-  w[4] += dpx * dx;
-  w[5] += dpx * dy;
-  w[6] += dpy * dy;
-  w[7] += dpx;
-  w[8] += dpy;
+  r.dop = raw > kAlphaMax ? 0.f : dalpha * G;
+  r.dpx = r.dop * pc.dx;
+  r.dpy = r.dop * pc.dy;
+  return r;
 }
 
 // The backward of diff_raster_backward_kernel<8, 8> for two tiles per wave.
@@ -1156,9 +1155,18 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
       const float4 a = s_rec[h][0][e], bb = s_rec[h][1][e], c = s_rec[h][2][e];
       const PixCover c0 = bwd_cover(s0, a, bb, fx, fy0, i), c1 = bwd_cover(s1, a, bb, fx, fy1, i);
       if (__ballot(c0.in | c1.in) == 0) continue;  // no pixel of either tile: the slots stay zero
-      float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      bwd_pixel(s0, c0, bb, c, w);
-      bwd_pixel(s1, c1, bb, c, w);
+      const PixTerms t0 = bwd_pixel(s0, c0, bb, c), t1 = bwd_pixel(s1, c1, bb, c);
+      // the lane's two pixels summed (FMAs), then over its half row
+      float w[9];
+      w[0] = fmaf(s1.g.x, t1.ta, s0.g.x * t0.ta);
+      w[1] = fmaf(s1.g.y, t1.ta, s0.g.y * t0.ta);
+      w[2] = fmaf(s1.g.z, t1.ta, s0.g.z * t0.ta);
+      w[3] = t0.dop + t1.dop;
+      w[4] = fmaf(t1.dpx, c1.dx, t0.dpx * c0.dx);
+      w[5] = fmaf(t1.dpx, c1.dy, t0.dpx * c0.dy);
+      w[6] = fmaf(t1.dpy, c1.dy, t0.dpy * c0.dy);
+      w[7] = t0.dpx + t1.dpx;
+      w[8] = t0.dpy + t1.dpy;
 #pragma unroll
       for (int k = 0; k < 9; ++k) w[k] = half_row_sum(w[k]);
       // keep the last DPP add ahead of the branch (every lane computes it)
