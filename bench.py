@@ -45,7 +45,9 @@ def parse_args(argv=None):
     ap.add_argument("--timed-frames", type=int, default=20, help="eager frames with per-stage HIP events")
     ap.add_argument("--cpu-frames", type=int, default=2, help="oracle frames for cpu_baseline (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="timed fwd+bwd steps of the batch workload (0 = skip)")
+    ap.add_argument("--fwd-bwd-steps", type=int, default=5, help="timed fwd+bwd steps of the batch workload (0 = skip)")
+    ap.add_argument("--no-fwd-bwd-tune", action="store_true",
+                    help="batched fwd+bwd forms with the default configuration instead of vkgs_diff_autotune's")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scales every scene size of the batch workload")
     ap.add_argument("--fwd-bwd-mode", choices=("both", "plain", "batched", "batched_eager"), default="both",
                     help="which fwd+bwd forms to time (one form alone is for profiling it)")
@@ -150,12 +152,14 @@ def bench_c4(steps: int):
             "c4_visible": D.last_info["visible"], "c4_pairs": D.last_info["pairs"]}
 
 
-def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
+def bench_fwd_bwd(steps: int, scale: float, mode: str = "both", tune: bool = True):
     """views/s of fwd+bwd (L1 loss, the mean over each view summed over views) for the batch
-    workload: one launch set and one loss per view (plain); one launch set for all views with the
-    loss formed over their flat (P, 4) output and the 32-view scene processed in spatial order
-    (batched_eager); and that step captured once into a hipGraph and replayed (batched: one graph
-    launch per step, its loss gradient formed elementwise inside the graph)."""
+    workload: one launch set and one loss per view (plain, default 8x8 configuration); one launch
+    set for all views with the loss formed over their flat (P, 4) output and the 32-view scene
+    processed in spatial order (batched_eager); and that step captured once into a hipGraph and
+    replayed (batched: one graph launch per step, its loss gradient formed elementwise inside the
+    graph).  With tune, the batched forms use the configuration vkgs_diff_autotune picks for the
+    batch (tile shape, LDS batch depth, resident-wave cap), chosen before any timed step."""
     import torch
     from vkgs_amd import diff as D
 
@@ -180,23 +184,30 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both"):
             img = D.render(**params[si], camera=c)
             (img - tgt).abs().mean().backward()
 
-    def batched_eager_step():
-        zero_grad()
-        flat, _ = D.render_batch_flat(params, views, spatial_order=True)
-        ((flat - tgt_flat).abs() * w_flat).sum().backward()
-
     def l1_grad(flat):  # d/d(flat) of the summed per-view L1 means: elementwise, no reduction
         return torch.sign(flat - tgt_flat) * w_flat
 
     graphed = {}
+    out = {}
+    cfg = None
+    if tune and mode != "plain":
+        t0 = time.perf_counter()
+        cfg, tuned_ms = D.autotune(params, views, spatial_order=True, steps=2)
+        out["tuned"] = D.describe(cfg)
+        out["tuned_ms_per_step"] = round(tuned_ms, 3)
+        log(f"fwd+bwd autotune: {out['tuned']} ({tuned_ms:.3f} ms/step, search {time.perf_counter() - t0:.1f} s)")
+
+    def batched_eager_step():
+        zero_grad()
+        flat, _ = D.render_batch_flat(params, views, spatial_order=True, config=cfg)
+        ((flat - tgt_flat).abs() * w_flat).sum().backward()
 
     def batched_step():  # the whole fwd + bwd step over all views as one hipGraph launch
         if "step" not in graphed:
             zero_grad()
-            graphed["step"] = D.GraphedStep(params, views, l1_grad, spatial_order=True)
+            graphed["step"] = D.GraphedStep(params, views, l1_grad, spatial_order=True, config=cfg)
         graphed["step"].step()
 
-    out = {}
     forms = (("plain", plain_step), ("batched_eager", batched_eager_step), ("batched", batched_step))
     for name, fn in forms:
         if mode not in ("both", name):
@@ -410,7 +421,8 @@ def main(argv=None) -> int:
     # ---- fwd+bwd views/s: plain (one view per launch set) vs batched ----
     if args.fwd_bwd_steps > 0:
         try:
-            result["fwd_bwd"] = bench_fwd_bwd(args.fwd_bwd_steps, args.batch_scale, args.fwd_bwd_mode)
+            result["fwd_bwd"] = bench_fwd_bwd(args.fwd_bwd_steps, args.batch_scale, args.fwd_bwd_mode,
+                                              tune=not args.no_fwd_bwd_tune)
             if args.fwd_bwd_mode == "both":
                 result["fwd_bwd"].update(bench_c4(max(args.fwd_bwd_steps, 3)))
             log("fwd+bwd", result["fwd_bwd"])

@@ -179,8 +179,15 @@ typedef struct vkgs_raster_config {
   float t_min;             /* stop a pixel once transmittance < t_min (0 = composite every splat) */
   uint32_t variant;        /* 0: the workgroup stages batches of records in LDS; 1: each wave
                               fetches records with scalar loads.  Both give the same image. */
-  uint32_t flags;          /* VKGS_DIFF_SPATIAL_ORDER (differentiable path only), or 0 */
-  uint32_t reserved[3];
+  uint32_t flags;          /* VKGS_DIFF_SPATIAL_ORDER (differentiable path only), VKGS_RASTER_TRIANGLES,
+                              VKGS_RASTER_PAIRS_ALWAYS or VKGS_RASTER_PAIRS_NEVER, or 0 */
+  uint32_t lds_batch;      /* list entries per LDS batch of the differentiable backward on 8x8 tiles:
+                              8, 16 or 32 (0 = 16).  Deeper batches take more LDS per wave. */
+  uint32_t waves_per_cu;   /* cap on the resident waves per CU of the raster kernels (plain and
+                              differentiable, forward and backward): 0 = none (registers and LDS
+                              decide), else 4..32.  Set through each launch's LDS size, so a cap
+                              below what registers and LDS allow keeps fewer waves in flight. */
+  uint32_t reserved[1];
 } vkgs_raster_config;
 
 /* vkgs_diff_forward: process the splats of every scene that has four or more
@@ -203,6 +210,17 @@ typedef struct vkgs_raster_config {
  * a quad vertex more than 10^6 pixels from the image origin are not drawn in
  * this mode. */
 #define VKGS_RASTER_TRIANGLES 2u
+
+/* vkgs_raster_config.flags bits: 8x8 tiles are rasterized two per wave (one
+ * 64-lane wave per pair of tiles, two pixels per lane) when the frame or batch
+ * has 8192 tiles or more, one per wave below that, where the pairs' halved
+ * wave count fills the GPU less.  These force one form (both: invalid).  In
+ * the differentiable path the two forms give the same image bit for bit; the
+ * plain raster stops compositing when every pixel of a wave is below t_min,
+ * so a pixel may add a few more splats in one form than in the other, which
+ * changes it by less than t_min x max(colour, 1). */
+#define VKGS_RASTER_PAIRS_ALWAYS 4u
+#define VKGS_RASTER_PAIRS_NEVER 8u
 
 /* default: 8x8 tiles, t_min = 1/8192 */
 void vkgs_raster_config_default(vkgs_raster_config* cfg);
@@ -277,10 +295,12 @@ int vkgs_renderer_frame_info(vkgs_renderer* r, uint32_t* visible, uint64_t* pair
 int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const uint32_t** d_keys);
 int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const uint32_t** d_list_ids,
                              uint32_t* tiles_x, uint32_t* tiles_y);
-/* Autotuner: renders `frames` graph replays of this camera for each tile shape
- * (8x8, 16x8, 8x16, 16x4, 16x16, 32x4), three rounds visiting the shapes in
- * turn, times them with HIP events and keeps the shape with the fastest round
- * (t_min unchanged).  Synchronous.  The chosen config and its ms/frame are
+/* Autotuner: renders `frames` graph replays of this camera per candidate over
+ * a coordinate search: tile shape (8x8 one or two tiles per wave, 16x8, 8x16,
+ * 16x4, 16x16, 32x4), then the resident-wave cap (none, 24, 16, 12, 8 per
+ * CU).  Each stage runs three rounds visiting its candidates in turn, times
+ * them with HIP events and keeps each one's fastest round (t_min unchanged).
+ * Synchronous.  The renderer keeps the chosen config; it and its ms/frame are
  * returned when the pointers are non-NULL. */
 int vkgs_renderer_autotune(vkgs_renderer* r, const vkgs_camera* camera, const float model[16], float* d_rgba,
                            uint32_t frames, vkgs_raster_config* chosen, float* best_ms, void* stream);
@@ -359,6 +379,20 @@ int vkgs_diff_info(vkgs_diff_ctx* ctx, uint32_t* visible, uint64_t* pairs, void*
  * hipGraph on one stream, after an eager call has sized every buffer for the
  * same scenes and views, and check this after replays. */
 int vkgs_diff_check(vkgs_diff_ctx* ctx, uint64_t* pairs_needed, void* stream);
+/* Autotuner of the differentiable path (no counterpart in the reference,
+ * whose renderer has fixed-function rasterization): times steps of
+ * vkgs_diff_forward + vkgs_diff_backward on these scenes and views (their
+ * images and gradients are overwritten; every view needs grad_rgba) over a
+ * coordinate search: tile shape (8x8 one or two tiles per wave, 16x8, 8x16,
+ * 16x4, 16x16, 32x4), then the LDS batch depth (8, 16, 32) when 8x8 wins,
+ * then the resident-wave cap (none, 24, 16, 12, 8 per CU).  Each stage runs
+ * three rounds visiting its candidates in turn and keeps each candidate's
+ * fastest round.  base (NULL = default) gives t_min and the other flags.
+ * *chosen = the fastest configuration, *best_ms its ms per step.
+ * Synchronous. */
+int vkgs_diff_autotune(vkgs_diff_ctx* ctx, const vkgs_diff_scene* scenes, uint32_t n_scenes,
+                       const vkgs_diff_view* views, uint32_t n_views, const vkgs_raster_config* base,
+                       uint32_t steps, vkgs_raster_config* chosen, float* best_ms, void* stream);
 
 #ifdef __cplusplus
 }

@@ -284,6 +284,8 @@ def test_autotune_keeps_the_image(c1):
     before = r.render(cam).cpu().numpy()
     tw, th, ms = r.autotune(cam, frames=3)
     assert (tw, th) in {(8, 8), (16, 8), (8, 16), (16, 4), (16, 16), (32, 4)} and ms > 0
+    assert r.tuned.waves_per_cu in (0, 24, 16, 12, 8)
+    print("renderer autotune", tw, th, r.tuned.flags, r.tuned.waves_per_cu, f"{ms:.3f} ms", flush=True)
     after = r.render(cam).cpu().numpy()
     np.testing.assert_array_equal(before, after)  # compositing per pixel does not depend on the tile shape
     r.close()

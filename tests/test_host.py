@@ -41,6 +41,29 @@ def test_camera_struct_layout_matches_uniforms_h():
     assert _lib.Camera.screen_size.offset == 144
 
 
+def test_raster_config_layout_and_launch_choice_validation():
+    """vkgs_raster_config keeps its 32-byte layout; the launch choices (pair form, LDS batch depth,
+    resident-wave cap) are validated on the host, here through vkgs_draw_storage_bytes, which
+    returns 0 for a configuration no launch accepts."""
+    assert C.sizeof(_lib.RasterConfig) == 32
+    assert _lib.RasterConfig.lds_batch.offset == 20 and _lib.RasterConfig.waves_per_cu.offset == 24
+    lib = _lib.lib()
+
+    def size(**kw):
+        cfg = _lib.default_raster_config()
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        return lib.vkgs_draw_storage_bytes(1000, 640, 480, 100000, C.byref(cfg))
+
+    assert size() > 0
+    for good in (dict(lds_batch=8), dict(lds_batch=32), dict(waves_per_cu=4), dict(waves_per_cu=32),
+                 dict(flags=4), dict(flags=8)):
+        assert size(**good) > 0, good
+    for bad in (dict(lds_batch=12), dict(lds_batch=64), dict(waves_per_cu=2), dict(waves_per_cu=33),
+                dict(flags=4 | 8)):
+        assert size(**bad) == 0, bad
+
+
 def _mats(cam):
     P, V, eye = S.camera_arrays(cam)
     return P.reshape(4, 4).T.astype(np.float64), V.reshape(4, 4).T.astype(np.float64), eye.astype(np.float64)

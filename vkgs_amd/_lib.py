@@ -64,7 +64,9 @@ class RasterConfig(C.Structure):
         ("t_min", C.c_float),
         ("variant", C.c_uint32),
         ("flags", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("lds_batch", C.c_uint32),
+        ("waves_per_cu", C.c_uint32),
+        ("reserved", C.c_uint32 * 1),
     ]
 
 
@@ -121,6 +123,7 @@ SIGNATURES = {
     "vkgs_diff_backward": (_I, [_P, _P, _U32, _P, _U32, _P]),
     "vkgs_diff_info": (_I, [_P, _P, _P, _P]),
     "vkgs_diff_check": (_I, [_P, _P, _P]),
+    "vkgs_diff_autotune": (_I, [_P, _P, _U32, _P, _U32, _P, _U32, _P, _P, _P]),
 }
 
 _lib = None

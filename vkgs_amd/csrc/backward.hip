@@ -33,10 +33,6 @@ namespace {
 
 constexpr float kAlphaMax = 0.999f;
 
-#ifndef VKGS_BWD_BATCH
-#define VKGS_BWD_BATCH 16  // list entries per batch of the one-wave raster backward; batched fwd+bwd
-                           // views/s against 16: 8 entries -11%, 32 -2%, 64 -17%
-#endif
 
 __device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
@@ -686,7 +682,7 @@ __device__ __forceinline__ float half_row_sum(float x) {
 // 9 words of gpart at its slot-order index q; the tile's entries past every
 // pixel's last one get zeros, so each pair is written once and gpart needs no
 // fill.  A reduction then sums each slot's contiguous run of pairs.
-template <int TW, int TH>
+template <int TW, int TH, int KB>
 __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                                       const uint32_t* __restrict__ vals,
                                                                       const uint32_t* __restrict__ ranges,
@@ -697,9 +693,9 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   constexpr int kRowsPerWave = kWave / TW;
   constexpr bool kOneWave = kN == kWave;  // the wave covers the tile: every listed record touches it
   // list entries per batch: one per thread for multi-wave tiles; a one-wave
-  // tile takes fewer, so the LDS footprint (records + partial sums) admits
-  // more resident waves
-  constexpr int kB = kOneWave ? VKGS_BWD_BATCH : kN;
+  // tile takes KB (8, 16 or 32; 0: 16), fewer than its lanes, so the LDS
+  // footprint (records + partial sums) admits more resident waves
+  constexpr int kB = kOneWave ? (KB ? KB : 16) : kN;
   static_assert(kB <= kN, "batch larger than the workgroup");
   __shared__ float4 s_rec[3][kB];
   __shared__ float s_g[9][8][kB + 1];  // per word, one partial per half DPP row; +1 spreads them over banks
@@ -886,13 +882,6 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
 // agree with the one-tile kernel's to float rounding.
 
 constexpr int kPairHalf = kWave / 2;  // lanes per tile
-#ifndef VKGS_DIFF_PAIR_TILES
-#define VKGS_DIFF_PAIR_TILES 1
-#endif
-constexpr bool kPairTiles = VKGS_DIFF_PAIR_TILES != 0;  // 8x8 tiles two per wave (0: one per workgroup)
-// Two tiles per wave halve the launch's waves: below this many tiles (a few
-// waves per CU) the one-tile kernels keep more waves in flight and are faster.
-constexpr uint32_t kPairMinTiles = 8192;
 
 // Maximum of v over the 32 lanes of each half wave (every lane gets its half's).
 __device__ __forceinline__ uint32_t half_max(uint32_t v) {
@@ -1072,12 +1061,13 @@ __device__ __forceinline__ PixTerms bwd_pixel(PixState& ps, const PixCover& pc, 
   return r;
 }
 
-// The backward of diff_raster_backward_kernel<8, 8> for two tiles per wave.
+// The backward of diff_raster_backward_kernel<8, 8, KB> for two tiles per wave.
+template <int KB>
 __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
     DiffLaunch L, const float4* __restrict__ rec, const uint32_t* __restrict__ vals,
     const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ pslot, const uint32_t* __restrict__ tile_order,
     float* __restrict__ gpart) {
-  constexpr int kB = VKGS_BWD_BATCH;  // list entries per batch of each tile
+  constexpr int kB = KB;  // list entries per batch of each tile
   constexpr int kSlots = kPairHalf / 8;  // half rows of 8 lanes per tile
   __shared__ float4 s_rec[2][3][kB];
   __shared__ float s_g[2][9][kSlots][kB + 1];
@@ -1701,21 +1691,42 @@ hipError_t launch_diff_bin_emit(const DiffLaunch& L, const TileRect* rect, const
 hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
                               const uint32_t* pslot, const uint32_t* tile_order, bool keep_state, hipStream_t s) {
   if (L.n_tiles == 0) return hipSuccess;
-  if (L.tile_w == 8 && L.tile_h == 8 && kPairTiles && L.n_tiles >= kPairMinTiles) {
-    diff_raster_pair_kernel<<<(L.n_tiles + 1) / 2, kWave, 0, s>>>(L, rec, vals, ranges, pslot, tile_order,
-                                                                  keep_state ? 1 : 0);
+  const uint32_t cap = L.waves_per_cu;
+  if (L.tile_w == 8 && L.tile_h == 8 && use_pairs(L.pairs, L.n_tiles)) {
+    const size_t pad = occupancy_lds_pad((const void*)diff_raster_pair_kernel, 1, cap);
+    diff_raster_pair_kernel<<<(L.n_tiles + 1) / 2, kWave, pad, s>>>(L, rec, vals, ranges, pslot, tile_order,
+                                                                    keep_state ? 1 : 0);
     return hipGetLastError();
   }
-#define VKGS_CASE(W_, H_)                                                                                  \
-  if (L.tile_w == W_ && L.tile_h == H_) {                                                                  \
-    diff_raster_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, pslot, tile_order,      \
-                                                              keep_state ? 1 : 0);                         \
-    return hipGetLastError();                                                                              \
+#define VKGS_CASE(W_, H_)                                                                                      \
+  if (L.tile_w == W_ && L.tile_h == H_) {                                                                      \
+    const size_t pad = occupancy_lds_pad((const void*)diff_raster_kernel<W_, H_>, W_ * H_ / kWave, cap);       \
+    diff_raster_kernel<W_, H_><<<L.n_tiles, W_ * H_, pad, s>>>(L, rec, vals, ranges, pslot, tile_order,        \
+                                                                keep_state ? 1 : 0);                           \
+    return hipGetLastError();                                                                                  \
   }
   VKGS_DIFF_CASES(VKGS_CASE)
 #undef VKGS_CASE
   return hipErrorInvalidValue;
 }
+
+namespace {
+
+template <int KB>
+void launch_backward_8x8(const DiffLaunch& L, const float4* rec, const uint32_t* vals, const uint32_t* ranges,
+                         const uint32_t* pslot, const uint32_t* tile_order, float* gpart, hipStream_t s) {
+  if (use_pairs(L.pairs, L.n_tiles)) {
+    const size_t pad = occupancy_lds_pad((const void*)diff_raster_backward_pair_kernel<KB>, 1, L.waves_per_cu);
+    diff_raster_backward_pair_kernel<KB><<<(L.n_tiles + 1) / 2, kWave, pad, s>>>(L, rec, vals, ranges, pslot,
+                                                                                tile_order, gpart);
+  } else {
+    const size_t pad = occupancy_lds_pad((const void*)diff_raster_backward_kernel<8, 8, KB>, 1, L.waves_per_cu);
+    diff_raster_backward_kernel<8, 8, KB><<<L.n_tiles, kWave, pad, s>>>(L, rec, vals, ranges, pslot, tile_order,
+                                                                        gpart);
+  }
+}
+
+}  // namespace
 
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
@@ -1725,16 +1736,22 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
   // partials of pairs no pixel reached stay zero
   if (L.n_tiles > 0) {
     bool launched = false;
-    if (L.tile_w == 8 && L.tile_h == 8 && kPairTiles && L.n_tiles >= kPairMinTiles) {
-      diff_raster_backward_pair_kernel<<<(L.n_tiles + 1) / 2, kWave, 0, s>>>(L, rec, vals, ranges, pslot, tile_order,
-                                                                           gpart);
+    if (L.tile_w == 8 && L.tile_h == 8) {
+      if (L.lds_batch == 8)
+        launch_backward_8x8<8>(L, rec, vals, ranges, pslot, tile_order, gpart, s);
+      else if (L.lds_batch == 32)
+        launch_backward_8x8<32>(L, rec, vals, ranges, pslot, tile_order, gpart, s);
+      else
+        launch_backward_8x8<16>(L, rec, vals, ranges, pslot, tile_order, gpart, s);
       launched = true;
     }
-#define VKGS_CASE(W_, H_)                                                                                  \
-  if (!launched && L.tile_w == W_ && L.tile_h == H_) {                                                     \
-    diff_raster_backward_kernel<W_, H_><<<L.n_tiles, W_ * H_, 0, s>>>(L, rec, vals, ranges, pslot,         \
-                                                                       tile_order, gpart);                 \
-    launched = true;                                                                                       \
+#define VKGS_CASE(W_, H_)                                                                                      \
+  if (!launched && L.tile_w == W_ && L.tile_h == H_) {                                                         \
+    const size_t pad =                                                                                         \
+        occupancy_lds_pad((const void*)diff_raster_backward_kernel<W_, H_, 0>, W_ * H_ / kWave, L.waves_per_cu); \
+    diff_raster_backward_kernel<W_, H_, 0><<<L.n_tiles, W_ * H_, pad, s>>>(L, rec, vals, ranges, pslot,        \
+                                                                            tile_order, gpart);                \
+    launched = true;                                                                                           \
   }
     VKGS_DIFF_CASES(VKGS_CASE)
 #undef VKGS_CASE

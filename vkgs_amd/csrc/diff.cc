@@ -7,6 +7,7 @@
 
 #include <vector>
 
+#include "autotune.h"
 #include "common.h"
 #include "diff.h"
 #include "kernels.h"
@@ -361,6 +362,9 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   L.tile_shift_x = rs0.tile_shift_x;
   L.tile_shift_y = rs0.tile_shift_y;
   L.t_min = cfg->t_min > kDiffTMinFloor ? cfg->t_min : kDiffTMinFloor;
+  L.pairs = rs0.pairs;
+  L.lds_batch = rs0.lds_batch;
+  L.waves_per_cu = rs0.waves_per_cu;
   c->cap_items = cap;
 
   // ---- processing order (when due) ----
@@ -490,6 +494,67 @@ int vkgs_diff_check(vkgs_diff_ctx* c, uint64_t* pairs_needed, void* stream) {
   VKGS_HIP_CHECK(fill_u32_launch(c->misc.as<uint32_t>() + kMiscSticky, 0u, 2, s));
   VKGS_HIP_CHECK(hipStreamSynchronize(s));
   return VKGS_ERR_OVERFLOW;
+}
+
+int vkgs_diff_autotune(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t n_scenes,
+                       const vkgs_diff_view* views, uint32_t n_views, const vkgs_raster_config* base_cfg,
+                       uint32_t steps, vkgs_raster_config* chosen, float* best_ms, void* stream) {
+  if (!c || !scenes || !views || n_views == 0 || !chosen) return VKGS_ERR_INVALID_ARG;
+  for (uint32_t v = 0; v < n_views; ++v)
+    if (!views[v].grad_rgba) return VKGS_ERR_INVALID_ARG;
+  if (steps == 0) steps = 3;
+  vkgs_raster_config base;
+  if (base_cfg)
+    base = *base_cfg;
+  else
+    vkgs_raster_config_default(&base);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  VKGS_HIP_CHECK(hipEventCreate(&e0));
+  VKGS_HIP_CHECK(hipEventCreate(&e1));
+  auto step = [&](const vkgs_raster_config& cfg) -> int {
+    const int st = vkgs_diff_forward(c, scenes, n_scenes, views, n_views, &cfg, stream);
+    return st != VKGS_OK ? st : vkgs_diff_backward(c, scenes, n_scenes, views, n_views, stream);
+  };
+  const TuneTimer time_one = [&](const vkgs_raster_config& cfg, float* ms) -> int {
+    // a first step sizes the storage for this tile shape; repeat while its pair list overflows
+    int st = VKGS_ERR_OVERFLOW;
+    for (int i = 0; i < 4 && st == VKGS_ERR_OVERFLOW; ++i) {
+      if ((st = step(cfg)) != VKGS_OK) return st;
+      st = vkgs_diff_check(c, nullptr, stream);
+    }
+    if (st != VKGS_OK) return st;
+    if ((st = step(cfg)) != VKGS_OK) return st;
+    hipEventRecord(e0, s);
+    for (uint32_t i = 0; i < steps && st == VKGS_OK; ++i) st = step(cfg);
+    hipEventRecord(e1, s);
+    if (st != VKGS_OK) return st;
+    if ((st = vkgs_diff_check(c, nullptr, stream)) != VKGS_OK) return st;
+    if (hipEventSynchronize(e1) != hipSuccess) return VKGS_ERR_HIP;
+    hipEventElapsedTime(ms, e0, e1);
+    *ms /= (float)steps;
+    return VKGS_OK;
+  };
+  // tile shape (8x8 one or two tiles per wave), then the backward's LDS batch
+  // depth on 8x8 tiles, then the resident-wave cap
+  const TuneStage batches = [](const vkgs_raster_config& from) {
+    std::vector<vkgs_raster_config> out;
+    if (from.tile_w == 8 && from.tile_h == 8)
+      for (uint32_t b : {8u, 16u, 32u}) {
+        vkgs_raster_config cfg = from;
+        cfg.lds_batch = b;
+        out.push_back(cfg);
+      }
+    return out;
+  };
+  const std::vector<TuneStage> stages = {tune_shapes, batches, tune_waves};
+  float t = 0.f;
+  const int status = tune_configs(base, stages, 3, time_one, chosen, &t);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (status != VKGS_OK) return status;
+  if (best_ms) *best_ms = t;
+  return VKGS_OK;
 }
 
 int vkgs_diff_info(vkgs_diff_ctx* c, uint32_t* visible, uint64_t* pairs, void* stream) {

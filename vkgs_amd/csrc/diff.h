@@ -68,6 +68,9 @@ struct DiffLaunch {
   uint32_t n_views, n_scenes, n_blocks, n_tiles, n_scene_blocks;
   uint32_t tile_w, tile_h, tile_shift_x, tile_shift_y;
   float t_min;
+  // launch choices (host side): 8x8 tiles two per wave (RasterShape::pairs),
+  // backward LDS batch depth on 8x8 tiles, resident-wave cap of the rasters
+  uint32_t pairs, lds_batch, waves_per_cu;
 };
 
 // Spatial processing order of every scene's splats: keys[g] = scene index in

@@ -20,6 +20,7 @@
 #include <thread>
 #include <vector>
 
+#include "autotune.h"
 #include "common.h"
 #include "kernels.h"
 #include "sort.h"
@@ -648,60 +649,39 @@ int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const 
 int vkgs_renderer_autotune(vkgs_renderer* r, const vkgs_camera* camera, const float model[16], float* d_rgba,
                            uint32_t frames, vkgs_raster_config* chosen, float* best_ms, void* stream) {
   if (!r || !camera || !d_rgba) return VKGS_ERR_INVALID_ARG;
-  static const uint32_t kShapes[][2] = {{8, 8}, {16, 8}, {8, 16}, {16, 4}, {16, 16}, {32, 4}};
   if (frames == 0) frames = 10;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const vkgs_raster_config base = r->cfg;
   const bool timing = r->timing;
   r->timing = false;
-  vkgs_raster_config best = base;
-  float best_t = 1e30f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   VKGS_HIP_CHECK(hipEventCreate(&e0));
   VKGS_HIP_CHECK(hipEventCreate(&e1));
-  int status = VKGS_OK;
-  // shapes are timed round-robin and each keeps its fastest round, so clock
-  // ramp-up or a transient slow period does not favour the shapes timed later
-  constexpr int kShapeCount = sizeof(kShapes) / sizeof(kShapes[0]);
-  constexpr int kRounds = 3;
-  float shape_ms[kShapeCount];
-  for (int i = 0; i < kShapeCount; ++i) shape_ms[i] = 1e30f;
-  for (int round = 0; round < kRounds && status == VKGS_OK; ++round) {
-    for (int si = 0; si < kShapeCount && status == VKGS_OK; ++si) {
-      vkgs_raster_config c = base;
-      c.tile_w = kShapes[si][0];
-      c.tile_h = kShapes[si][1];
-      if ((status = vkgs_renderer_set_config(r, &c)) != VKGS_OK) break;
-      // the first frame sizes the pair storage; repeat while it overflows
-      for (int i = 0; i < 4; ++i) {
-        if ((status = vkgs_renderer_render(r, camera, model, d_rgba, stream)) != VKGS_OK) break;
-        status = vkgs_renderer_frame_info(r, nullptr, nullptr, stream);
-        if (status != VKGS_ERR_OVERFLOW) break;
-      }
-      if (status != VKGS_OK) break;
-      for (uint32_t i = 0; i < 2 && status == VKGS_OK; ++i) status = vkgs_renderer_render(r, camera, model, d_rgba, stream);
-      hipEventRecord(e0, s);
-      for (uint32_t i = 0; i < frames && status == VKGS_OK; ++i)
-        status = vkgs_renderer_render(r, camera, model, d_rgba, stream);
-      hipEventRecord(e1, s);
-      if (status != VKGS_OK) break;
-      if (hipEventSynchronize(e1) != hipSuccess) {
-        status = VKGS_ERR_HIP;
-        break;
-      }
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, e0, e1);
-      ms /= (float)frames;
-      if (ms < shape_ms[si]) shape_ms[si] = ms;
+  const TuneTimer time_one = [&](const vkgs_raster_config& c, float* ms) -> int {
+    int st = vkgs_renderer_set_config(r, &c);
+    if (st != VKGS_OK) return st;
+    // the first frame sizes the pair storage; repeat while it overflows
+    for (int i = 0; i < 4; ++i) {
+      if ((st = vkgs_renderer_render(r, camera, model, d_rgba, stream)) != VKGS_OK) return st;
+      st = vkgs_renderer_frame_info(r, nullptr, nullptr, stream);
+      if (st != VKGS_ERR_OVERFLOW) break;
     }
-  }
-  for (int si = 0; si < kShapeCount && status == VKGS_OK; ++si)
-    if (shape_ms[si] < best_t) {
-      best_t = shape_ms[si];
-      best = base;
-      best.tile_w = kShapes[si][0];
-      best.tile_h = kShapes[si][1];
-    }
+    if (st != VKGS_OK) return st;
+    for (uint32_t i = 0; i < 2 && st == VKGS_OK; ++i) st = vkgs_renderer_render(r, camera, model, d_rgba, stream);
+    hipEventRecord(e0, s);
+    for (uint32_t i = 0; i < frames && st == VKGS_OK; ++i) st = vkgs_renderer_render(r, camera, model, d_rgba, stream);
+    hipEventRecord(e1, s);
+    if (st != VKGS_OK) return st;
+    if (hipEventSynchronize(e1) != hipSuccess) return VKGS_ERR_HIP;
+    hipEventElapsedTime(ms, e0, e1);
+    *ms /= (float)frames;
+    return VKGS_OK;
+  };
+  // tile shape (and 8x8 one or two tiles per wave), then the resident-wave cap
+  const std::vector<TuneStage> stages = {tune_shapes, tune_waves};
+  vkgs_raster_config best = base;
+  float best_t = 0.f;
+  const int status = tune_configs(base, stages, 3, time_one, &best, &best_t);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   r->timing = timing;

@@ -64,7 +64,20 @@ struct RasterShape {
   float t_min;
   uint32_t variant;    // 0: LDS batches, 1: scalar-load waves
   uint32_t triangles;  // 1: triangle coverage (VKGS_RASTER_TRIANGLES)
+  uint32_t pairs;      // 8x8 tiles two per wave: 0 from kPairMinTiles tiles up, 1 always, 2 never
+  uint32_t lds_batch;  // list entries per LDS batch of the differentiable 8x8 backward
+  uint32_t waves_per_cu;  // resident-wave cap of the raster kernels (0: none)
 };
+
+// Tile count from which 8x8 tiles are rasterized two per wave by default:
+// below it the one-tile kernels' twice as many waves fill the GPU better.
+constexpr uint32_t kPairMinTiles = 8192;
+inline bool use_pairs(uint32_t pairs, uint32_t tiles) { return pairs == 1 || (pairs == 0 && tiles >= kPairMinTiles); }
+
+// Dynamic LDS bytes that, added to `kernel`'s static LDS, keep at most about
+// waves_per_cu of its waves (wg_waves per workgroup) resident on a CU; 0 when
+// waves_per_cu is 0 or the static LDS already allows no more.
+size_t occupancy_lds_pad(const void* kernel, uint32_t wg_waves, uint32_t waves_per_cu);
 
 bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config* cfg, RasterShape* rs);
 

@@ -32,6 +32,8 @@
 #include "device.h"
 #include "kernels.h"
 
+#include <mutex>
+
 namespace vkgs {
 namespace {
 
@@ -665,6 +667,45 @@ int ilog2(uint32_t v) {
 
 }  // namespace
 
+size_t occupancy_lds_pad(const void* kernel, uint32_t wg_waves, uint32_t waves_per_cu) {
+  if (waves_per_cu == 0) return 0;
+  // per-device LDS per CU and per-kernel static LDS, looked up once
+  struct Known {
+    const void* kernel;
+    size_t lds;
+  };
+  static std::mutex mu;
+  static Known known[64];
+  static int n_known = 0;
+  static size_t cu_lds[16] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (cu_lds[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || v <= 0)
+      return 0;
+    cu_lds[dev] = (size_t)v;
+  }
+  size_t fixed = SIZE_MAX;
+  for (int i = 0; i < n_known; ++i)
+    if (known[i].kernel == kernel) fixed = known[i].lds;
+  if (fixed == SIZE_MAX) {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, kernel) != hipSuccess) return 0;
+    fixed = a.sharedSizeBytes;
+    if (n_known < 64) known[n_known++] = {kernel, fixed};
+  }
+  // at least 3 workgroups per CU keeps a workgroup's LDS within the 64 KiB a
+  // launch may ask for without opting in to more
+  uint32_t wgs = waves_per_cu / (wg_waves ? wg_waves : 1u);
+  if (wgs < 3) wgs = 3;
+  // rounded down to 512 bytes (an LDS allocation granule): wgs workgroups fit, and no more
+  // for wgs up to 18 and the even values up to 24 (160 KiB per CU)
+  const size_t per = (cu_lds[dev] / wgs) & ~(size_t)511;
+  return per > fixed ? per - fixed : 0;
+}
+
 bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config* cfg, RasterShape* rs) {
   vkgs_raster_config d;
   if (cfg == nullptr) {
@@ -678,8 +719,15 @@ bool make_raster_shape(uint32_t width, uint32_t height, const vkgs_raster_config
   if (!ok_shape || width == 0 || height == 0 || width > 32768 || height > 32768) return false;
   if (!(cfg->t_min >= 0.f && cfg->t_min < 1.f)) return false;
   if (cfg->variant > 1) return false;
+  const uint32_t pf = cfg->flags & (VKGS_RASTER_PAIRS_ALWAYS | VKGS_RASTER_PAIRS_NEVER);
+  if (pf == (VKGS_RASTER_PAIRS_ALWAYS | VKGS_RASTER_PAIRS_NEVER)) return false;
+  if (cfg->lds_batch != 0 && cfg->lds_batch != 8 && cfg->lds_batch != 16 && cfg->lds_batch != 32) return false;
+  if (cfg->waves_per_cu != 0 && (cfg->waves_per_cu < 4 || cfg->waves_per_cu > 32)) return false;
   rs->variant = cfg->variant;
   rs->triangles = (cfg->flags & VKGS_RASTER_TRIANGLES) ? 1u : 0u;
+  rs->pairs = pf == VKGS_RASTER_PAIRS_ALWAYS ? 1u : pf == VKGS_RASTER_PAIRS_NEVER ? 2u : 0u;
+  rs->lds_batch = cfg->lds_batch ? cfg->lds_batch : 16u;
+  rs->waves_per_cu = cfg->waves_per_cu;
   rs->width = width;
   rs->height = height;
   rs->tile_w = tw;
@@ -753,19 +801,25 @@ hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint3
                             const uint32_t* tile_order, float4* out, hipStream_t s, uint32_t* work) {
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tw = rs.tile_w, th = rs.tile_h;
-#define VKGS_RASTER_CASE(W_, H_)                                                                   \
-  if (tw == W_ && th == H_) {                                                                      \
-    if (rs.variant == 0)                                                                           \
-      rasterize_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, tile_order, out, work); \
-    else                                                                                           \
-      rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, out, work);   \
-    return hipGetLastError();                                                                      \
+  const uint32_t cap = rs.waves_per_cu;
+#define VKGS_RASTER_CASE(W_, H_)                                                                             \
+  if (tw == W_ && th == H_) {                                                                                \
+    constexpr uint32_t kWg = W_ * H_ / kWave;                                                                \
+    if (rs.variant == 0) {                                                                                   \
+      const size_t pad = occupancy_lds_pad((const void*)rasterize_kernel<W_, H_>, kWg, cap);                 \
+      rasterize_kernel<W_, H_><<<tiles, W_ * H_, pad, s>>>(rec, vals, ranges, rs, tile_order, out, work);    \
+    } else {                                                                                                 \
+      const size_t pad = occupancy_lds_pad((const void*)rasterize_wave_kernel<W_, H_>, kWg, cap);            \
+      rasterize_wave_kernel<W_, H_><<<tiles, W_ * H_, pad, s>>>(rec, vals, ranges, rs, out, work);           \
+    }                                                                                                        \
+    return hipGetLastError();                                                                                \
   }
   if (rs.triangles) {
-#define VKGS_TRI_CASE(W_, H_)                                                                        \
-  if (tw == W_ && th == H_) {                                                                         \
-    rasterize_tri_kernel<W_, H_><<<tiles, W_ * H_, 0, s>>>(rec, vals, ranges, rs, tile_order, out, work); \
-    return hipGetLastError();                                                                         \
+#define VKGS_TRI_CASE(W_, H_)                                                                                \
+  if (tw == W_ && th == H_) {                                                                                \
+    const size_t pad = occupancy_lds_pad((const void*)rasterize_tri_kernel<W_, H_>, W_ * H_ / kWave, cap);   \
+    rasterize_tri_kernel<W_, H_><<<tiles, W_ * H_, pad, s>>>(rec, vals, ranges, rs, tile_order, out, work);  \
+    return hipGetLastError();                                                                                \
   }
     VKGS_TRI_CASE(8, 8)
     VKGS_TRI_CASE(16, 16)
@@ -778,9 +832,10 @@ hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint3
 #undef VKGS_TRI_CASE
     return hipErrorInvalidValue;
   }
-  // two tiles per wave halve the launch's waves: only where there are enough tiles to fill the GPU
-  if (tw == 8 && th == 8 && rs.variant == 0 && tiles >= 8192) {
-    rasterize_pair_kernel<<<(tiles + 1) / 2, kWave, 0, s>>>(rec, vals, ranges, rs, tile_order, out, work);
+  // two tiles per wave halve the launch's waves: by default only where there are enough tiles to fill the GPU
+  if (tw == 8 && th == 8 && rs.variant == 0 && use_pairs(rs.pairs, tiles)) {
+    const size_t pad = occupancy_lds_pad((const void*)rasterize_pair_kernel, 1, cap);
+    rasterize_pair_kernel<<<(tiles + 1) / 2, kWave, pad, s>>>(rec, vals, ranges, rs, tile_order, out, work);
     return hipGetLastError();
   }
   VKGS_RASTER_CASE(16, 16)

@@ -40,6 +40,7 @@ _SIGS = {
     "vkgs_diff_backward": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P]),
     "vkgs_diff_info": (C.c_int, [_P, _P, _P, _P]),
     "vkgs_diff_check": (C.c_int, [_P, _P, _P]),
+    "vkgs_diff_autotune": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P, _P, _P]),
 }
 
 
@@ -100,7 +101,13 @@ def _stream() -> int:
 SPATIAL_ORDER = 1  # VKGS_DIFF_SPATIAL_ORDER
 
 
-def _config(tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False):
+def _config(tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False, config=None):
+    """The raster configuration of a call: `config` (a RasterConfig, e.g. from autotune()) as
+    given, else one built from tile, t_min and spatial_order."""
+    if config is not None:
+        cfg = _lib.RasterConfig()
+        C.memmove(C.byref(cfg), C.byref(config), C.sizeof(cfg))
+        return cfg
     cfg = _lib.RasterConfig()
     cfg.tile_w, cfg.tile_h, cfg.t_min = tile[0], tile[1], t_min
     cfg.flags = SPATIAL_ORDER if spatial_order else 0
@@ -228,23 +235,59 @@ def _batch_args(scenes, views):
     return vs, params
 
 
-def render_batch(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False):
+def render_batch(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False, config=None):
     """scenes: list of dicts with PARAM_NAMES tensors (cuda);
     views: list of (scene_index, camera[, model]).  Returns a list of (H, W, 4) images.
     spatial_order: process the splats of scenes with four or more views in this call in a spatial
-    (Morton) order (faster; equal depth keys then blend in that order instead of ascending id)."""
+    (Morton) order (faster; equal depth keys then blend in that order instead of ascending id).
+    config: a RasterConfig (from autotune()) used instead of tile, t_min and spatial_order."""
     vs, params = _batch_args(scenes, views)
-    return list(_RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order), False, None), *params))
+    cfg = _config(tile, t_min, spatial_order, config)
+    return list(_RenderBatch.apply((vs, len(scenes), cfg, False, None), *params))
 
 
-def render_batch_flat(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False):
+def render_batch_flat(scenes, views, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False, config=None):
     """Same as render_batch, with every view's pixels in one (P, 4) tensor: returns (flat, spans),
     spans[i] = (offset, H, W) of view i, whose image is flat[offset:offset + H * W].view(H, W, 4)
     (see view_images).  A loss over all views can then be formed with a few whole-batch tensor
     operations instead of a few per view."""
     vs, params = _batch_args(scenes, views)
-    flat = _RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order), True, None), *params)
+    flat = _RenderBatch.apply((vs, len(scenes), _config(tile, t_min, spatial_order, config), True, None), *params)
     return flat, _spans(vs)[0]
+
+
+def autotune(scenes, views, t_min=1.0 / 8192.0, spatial_order=False, steps=3, grad=None):
+    """Chooses the tile shape, the backward's LDS batch depth and the resident-wave cap of
+    forward + backward steps over this batch (vkgs_diff_autotune): returns (RasterConfig, ms per
+    step) for the config= argument of render_batch, render_batch_flat and GraphedStep.  grad: the
+    (P, 4) dL/d(flat output) the steps use (default: ones).  Synchronous; the parameters' .grad
+    are not touched."""
+    vs, params = _batch_args(scenes, views)
+    n_scenes = len(scenes)
+    groups = [tuple(_aligned(p.detach().contiguous().float()) for p in params[5 * i:5 * i + 5])
+              for i in range(n_scenes)]
+    dev = groups[0][0].device
+    spans, total = _spans(vs)
+    buf = torch.empty((total, 4), dtype=torch.float32, device=dev)
+    g = torch.ones((total, 4), dtype=torch.float32, device=dev) if grad is None else grad.contiguous().float()
+    outs = [buf[o:o + h * w].view(h, w, 4) for o, h, w in spans]
+    gouts = [g[o:o + h * w].view(h, w, 4) for o, h, w in spans]
+    grads = [tuple(torch.empty_like(t) for t in gr) for gr in groups]
+    sarr = _scene_structs(groups, grads)
+    varr = _view_structs(vs, outs, gouts)
+    base = _config(t_min=t_min, spatial_order=spatial_order)
+    chosen, ms = _lib.RasterConfig(), C.c_float()
+    c = _Ctx()
+    check(_L().vkgs_diff_autotune(c.h, sarr, n_scenes, varr, len(vs), C.byref(base), steps, C.byref(chosen),
+                                  C.byref(ms), _stream()), "vkgs_diff_autotune")
+    return chosen, float(ms.value)
+
+
+def describe(cfg) -> str:
+    """A RasterConfig as text: tile shape, pair form, LDS batch, wave cap, t_min."""
+    pairs = {4: " pairs", 8: " single"}.get(cfg.flags & 12, "")
+    return (f"{cfg.tile_w}x{cfg.tile_h}{pairs} lds_batch={cfg.lds_batch or 16} "
+            f"waves/CU={cfg.waves_per_cu or 'max'} t_min={cfg.t_min:g}")
 
 
 class GraphedStep:
@@ -264,10 +307,11 @@ class GraphedStep:
     processing order are those of the capture; after check() reports an overflow, build a new step
     (its warm-up sizes the pair storage for the longer list)."""
 
-    def __init__(self, scenes, views, grad_fn, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False, warmup=2):
+    def __init__(self, scenes, views, grad_fn, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False, warmup=2,
+                 config=None):
         self.vs, self.params = _batch_args(scenes, views)
         self.n_scenes = len(scenes)
-        self.cfg = _config(tile, t_min, spatial_order)
+        self.cfg = _config(tile, t_min, spatial_order, config)
         self.grad_fn = grad_fn
         self.ctx = _Ctx()
         self.spans = _spans(self.vs)[0]
@@ -321,10 +365,11 @@ def view_images(flat, spans):
     return [flat[o:o + h * w].view(h, w, 4) for o, h, w in spans]
 
 
-def render(means, log_scales, quats, opacity_logits, sh, camera, model=None, tile=(8, 8), t_min=1.0 / 8192.0):
+def render(means, log_scales, quats, opacity_logits, sh, camera, model=None, tile=(8, 8), t_min=1.0 / 8192.0,
+           config=None):
     """One view of one scene; differentiable w.r.t. every parameter tensor."""
     scene = dict(means=means, log_scales=log_scales, quats=quats, opacity_logits=opacity_logits, sh=sh)
-    return render_batch([scene], [(0, camera, model)], tile, t_min)[0]
+    return render_batch([scene], [(0, camera, model)], tile, t_min, config=config)[0]
 
 
 def scene_tensors(sc, device="cuda", requires_grad=True):

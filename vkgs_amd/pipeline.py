@@ -180,6 +180,7 @@ class Renderer:
         self._h = C.c_void_p()
         check(lib().vkgs_renderer_create(capacity, C.byref(self._h)), "vkgs_renderer_create")
         self.capacity = capacity
+        self.tuned = None  # the RasterConfig of the last autotune()
 
     def close(self):
         if self._h:
@@ -229,12 +230,18 @@ class Renderer:
             _copy_device(t.data_ptr(), p.value, t.numel() * t.element_size())
         return out
 
-    def set_config(self, tile_w=16, tile_h=16, t_min=1.0 / 8192.0, triangles=False) -> None:
-        """Tile shape, early-exit threshold, and triangles=True for the reference's triangle coverage
-        (VKGS_RASTER_TRIANGLES) instead of the analytic square."""
+    def set_config(self, tile_w=16, tile_h=16, t_min=1.0 / 8192.0, triangles=False, waves_per_cu=0,
+                   config=None) -> None:
+        """Tile shape, early-exit threshold, triangles=True for the reference's triangle coverage
+        (VKGS_RASTER_TRIANGLES) instead of the analytic square, and a resident-wave cap of the
+        raster (0: none); or a whole RasterConfig (config=, e.g. the autotuner's)."""
         cfg = _lib.RasterConfig()
-        cfg.tile_w, cfg.tile_h, cfg.t_min = tile_w, tile_h, t_min
-        cfg.flags = RASTER_TRIANGLES if triangles else 0
+        if config is not None:
+            C.memmove(C.byref(cfg), C.byref(config), C.sizeof(cfg))
+        else:
+            cfg.tile_w, cfg.tile_h, cfg.t_min = tile_w, tile_h, t_min
+            cfg.flags = RASTER_TRIANGLES if triangles else 0
+            cfg.waves_per_cu = waves_per_cu
         check(lib().vkgs_renderer_set_config(self._h, C.byref(cfg)), "vkgs_renderer_set_config")
 
     def render(self, cam, out: torch.Tensor | None = None, model=None) -> torch.Tensor:
@@ -273,7 +280,9 @@ class Renderer:
         return ranges, _download_u32(lp.value, k)
 
     def autotune(self, cam, out: torch.Tensor | None = None, frames: int = 10, model=None):
-        """Picks the fastest tile shape for this camera; returns (tile_w, tile_h, ms_per_frame)."""
+        """Picks the fastest tile shape (8x8 one or two tiles per wave, ...) and resident-wave cap for
+        this camera and keeps it; returns (tile_w, tile_h, ms_per_frame).  The whole chosen
+        RasterConfig is self.tuned afterwards."""
         w, h = cam.screen_size[0], cam.screen_size[1]
         if out is None:
             out = torch.empty((h, w, 4), dtype=torch.float32, device=_dev())
@@ -282,6 +291,7 @@ class Renderer:
         ms = C.c_float()
         check(lib().vkgs_renderer_autotune(self._h, C.byref(cam), mp, ptr(out), frames, C.byref(cfg), C.byref(ms),
                                            _stream()), "vkgs_renderer_autotune")
+        self.tuned = cfg
         return cfg.tile_w, cfg.tile_h, float(ms.value)
 
     def set_timing(self, on: bool) -> None:
