@@ -30,6 +30,11 @@ FP32_PEAK_TFLOPS = 157.3
 # FP32 operations per (pixel, splat) evaluation of the reference fragment and blend, exp counted as
 # 1: eigenframe coordinates, falloff, alpha, and the blend of colour and alpha
 RASTER_FLOPS_PER_EVAL = 20
+# FP32 operations per (pixel, entry) evaluation of the blend backward, counted the same way (FMA = 1,
+# exp and reciprocal = 1 each): eigenframe coordinates and the coverage test 9; falloff, alpha and its
+# clamp, transmittance recovery T / (1 - alpha), dL/dalpha, the behind-colour update and the opacity
+# and centre terms 25; the nine per-entry words (colour 3, conic 3) 6; their sum over the pixels 9
+BACKWARD_OPS_PER_EVAL = 49
 
 
 def parse_args(argv=None):
@@ -43,7 +48,7 @@ def parse_args(argv=None):
     ap.add_argument("--triangles", action="store_true",
                     help="triangle coverage (VKGS_RASTER_TRIANGLES), the reference's rasterization, for its cost")
     ap.add_argument("--timed-frames", type=int, default=20, help="eager frames with per-stage HIP events")
-    ap.add_argument("--cpu-frames", type=int, default=2, help="oracle frames for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-frames", type=int, default=3, help="oracle frames for cpu_baseline (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fwd-bwd-steps", type=int, default=5, help="timed fwd+bwd steps of the batch workload (0 = skip)")
     ap.add_argument("--no-fwd-bwd-tune", action="store_true",
@@ -226,6 +231,25 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both", tune: bool = Tru
             out["batched_pairs"] = D.last_info["pairs"]
     if "step" in graphed:
         graphed["step"].check()  # no replay overflowed its pair list
+    # rooflines: one eager step of each form with per-stage events and the rasters' work counters
+    # (outside the timed steps)
+    try:
+        rooflines = {}
+        if mode in ("both", "plain"):
+            profs = []
+            for (si, c), tgt in zip(views, targets):
+                w = 1.0 / (4 * c.screen_size[0] * c.screen_size[1])
+                t2 = tgt.reshape(-1, 4)
+                profs.append(D.profile_step([params[si]], [(0, c)], lambda f, t2=t2, w=w: torch.sign(f - t2) * w))
+            rooflines["plain"] = diff_roofline(profs)
+        if mode in ("both", "batched", "batched_eager"):
+            zero_grad()
+            rooflines["batched"] = diff_roofline([D.profile_step(params, views, l1_grad, spatial_order=True,
+                                                                 config=cfg)])
+        zero_grad()
+        out["roofline"] = rooflines
+    except Exception as e:  # reported, never hides the step figures
+        out["roofline"] = {"error": repr(e)}
     if mode == "both":
         out["batched_over_plain"] = round(out["batched_views_per_s"] / out["plain_views_per_s"], 3)
         out["batched_eager_over_plain"] = round(out["batched_eager_views_per_s"] / out["plain_views_per_s"], 3)
@@ -234,13 +258,54 @@ def bench_fwd_bwd(steps: int, scale: float, mode: str = "both", tune: bool = Tru
     return out
 
 
+def diff_roofline(profs):
+    """Roofline of the fwd+bwd step's dominant kernel, the raster backward (VALU-bound), from
+    vkgs_diff_stage_ms / vkgs_diff_raster_work of one eager step (profs: one profile per call, summed):
+    executed (pixel, entry) evaluations = 64 x the entries the waves stepped through."""
+    ms = {k: sum(p["ms"][k] for p in profs) for k in profs[0]["ms"]}
+    f_eval = 64.0 * sum(p["forward_entries"] for p in profs)
+    b_eval = 64.0 * sum(p["backward_entries"] for p in profs)
+    b_tf = BACKWARD_OPS_PER_EVAL * b_eval / (ms["backward_raster"] * 1e-3) / 1e12
+    f_tf = RASTER_FLOPS_PER_EVAL * f_eval / (ms["forward_raster"] * 1e-3) / 1e12
+    return {"kernel": "diff_raster_backward_pair_kernel / diff_raster_backward_kernel", "bound": "valu",
+            "achieved": round(b_tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(b_tf / FP32_PEAK_TFLOPS, 4), "traffic": None, "avg_ms": round(ms["backward_raster"], 4),
+            "work": f"{BACKWARD_OPS_PER_EVAL} FP32 ops (FMA, exp, rcp = 1) x {b_eval:.4g} executed (pixel, entry) "
+                    f"evaluations over {len(profs)} call(s)",
+            "forward_raster": {"achieved": round(f_tf, 3), "frac": round(f_tf / FP32_PEAK_TFLOPS, 4),
+                               "avg_ms": round(ms["forward_raster"], 4),
+                               "work": f"{RASTER_FLOPS_PER_EVAL} FP32 ops x {f_eval:.4g} executed evaluations"},
+            "stage_ms": {k: round(v, 4) for k, v in ms.items()}}
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def median(xs):
     xs = sorted(xs)
-    return xs[len(xs) // 2] if xs else float("nan")
+    if not xs:
+        return float("nan")
+    m = len(xs) // 2
+    return xs[m] if len(xs) % 2 else 0.5 * (xs[m - 1] + xs[m])
+
+
+def host_cpu():
+    """(CPU model name, logical CPUs this process may run on, logical CPUs of the machine)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 0
+    return model, usable, os.cpu_count() or 0
 
 
 def main(argv=None) -> int:
@@ -443,11 +508,15 @@ def main(argv=None) -> int:
                 times.append(time.perf_counter() - c0)
             assert ref["visible"] == V
             cpu_fps = 1.0 / median(times)
+            model, usable, total = host_cpu()
             result["cpu_baseline"] = {
                 "value": round(cpu_fps, 4), "unit": "frames/s", "cores": oracle.num_threads(), "kind": "port",
-                "sample": f"{args.cpu_frames} full frames of the same {wl.name} workload (rank, stable sort, inverse, "
-                          "projection, back-to-front draw) by the C oracle, median"}
-            log(f"cpu oracle {median(times):.2f} s/frame on {oracle.num_threads()} threads")
+                "sample": f"median of {args.cpu_frames} full frames of the same {wl.name} workload (rank, stable "
+                          "sort, inverse, projection, back-to-front draw) by the C oracle with OpenMP",
+                "frame_s": [round(t, 3) for t in times], "cpu_model": model, "cpus_usable": usable,
+                "cpus_machine": total}
+            log(f"cpu oracle {median(times):.2f} s/frame on {oracle.num_threads()} threads ({model}, "
+                f"{usable} of {total} CPUs usable)")
         except Exception as e:  # the baseline never blocks the GPU figure
             result["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(result), flush=True)

@@ -333,7 +333,7 @@ int vkgs_renderer_raster_work(vkgs_renderer* r, uint64_t* wave_entries);
  * stays a normal float (t_min = 0 therefore omits at most 2^-20 of a pixel's
  * colour on this path).
  * ------------------------------------------------------------------------- */
-/* sh and grad_sh must be 16-byte aligned (VKGS_ERR_INVALID_ARG otherwise). */
+/* sh, quats, grad_sh and grad_quats must be 16-byte aligned (VKGS_ERR_INVALID_ARG otherwise). */
 typedef struct vkgs_diff_scene {
   const float* means;
   const float* log_scales;
@@ -379,6 +379,22 @@ int vkgs_diff_info(vkgs_diff_ctx* ctx, uint32_t* visible, uint64_t* pairs, void*
  * hipGraph on one stream, after an eager call has sized every buffer for the
  * same scenes and views, and check this after replays. */
 int vkgs_diff_check(vkgs_diff_ctx* ctx, uint64_t* pairs_needed, void* stream);
+/* Per-stage timing of the differentiable path (the reference's timestamp
+ * breakdown, engine.cc:1031-1045, applied to this path): while enabled, each
+ * forward and backward records HIP events around its stages and its raster
+ * kernels count the list entries each wave stepped through.  For eager calls
+ * (not inside a captured graph). */
+int vkgs_diff_set_timing(vkgs_diff_ctx* ctx, int enable);
+/* ms of the last timed forward + backward (synchronous): [forward up to the
+ * raster (cull, project, sorts, binning), forward raster, forward total,
+ * backward raster, gradient reduction, projection backward, backward total,
+ * 0]. */
+int vkgs_diff_stage_ms(vkgs_diff_ctx* ctx, float ms[8]);
+/* Raster work of the last timed forward and backward (synchronous): list
+ * entries stepped through, summed over waves (the two-tiles-per-wave kernels
+ * count each tile's own entries); executed (pixel, entry) evaluations = 64 x
+ * each.  Either pointer may be NULL. */
+int vkgs_diff_raster_work(vkgs_diff_ctx* ctx, uint64_t* forward_entries, uint64_t* backward_entries);
 /* Autotuner of the differentiable path (no counterpart in the reference,
  * whose renderer has fixed-function rasterization): times steps of
  * vkgs_diff_forward + vkgs_diff_backward on these scenes and views (their

@@ -125,3 +125,28 @@ def test_diff_autotune_choice_renders_the_same_image():
     plain = [i.detach().cpu().numpy() for i in D.render_batch([p], views)]
     for a, b in zip(tuned, plain):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("flags", [PAIRS_NEVER, PAIRS_ALWAYS])
+def test_diff_timing_counts_work_and_keeps_results(flags):
+    """vkgs_diff_set_timing: per-stage times are positive and add up, the raster work counters obey
+    0 < backward entries <= forward entries <= pairs (the backward walks each tile's list only up to
+    its pixels' last entry), and the gradients equal an untimed step's."""
+    sc = diff_scene(8000, 53)
+    views = [(0, S.orbit_camera(160, 120, 4.0)), (0, S.orbit_camera(200, 100, 3.0, theta=0.9))]
+    spans, total = D._spans([(0, c, None) for _, c in views])
+    w = torch.tensor(np.random.default_rng(9).uniform(0.5, 1.5, (total, 4)), dtype=torch.float32, device="cuda")
+    cfg = cfg_of(flags=flags)
+    p = D.scene_tensors(sc)
+    prof = D.profile_step([p], views, lambda f: w, config=cfg)
+    g_timed = {k: p[k].grad.detach().cpu().numpy() for k in NAMES}
+    ms = prof["ms"]
+    print("diff timing", flags, ms, prof["forward_entries"], prof["backward_entries"], D.last_info, flush=True)
+    assert all(v > 0 for v in ms.values())
+    assert ms["forward_total"] >= ms["forward_raster"] and ms["backward_total"] >= ms["backward_raster"]
+    p2 = D.scene_tensors(sc)
+    flat, _ = D.render_batch_flat([p2], views, config=cfg)
+    (flat * w).sum().backward()
+    assert 0 < prof["backward_entries"] <= prof["forward_entries"] <= D.last_info["pairs"]
+    for k in NAMES:
+        np.testing.assert_array_equal(g_timed[k], p2[k].grad.detach().cpu().numpy())

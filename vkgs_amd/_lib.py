@@ -123,6 +123,9 @@ SIGNATURES = {
     "vkgs_diff_backward": (_I, [_P, _P, _U32, _P, _U32, _P]),
     "vkgs_diff_info": (_I, [_P, _P, _P, _P]),
     "vkgs_diff_check": (_I, [_P, _P, _P]),
+    "vkgs_diff_set_timing": (_I, [_P, _I]),
+    "vkgs_diff_stage_ms": (_I, [_P, _P]),
+    "vkgs_diff_raster_work": (_I, [_P, _P, _P]),
     "vkgs_diff_autotune": (_I, [_P, _P, _U32, _P, _U32, _P, _U32, _P, _P, _P]),
 }
 

@@ -27,14 +27,13 @@
 #include "diff.h"
 #include "device.h"
 #include "kernels.h"
+#include "stage.h"
 
 namespace vkgs {
 namespace {
 
 constexpr float kAlphaMax = 0.999f;
 
-
-__device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
 // ------------------------------------------------------ forward projection ---
 
@@ -57,8 +56,8 @@ __device__ __forceinline__ void quat_rot(const float q[4], float R[9]) {
 
 // Normalized (x, y, z, w) from the PLY order (w, x, y, z); returns |q|.
 __device__ __forceinline__ float load_quat(const DiffScene& sc, uint32_t id, float q[4]) {
-  const float w = sc.quats[4 * (size_t)id + 0], x = sc.quats[4 * (size_t)id + 1];
-  const float y = sc.quats[4 * (size_t)id + 2], z = sc.quats[4 * (size_t)id + 3];
+  const float4 q4 = ld_global(reinterpret_cast<const float4*>(sc.quats) + id);
+  const float w = q4.x, x = q4.y, y = q4.z, z = q4.w;
   const float len = sqrtf(((x * x + y * y) + z * z) + w * w);
   q[0] = x / len;
   q[1] = y / len;
@@ -547,8 +546,42 @@ __global__ __launch_bounds__(256) void diff_bin_emit_packed_kernel(DiffLaunch L,
   }
 }
 
+// ------------------------------------------------- differentiable raster ---
+//
+// Records are restaged per tile as in the plain raster (stage_record: the
+// inverse rot_scale rows scaled by kFrameScale with their offsets at the
+// tile's 64 x 64 block corner, -log2(opacity)), so a pixel's frame
+// coordinates u' are two FMAs each and alpha = min(exp2(-|u'|^2 - log2(1 /
+// opacity)), kAlphaMax) over the square max(|u'_x|, |u'_y|) <= kCoverScaled.
+// The forward and the backward stage the same records at the same corner, so
+// the backward recomputes every alpha bit for bit.  Each pixel stops on its
+// own once T < t_min.
+
+// One forward (pixel, entry) step; idx = the entry's 1-based place in the
+// tile list.  A pixel composites while T >= t_min (T only decreases, so once
+// below it stays below); while it does, last = idx, so a pixel that stops
+// keeps the entry that stopped it.  The state is re-derived from T on every
+// step rather than carried as a flag, which keeps it in a lane mask.
+__device__ __forceinline__ void diff_step(const float4& a, const float4& b, const float4& c, float lx, float ly,
+                                          uint32_t idx, float t_min, float& T, float& cr, float& cg, float& cb,
+                                          float& aa, uint32_t& last) {
+  const bool alive = T >= t_min;
+  const float ux = fmaf(a.x, lx, fmaf(a.y, ly, a.z));
+  const float uy = fmaf(b.x, lx, fmaf(b.y, ly, b.z));
+  const float m = fmaxf(fabsf(ux), fabsf(uy));
+  const float e = __builtin_amdgcn_exp2f(-fmaf(ux, ux, fmaf(uy, uy, a.w)));
+  const float alpha = ((m <= kCoverScaled) & alive) ? fminf(e, kAlphaMax) : 0.f;
+  last = alive ? idx : last;
+  const float ta = T * alpha;
+  cr = fmaf(c.x, ta, cr);
+  cg = fmaf(c.y, ta, cg);
+  cb = fmaf(c.z, ta, cb);
+  aa = fmaf(alpha, ta, aa);
+  T = T - ta;
+}
+
 // Forward raster of one global tile; optionally keeps the per-pixel state the
-// backward needs.  Same compositing as the plain raster, with alpha clamped.
+// backward needs (final T and the entries each pixel composited).
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                              const uint32_t* __restrict__ vals,
@@ -571,10 +604,12 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
   const int col0 = __builtin_amdgcn_readfirstlane((int)tx * TW), col1 = col0 + TW - 1;
   const int row0 = __builtin_amdgcn_readfirstlane((int)ty * TH + (t / kWave) * kRowsPerWave);
   const int row1 = row0 + kRowsPerWave - 1;
+  const int bx0 = col0 & ~(kRasterBlock - 1), by0 = __builtin_amdgcn_readfirstlane(((int)ty * TH) & ~(kRasterBlock - 1));
+  const float X0 = (float)bx0, Y0 = (float)by0;
+  const float lx = (float)(px - bx0) + 0.5f, ly = (float)(py - by0) + 0.5f;
   const uint32_t start = ranges[2 * gtile], end = ranges[2 * gtile + 1];
   const uint32_t items = end - start;
   const uint32_t batches = (items + kN - 1) / kN;
-  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   const float t_min = L.t_min;
 
   float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
@@ -584,15 +619,13 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
     p1 = rec[3 * (size_t)k + 1];
     p2 = rec[3 * (size_t)k + 2];
   }
-  // T >= t_min is "still compositing" (T only decreases; pixels outside the
-  // image start at T = 0 and store nothing)
+  // alive = "still compositing" (T only decreases; pixels outside the image
+  // start at T = 0 and store nothing)
   float T = inside ? 1.f : 0.f, cr = 0.f, cg = 0.f, cb = 0.f, aa = 0.f;
-  uint32_t last = items;
+  uint32_t last = 0, walked = 0;
   for (uint32_t b = 0; b < batches; ++b) {
     const int buf = b & 1;
-    s_rec[buf][0][t] = p0;
-    s_rec[buf][1][t] = p1;
-    s_rec[buf][2][t] = p2;
+    stage_record(p0, p1, p2, b * kN + t < items, X0, Y0, &s_rec[buf][0][t], &s_rec[buf][1][t], &s_rec[buf][2][t]);
     if (__syncthreads_count(T >= t_min) == 0) break;
     const uint32_t next = (b + 1) * kN + t;
     if (next < items) {
@@ -602,38 +635,22 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
       p2 = rec[3 * (size_t)k + 2];
     }
     const int cnt = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kN, items - b * kN));
-    bool alive = T >= t_min;
-    if (__ballot(alive) != 0) {
-      for (int e = 0; e < cnt; ++e) {
-        const float4 a = s_rec[buf][0][e];
-        const float4 bb = s_rec[buf][1][e];
-        const float4 c = s_rec[buf][2][e];
-        if (!kOneWave) {
-          const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
-          const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
-          const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
-          const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
-          if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
-        }
-        const float dx = fx - a.x, dy = fy - a.y;
-        const float ux = fmaf(a.z, dx, a.w * dy);
-        const float uy = fmaf(bb.x, dx, bb.y * dy);
-        const bool in = (fabsf(ux) <= 3.f) & (fabsf(uy) <= 3.f) & alive;
-        const float q = fmaf(ux, ux, uy * uy);
-        const float alpha = in ? fminf(bb.z * gauss_falloff(q), kAlphaMax) : 0.f;
-        const float ta = T * alpha;
-        cr = fmaf(c.x, ta, cr);
-        cg = fmaf(c.y, ta, cg);
-        cb = fmaf(c.z, ta, cb);
-        aa = fmaf(alpha, ta, aa);
-        T = T - ta;
-        const bool was = alive;
-        alive = T >= t_min;
-        if (was && !alive) last = b * kN + e + 1;  // entries processed, nearest first
-        if (__ballot(alive) == 0) break;
+    if (__ballot(T >= t_min) == 0) continue;
+    for (int e = 0; e < cnt; ++e) {
+      const float4 a = s_rec[buf][0][e], bb = s_rec[buf][1][e], c = s_rec[buf][2][e];
+      if (!kOneWave) {
+        const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
+        const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
+        const int bx0r = (int)(bx & 0xffffu), bx1r = (int)(bx >> 16);
+        const int by0r = (int)(by & 0xffffu), by1r = (int)(by >> 16);
+        if (by1r < row0 || by0r > row1 || bx1r < col0 || bx0r > col1) continue;
       }
+      ++walked;
+      diff_step(a, bb, c, lx, ly, b * kN + e + 1, t_min, T, cr, cg, cb, aa, last);
+      if (__ballot(T >= t_min) == 0) break;
     }
   }
+  if (T >= t_min) last = items;
   if (inside) {
     const size_t pix = (size_t)py * W + px;
     view.out[pix] = make_float4(cr, cg, cb, aa + T);
@@ -642,6 +659,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
       view.last[pix] = last;
     }
   }
+  if (L.work && t % kWave == 0) L.work[(size_t)blockIdx.x * (kN / kWave) + t / kWave] = walked;
 }
 
 // -------------------------------------------------------------- backward ---
@@ -669,19 +687,56 @@ __device__ __forceinline__ float half_row_sum(float x) {
   return dpp_add<0x124>(x);
 }
 
+// Finite offsets for a backward lane with no record: its coverage test fails
+// and every gradient term it forms is an exact zero times a finite value.
+constexpr float kInertFinite = 1e18f;
+// 1 / kFrameScale^2 = 2 ln 2: converts a sum over the scaled frame back to
+// the pixel frame.
+constexpr float kInvScale2 = 1.3862943611198906f;
+
+// The 9 gradient words of one (tile, entry) pair from its per-pixel sums:
+//   u[0..2] = sum g_rgb * T alpha (colour),
+//   u[3] = sum r (opacity, r = dL/dalpha * raw alpha, 0 where alpha is clamped),
+//   u[4..6] = sum r u'x^2, r u'x u'y, r u'y^2,  u[7..8] = sum r u'x, r u'y,
+// with u' = M' d the scaled frame coordinates of the pixel offset d from the
+// centre (M' the staged rows (a.x, a.y), (b.x, b.y)).  alpha = opacity *
+// exp(-q / 2) with q = d^T C d (C the conic), so dL/dq = -r / 2 and
+//   dL/d(opacity) = sum r / opacity,
+//   dL/dC = sum dL/dq d d^T = M'^-1 S M'^-T,  S = -1/2 [[u4, u5], [u5, u6]],
+//   dL/dcentre = sum dL/dq (-2 C d) = -2 / kFrameScale^2 M'^T s,  s = -1/2 (u7, u8),
+// written as (colour 3, opacity, C00, 2 C01, C11, centre x, centre y).  The
+// opacity is recovered from the staged -log2(opacity) in a.w.
+__device__ __forceinline__ void pair_words(const float u[9], const float4& a, const float4& b,
+                                           float* __restrict__ out) {
+  const float opacity = __builtin_amdgcn_exp2f(-a.w);
+  out[0] = u[0];
+  out[1] = u[1];
+  out[2] = u[2];
+  out[3] = opacity > 0.f ? u[3] / opacity : 0.f;
+  // M'^-1 = adj(M') / det; det = ab(c^2 + s^2)-like products of one sign, no cancellation
+  const float det = a.x * b.y - a.y * b.x;
+  const float id = 1.f / det;
+  const float A00 = b.y * id, A01 = -a.y * id, A10 = -b.x * id, A11 = a.x * id;
+  const float p = -0.5f * u[4], r = -0.5f * u[5], q = -0.5f * u[6];
+  const float t00 = A00 * p + A01 * r, t01 = A00 * r + A01 * q;  // row 0 of A S
+  const float t10 = A10 * p + A11 * r, t11 = A10 * r + A11 * q;  // row 1 of A S
+  out[4] = t00 * A00 + t01 * A01;
+  out[5] = 2.f * (t00 * A10 + t01 * A11);
+  out[6] = t10 * A10 + t11 * A11;
+  out[7] = kInvScale2 * (a.x * u[7] + b.x * u[8]);
+  out[8] = kInvScale2 * (a.y * u[7] + b.y * u[8]);
+}
+
 // Re-walks one tile back to front from the stored per-pixel state.  Per list
-// entry 9 sums over the tile's pixels are formed: within each half of a
-// 16-lane row with DPP, then stored by that half row's last lane into a slot
-// of its own (8 per wave per word).  With one wave per tile every slot has one
-// writer and plain LDS stores suffice; LDS float atomics, used only when
-// several waves share a tile, cost most of the kernel's time in a measured
-// variant.  The sums are
-// colour 3, opacity 1, dq*dx^2, dq*dx*dy, dq*dy^2, dq*dx, dq*dy (dq = dL/dq of
-// the exponent's quadratic form); the conic and centre words follow from the
-// last five and the entry's conic rows when the batch is flushed to the pair's
-// 9 words of gpart at its slot-order index q; the tile's entries past every
-// pixel's last one get zeros, so each pair is written once and gpart needs no
-// fill.  A reduction then sums each slot's contiguous run of pairs.
+// entry 9 sums over the tile's pixels are formed (pair_words): within each
+// half of a 16-lane row with DPP, then stored by that half row's last lane
+// into a slot of its own (8 per wave per word).  With one wave per tile every
+// slot has one writer and plain LDS stores suffice; LDS float atomics, used
+// only when several waves share a tile, cost most of the kernel's time in a
+// measured variant.  Each batch is flushed to the pairs' 9 words of gpart at
+// their slot-order index q; the tile's entries past every pixel's last one
+// get zeros, so each pair is written once and gpart needs no fill.  A
+// reduction then sums each slot's contiguous run of pairs.
 template <int TW, int TH, int KB>
 __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch L, const float4* __restrict__ rec,
                                                                       const uint32_t* __restrict__ vals,
@@ -713,8 +768,10 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   const int col0 = __builtin_amdgcn_readfirstlane((int)tx * TW), col1 = col0 + TW - 1;
   const int row0 = __builtin_amdgcn_readfirstlane((int)ty * TH + (t / kWave) * kRowsPerWave);
   const int row1 = row0 + kRowsPerWave - 1;
+  const int bx0 = col0 & ~(kRasterBlock - 1), by0 = __builtin_amdgcn_readfirstlane(((int)ty * TH) & ~(kRasterBlock - 1));
+  const float X0 = (float)bx0, Y0 = (float)by0;
+  const float lx = (float)(px - bx0) + 0.5f, ly = (float)(py - by0) + 0.5f;
   const uint32_t end = ranges[2 * gtile + 1];
-  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   const bool half_last = (__lane_id() & 7u) == 7u;  // lanes 7 and 15 of each DPP row
   const uint32_t row = __lane_id() >> 3;              // half-row slot 0..7
 
@@ -737,6 +794,7 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   // r <- alpha * x + (1 - alpha) * r, so their g-weighted sum does too
   float acc = g.w;
   const float gw2 = 2.f * g.w;
+  uint32_t walked = 0;  // entries this wave stepped through (work counter)
 
   // traversal index i (0 = nearest) maps to list position end - 1 - i; walk i downwards
   for (uint32_t hi = max_last; hi > 0;) {
@@ -748,38 +806,33 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const uint32_t j = vals[end - 1 - i];
       const uint32_t k = pslot[j];
       s_j[t] = j;
-      s_rec[0][t] = rec[3 * (size_t)k];
-      s_rec[1][t] = rec[3 * (size_t)k + 1];
-      s_rec[2][t] = rec[3 * (size_t)k + 2];
+      const float4 r0 = rec[3 * (size_t)k], r1 = rec[3 * (size_t)k + 1], r2 = rec[3 * (size_t)k + 2];
+      stage_record(r0, r1, r2, true, X0, Y0, &s_rec[0][t], &s_rec[1][t], &s_rec[2][t]);
 #pragma unroll
-      for (int j = 0; j < 9; ++j)
+      for (int w = 0; w < 9; ++w)
 #pragma unroll
-        for (int r = 0; r < 8; ++r) s_g[j][r][t] = 0.f;
+        for (int r = 0; r < 8; ++r) s_g[w][r][t] = 0.f;
     }
     __syncthreads();
     for (int e = 0; e < cnt; ++e) {
       const uint32_t i = hi - 1 - e;
-      const float4 a = s_rec[0][e];
-      const float4 bb = s_rec[1][e];
-      const float4 c = s_rec[2][e];
+      const float4 a = s_rec[0][e], bb = s_rec[1][e], c = s_rec[2][e];
       if (!kOneWave) {
         const uint32_t bx = __builtin_amdgcn_readfirstlane(__float_as_uint(bb.w));
         const uint32_t by = __builtin_amdgcn_readfirstlane(__float_as_uint(c.w));
-        const int bx0 = (int)(bx & 0xffffu), bx1 = (int)(bx >> 16);
-        const int by0 = (int)(by & 0xffffu), by1 = (int)(by >> 16);
-        if (by1 < row0 || by0 > row1 || bx1 < col0 || bx0 > col1) continue;
+        const int bx0r = (int)(bx & 0xffffu), bx1r = (int)(bx >> 16);
+        const int by0r = (int)(by & 0xffffu), by1r = (int)(by >> 16);
+        if (by1r < row0 || by0r > row1 || bx1r < col0 || bx0r > col1) continue;
       }
-      const float dx = fx - a.x, dy = fy - a.y;
-      const float ux = fmaf(a.z, dx, a.w * dy);
-      const float uy = fmaf(bb.x, dx, bb.y * dy);
-      const bool in = (i < last) & (fabsf(ux) <= 3.f) & (fabsf(uy) <= 3.f);
+      const float ux = fmaf(a.x, lx, fmaf(a.y, ly, a.z));
+      const float uy = fmaf(bb.x, lx, fmaf(bb.y, ly, bb.z));
+      const bool in = (i < last) & (fmaxf(fabsf(ux), fabsf(uy)) <= kCoverScaled);
       if (__ballot(in) == 0) continue;
+      ++walked;
       // every lane takes part from here (DPP reads all lanes); lanes with
-      // !in get G = 0, hence alpha = 0: their T (rcp(1) = 1), acc and every
-      // contribution stay unchanged or zero
-      const float q = fmaf(ux, ux, uy * uy);
-      const float G = in ? gauss_falloff(q) : 0.f;
-      const float raw = bb.z * G;
+      // !in get raw alpha 0: their T (rcp(1) = 1), acc and every term stay
+      // unchanged or zero
+      const float raw = in ? __builtin_amdgcn_exp2f(-fmaf(ux, ux, fmaf(uy, uy, a.w))) : 0.f;
       const float alpha = fminf(raw, kAlphaMax);
       const float om = 1.f - alpha;
       T = T * __builtin_amdgcn_rcpf(om);  // transmittance in front of entry i
@@ -788,18 +841,15 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const float gc = fmaf(g.z, c.z, fmaf(g.y, c.y, g.x * c.x));
       const float dalpha = T * (fmaf(gw2, alpha, gc) - acc);
       acc = fmaf(alpha, fmaf(g.w, alpha, gc), om * acc);
-      // dL/d(opacity) = dalpha * G and dL/dq = -dalpha * alpha / 2 =
-      // -(opacity / 2) * dop (both 0 where alpha is clamped): the q words are
-      // summed as dop * (...) and scaled by -opacity / 2 at the flush
-      const float dop = raw > kAlphaMax ? 0.f : dalpha * G;
-      const float dpx = dop * dx, dpy = dop * dy;
+      const float rr = raw > kAlphaMax ? 0.f : dalpha * raw;
+      const float dpx = rr * ux, dpy = rr * uy;
       float s0 = half_row_sum(g.x * ta);
       float s1 = half_row_sum(g.y * ta);
       float s2 = half_row_sum(g.z * ta);
-      float s3 = half_row_sum(dop);
-      float s4 = half_row_sum(dpx * dx);
-      float s5 = half_row_sum(dpx * dy);
-      float s6 = half_row_sum(dpy * dy);
+      float s3 = half_row_sum(rr);
+      float s4 = half_row_sum(dpx * ux);
+      float s5 = half_row_sum(dpx * uy);
+      float s6 = half_row_sum(dpy * uy);
       float s7 = half_row_sum(dpx);
       float s8 = half_row_sum(dpy);
       // keep the last DPP add ahead of the branch (every lane computes it), so
@@ -831,28 +881,12 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
     }
     __syncthreads();
     if (t < cnt) {
-      // conic rows (a.z, a.w) = (i00, i01) and (bb.x, bb.y) = (i10, i11):
-      // sum dq (i00 ux + i10 uy) = P sum dq dx + Q sum dq dy, and likewise
-      const float4 a = s_rec[0][t], bb = s_rec[1][t];
-      const float P = a.z * a.z + bb.x * bb.x, Q = a.z * a.w + bb.x * bb.y, R = a.w * a.w + bb.y * bb.y;
       float u[9];
 #pragma unroll
-      for (int j = 0; j < 9; ++j)
-        u[j] = ((s_g[j][0][t] + s_g[j][1][t]) + (s_g[j][2][t] + s_g[j][3][t])) +
-               ((s_g[j][4][t] + s_g[j][5][t]) + (s_g[j][6][t] + s_g[j][7][t]));
-      const float hq = -0.5f * bb.z;  // dq = hq * dop
-#pragma unroll
-      for (int j = 4; j < 9; ++j) u[j] *= hq;
-      float* out = gpart + 9 * (size_t)s_j[t];  // at the pair's slot-order index
-      out[0] = u[0];
-      out[1] = u[1];
-      out[2] = u[2];
-      out[3] = u[3];
-      out[4] = u[4];
-      out[5] = 2.f * u[5];
-      out[6] = u[6];
-      out[7] = -2.f * (P * u[7] + Q * u[8]);
-      out[8] = -2.f * (Q * u[7] + R * u[8]);
+      for (int w = 0; w < 9; ++w)
+        u[w] = ((s_g[w][0][t] + s_g[w][1][t]) + (s_g[w][2][t] + s_g[w][3][t])) +
+               ((s_g[w][4][t] + s_g[w][5][t]) + (s_g[w][6][t] + s_g[w][7][t]));
+      pair_words(u, s_rec[0][t], s_rec[1][t], gpart + 9 * (size_t)s_j[t]);  // at the pair's q
     }
     hi = lo;
   }
@@ -862,8 +896,9 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   for (uint32_t i = max_last + t; i < items; i += kN) {
     float* out = gpart + 9 * (size_t)vals[end - 1 - i];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) out[j] = 0.f;
+    for (int w = 0; w < 9; ++w) out[w] = 0.f;
   }
+  if (L.work && t % kWave == 0) L.work[(size_t)blockIdx.x * (kN / kWave) + t / kWave] = walked;
 }
 
 // ------------------------------------------- two 8x8 tiles per wave ---
@@ -909,6 +944,9 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
   const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
   const int px = (int)tx * 8 + l % 8, py0 = (int)ty * 8 + l / 8, py1 = py0 + 4;
   const bool in0 = has_tile && px < (int)W && py0 < (int)H, in1 = has_tile && px < (int)W && py1 < (int)H;
+  const int bx0 = ((int)tx * 8) & ~(kRasterBlock - 1), by0 = ((int)ty * 8) & ~(kRasterBlock - 1);
+  const float X0 = (float)bx0, Y0 = (float)by0;
+  const float lx = (float)(px - bx0) + 0.5f, ly0 = (float)(py0 - by0) + 0.5f, ly1 = ly0 + 4.f;
   uint32_t end = 0, items = 0;
   if (has_tile) {
     end = ranges[2 * gtile + 1];
@@ -916,7 +954,6 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
   }
   const uint32_t items_a = __builtin_amdgcn_readlane(items, 0), items_b = __builtin_amdgcn_readlane(items, kPairHalf);
   const uint32_t batches = (max(items_a, items_b) + kPairHalf - 1) / kPairHalf;
-  const float fx = (float)px + 0.5f, fy0 = (float)py0 + 0.5f, fy1 = (float)py1 + 0.5f;
   const float t_min = L.t_min;
 
   float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
@@ -928,17 +965,16 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
   }
   float T0 = in0 ? 1.f : 0.f, r0 = 0.f, g0 = 0.f, b0 = 0.f, a0 = 0.f;
   float T1 = in1 ? 1.f : 0.f, r1 = 0.f, g1 = 0.f, b1 = 0.f, a1 = 0.f;
-  uint32_t last0 = items, last1 = items;
-  bool alive0 = T0 >= t_min, alive1 = T1 >= t_min;
+  uint32_t last0 = 0, last1 = 0;
+  uint32_t walked_a = 0, walked_b = 0;  // each tile's own list entries stepped through (work counter)
   for (uint32_t b = 0; b < batches; ++b) {
     const int buf = b & 1;
-    // entries past a half's list (the other tile's is longer) are staged with opacity 0: alpha is 0, so
-    // they change no pixel
-    const bool valid = b * kPairHalf + l < items;
-    s_rec[buf][0][h * kStride + l] = p0;
-    s_rec[buf][1][h * kStride + l] = valid ? p1 : make_float4(0.f, 0.f, 0.f, 0.f);
-    s_rec[buf][2][h * kStride + l] = p2;
-    if (__syncthreads_count(alive0 | alive1) == 0) break;
+    // entries past a half's list (the other tile's is longer) are staged as
+    // records that cover nothing, so they change no pixel
+    const int at = h * kStride + l;
+    stage_record(p0, p1, p2, b * kPairHalf + l < items, X0, Y0, &s_rec[buf][0][at], &s_rec[buf][1][at],
+                 &s_rec[buf][2][at]);
+    if (__syncthreads_count((T0 >= t_min) | (T1 >= t_min)) == 0) break;
     const uint32_t next = (b + 1) * kPairHalf + l;
     if (next < items) {
       const uint32_t k = pslot[vals[end - 1 - next]];
@@ -950,50 +986,28 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
     const int cnt_a = items_a > done ? (int)min((uint32_t)kPairHalf, items_a - done) : 0;
     const int cnt_b = items_b > done ? (int)min((uint32_t)kPairHalf, items_b - done) : 0;
     const int cnt = max(cnt_a, cnt_b);
-    if (__ballot(alive0 | alive1) == 0) continue;
+    if (__ballot((T0 >= t_min) | (T1 >= t_min)) == 0) continue;
     const float4* s0 = s_rec[buf][0] + h * kStride;
     const float4* s1 = s_rec[buf][1] + h * kStride;
     const float4* s2 = s_rec[buf][2] + h * kStride;
-    for (int e = 0; e < cnt; ++e) {
+    int e = 0;
+    for (; e < cnt; ++e) {
       const float4 a = s0[e], bb = s1[e], c = s2[e];
-      const uint32_t idx = b * kPairHalf + e + 1;  // entries processed once this one is
-      {
-        const float dx = fx - a.x, dy = fy0 - a.y;
-        const float ux = fmaf(a.z, dx, a.w * dy);
-        const float uy = fmaf(bb.x, dx, bb.y * dy);
-        const bool in = (fabsf(ux) <= 3.f) & (fabsf(uy) <= 3.f) & alive0;
-        const float q = fmaf(ux, ux, uy * uy);
-        const float alpha = in ? fminf(bb.z * gauss_falloff(q), kAlphaMax) : 0.f;
-        const float ta = T0 * alpha;
-        r0 = fmaf(c.x, ta, r0);
-        g0 = fmaf(c.y, ta, g0);
-        b0 = fmaf(c.z, ta, b0);
-        a0 = fmaf(alpha, ta, a0);
-        T0 = T0 - ta;
-        const bool was = alive0;
-        alive0 = T0 >= t_min;
-        if (was && !alive0) last0 = idx;
+      const uint32_t idx = done + e + 1;
+      diff_step(a, bb, c, lx, ly0, idx, t_min, T0, r0, g0, b0, a0, last0);
+      diff_step(a, bb, c, lx, ly1, idx, t_min, T1, r1, g1, b1, a1, last1);
+      if (__ballot((T0 >= t_min) | (T1 >= t_min)) == 0) {
+        ++e;
+        break;
       }
-      {
-        const float dx = fx - a.x, dy = fy1 - a.y;
-        const float ux = fmaf(a.z, dx, a.w * dy);
-        const float uy = fmaf(bb.x, dx, bb.y * dy);
-        const bool in = (fabsf(ux) <= 3.f) & (fabsf(uy) <= 3.f) & alive1;
-        const float q = fmaf(ux, ux, uy * uy);
-        const float alpha = in ? fminf(bb.z * gauss_falloff(q), kAlphaMax) : 0.f;
-        const float ta = T1 * alpha;
-        r1 = fmaf(c.x, ta, r1);
-        g1 = fmaf(c.y, ta, g1);
-        b1 = fmaf(c.z, ta, b1);
-        a1 = fmaf(alpha, ta, a1);
-        T1 = T1 - ta;
-        const bool was = alive1;
-        alive1 = T1 >= t_min;
-        if (was && !alive1) last1 = idx;
-      }
-      if (__ballot(alive0 | alive1) == 0) break;
     }
+    walked_a += (uint32_t)min(e, cnt_a);
+    walked_b += (uint32_t)min(e, cnt_b);
   }
+  // a pixel still compositing went through its whole list (idx also ran over
+  // the padding of a shorter half)
+  if (T0 >= t_min) last0 = items;
+  if (T1 >= t_min) last1 = items;
   if (in0) {
     const size_t pix = (size_t)py0 * W + px;
     view.out[pix] = make_float4(r0, g0, b0, a0 + T0);
@@ -1010,11 +1024,12 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
       view.last[pix] = last1;
     }
   }
+  if (L.work && l == 0 && has_tile) L.work[slot] = h ? walked_b : walked_a;
 }
 
 // Backward of one (pixel, entry) with the arithmetic of
-// diff_raster_backward_kernel: a pixel outside the footprint or past its last
-// entry gets G = 0, so its terms are zero and T and acc stay.
+// diff_raster_backward_kernel: a pixel outside the square or past its last
+// entry gets raw alpha 0, so its terms are zero and T and acc stay.
 struct PixState {
   float T, acc, gw2;
   float4 g;
@@ -1022,31 +1037,28 @@ struct PixState {
 };
 
 struct PixCover {
-  float dx, dy, ux, uy;
+  float ux, uy;
   bool in;
 };
 
-__device__ __forceinline__ PixCover bwd_cover(const PixState& ps, const float4& a, const float4& bb, float fx, float fy,
+__device__ __forceinline__ PixCover bwd_cover(const PixState& ps, const float4& a, const float4& bb, float lx, float ly,
                                               uint32_t i) {
   PixCover pc;
-  pc.dx = fx - a.x;
-  pc.dy = fy - a.y;
-  pc.ux = fmaf(a.z, pc.dx, a.w * pc.dy);
-  pc.uy = fmaf(bb.x, pc.dx, bb.y * pc.dy);
-  pc.in = (i < ps.last) & (fabsf(pc.ux) <= 3.f) & (fabsf(pc.uy) <= 3.f);
+  pc.ux = fmaf(a.x, lx, fmaf(a.y, ly, a.z));
+  pc.uy = fmaf(bb.x, lx, fmaf(bb.y, ly, bb.z));
+  pc.in = (i < ps.last) & (fmaxf(fabsf(pc.ux), fabsf(pc.uy)) <= kCoverScaled);
   return pc;
 }
 
 // The per-pixel terms of one entry: ta = T alpha (the colour words' weight),
-// dop = dL/d(opacity term) and dop dx, dop dy; updates T and acc.
+// rr = dL/dalpha * raw alpha (0 where alpha is clamped) and rr u'x, rr u'y;
+// updates T and acc.
 struct PixTerms {
-  float ta, dop, dpx, dpy;
+  float ta, rr, dpx, dpy;
 };
 
-__device__ __forceinline__ PixTerms bwd_pixel(PixState& ps, const PixCover& pc, const float4& bb, const float4& c) {
-  const float q = fmaf(pc.ux, pc.ux, pc.uy * pc.uy);
-  const float G = pc.in ? gauss_falloff(q) : 0.f;
-  const float raw = bb.z * G;
+__device__ __forceinline__ PixTerms bwd_pixel(PixState& ps, const PixCover& pc, const float4& a, const float4& c) {
+  const float raw = pc.in ? __builtin_amdgcn_exp2f(-fmaf(pc.ux, pc.ux, fmaf(pc.uy, pc.uy, a.w))) : 0.f;
   const float alpha = fminf(raw, kAlphaMax);
   const float om = 1.f - alpha;
   ps.T = ps.T * __builtin_amdgcn_rcpf(om);
@@ -1055,9 +1067,9 @@ __device__ __forceinline__ PixTerms bwd_pixel(PixState& ps, const PixCover& pc, 
   const float gc = fmaf(ps.g.z, c.z, fmaf(ps.g.y, c.y, ps.g.x * c.x));
   const float dalpha = ps.T * (fmaf(ps.gw2, alpha, gc) - ps.acc);
   ps.acc = fmaf(alpha, fmaf(ps.g.w, alpha, gc), om * ps.acc);
-  r.dop = raw > kAlphaMax ? 0.f : dalpha * G;
-  r.dpx = r.dop * pc.dx;
-  r.dpy = r.dop * pc.dy;
+  r.rr = raw > kAlphaMax ? 0.f : dalpha * raw;
+  r.dpx = r.rr * pc.ux;
+  r.dpy = r.rr * pc.uy;
   return r;
 }
 
@@ -1083,12 +1095,14 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
   const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
   const int px = (int)tx * 8 + l % 8, py0 = (int)ty * 8 + l / 8, py1 = py0 + 4;
   const bool in0 = has_tile && px < (int)W && py0 < (int)H, in1 = has_tile && px < (int)W && py1 < (int)H;
+  const int bx0 = ((int)tx * 8) & ~(kRasterBlock - 1), by0 = ((int)ty * 8) & ~(kRasterBlock - 1);
+  const float X0 = (float)bx0, Y0 = (float)by0;
+  const float lx = (float)(px - bx0) + 0.5f, ly0 = (float)(py0 - by0) + 0.5f, ly1 = ly0 + 4.f;
   uint32_t end = 0, items = 0;
   if (has_tile) {
     end = ranges[2 * gtile + 1];
     items = end - ranges[2 * gtile];
   }
-  const float fx = (float)px + 0.5f, fy0 = (float)py0 + 0.5f, fy1 = (float)py1 + 0.5f;
   const bool half_last = (__lane_id() & 7u) == 7u;  // the last lane of each half row of 8
   const uint32_t row = (uint32_t)l >> 3;           // half-row slot 0..3 of the tile
 
@@ -1113,6 +1127,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
   const uint32_t max_half = half_max(max(s0.last, s1.last));
   const uint32_t max_a = __builtin_amdgcn_readlane(max_half, 0), max_b = __builtin_amdgcn_readlane(max_half, kPairHalf);
   const uint32_t max_all = max(max_a, max_b);
+  uint32_t walked_a = 0, walked_b = 0;  // each tile's own entries stepped through (work counter)
 
   // traversal index i (0 = nearest) maps to list position end - 1 - i; walk i downwards
   for (uint32_t hi = max_all; hi > 0;) {
@@ -1123,7 +1138,8 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
       const uint32_t i = hi - 1 - l;
       float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
       uint32_t j = 0;
-      if (i < max_half) {  // an entry some pixel of this tile composited
+      const bool mine = i < max_half;  // an entry some pixel of this tile composited
+      if (mine) {
         j = vals[end - 1 - i];
         const uint32_t k = pslot[j];
         r0 = rec[3 * (size_t)k];
@@ -1131,9 +1147,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
         r2 = rec[3 * (size_t)k + 2];
       }
       s_j[h][l] = j;
-      s_rec[h][0][l] = r0;
-      s_rec[h][1][l] = r1;
-      s_rec[h][2][l] = r2;
+      stage_record(r0, r1, r2, mine, X0, Y0, &s_rec[h][0][l], &s_rec[h][1][l], &s_rec[h][2][l], kInertFinite);
 #pragma unroll
       for (int w = 0; w < 9; ++w)
 #pragma unroll
@@ -1143,18 +1157,20 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
     for (int e = 0; e < cnt; ++e) {
       const uint32_t i = hi - 1 - e;
       const float4 a = s_rec[h][0][e], bb = s_rec[h][1][e], c = s_rec[h][2][e];
-      const PixCover c0 = bwd_cover(s0, a, bb, fx, fy0, i), c1 = bwd_cover(s1, a, bb, fx, fy1, i);
+      const PixCover c0 = bwd_cover(s0, a, bb, lx, ly0, i), c1 = bwd_cover(s1, a, bb, lx, ly1, i);
       if (__ballot(c0.in | c1.in) == 0) continue;  // no pixel of either tile: the slots stay zero
-      const PixTerms t0 = bwd_pixel(s0, c0, bb, c), t1 = bwd_pixel(s1, c1, bb, c);
+      walked_a += i < max_a ? 1u : 0u;
+      walked_b += i < max_b ? 1u : 0u;
+      const PixTerms t0 = bwd_pixel(s0, c0, a, c), t1 = bwd_pixel(s1, c1, a, c);
       // the lane's two pixels summed (FMAs), then over its half row
       float w[9];
       w[0] = fmaf(s1.g.x, t1.ta, s0.g.x * t0.ta);
       w[1] = fmaf(s1.g.y, t1.ta, s0.g.y * t0.ta);
       w[2] = fmaf(s1.g.z, t1.ta, s0.g.z * t0.ta);
-      w[3] = t0.dop + t1.dop;
-      w[4] = fmaf(t1.dpx, c1.dx, t0.dpx * c0.dx);
-      w[5] = fmaf(t1.dpx, c1.dy, t0.dpx * c0.dy);
-      w[6] = fmaf(t1.dpy, c1.dy, t0.dpy * c0.dy);
+      w[3] = t0.rr + t1.rr;
+      w[4] = fmaf(t1.dpx, c1.ux, t0.dpx * c0.ux);
+      w[5] = fmaf(t1.dpx, c1.uy, t0.dpx * c0.uy);
+      w[6] = fmaf(t1.dpy, c1.uy, t0.dpy * c0.uy);
       w[7] = t0.dpx + t1.dpx;
       w[8] = t0.dpy + t1.dpy;
 #pragma unroll
@@ -1169,24 +1185,10 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
     }
     __syncthreads();
     if (l < cnt && hi - 1 - l < max_half) {
-      const float4 a = s_rec[h][0][l], bb = s_rec[h][1][l];
-      const float P = a.z * a.z + bb.x * bb.x, Q = a.z * a.w + bb.x * bb.y, R = a.w * a.w + bb.y * bb.y;
       float u[9];
 #pragma unroll
       for (int k = 0; k < 9; ++k) u[k] = (s_g[h][k][0][l] + s_g[h][k][1][l]) + (s_g[h][k][2][l] + s_g[h][k][3][l]);
-      const float hq = -0.5f * bb.z;
-#pragma unroll
-      for (int k = 4; k < 9; ++k) u[k] *= hq;
-      float* out = gpart + 9 * (size_t)s_j[h][l];
-      out[0] = u[0];
-      out[1] = u[1];
-      out[2] = u[2];
-      out[3] = u[3];
-      out[4] = u[4];
-      out[5] = 2.f * u[5];
-      out[6] = u[6];
-      out[7] = -2.f * (P * u[7] + Q * u[8]);
-      out[8] = -2.f * (Q * u[7] + R * u[8]);
+      pair_words(u, s_rec[h][0][l], s_rec[h][1][l], gpart + 9 * (size_t)s_j[h][l]);
     }
     hi = lo;
   }
@@ -1196,6 +1198,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
 #pragma unroll
     for (int k = 0; k < 9; ++k) out[k] = 0.f;
   }
+  if (L.work && l == 0 && has_tile) L.work[slot] = h ? walked_b : walked_a;
 }
 
 // Sum of x over the 64 lanes of a wave (all lanes active); lane 63 gets it.
@@ -1245,25 +1248,56 @@ __global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* _
       uint64_t r1 = ((uint64_t)lane_u32((uint32_t)(e >> 32), last) << 32) | lane_u32((uint32_t)e, last);
       r1 = r1 < max_pairs ? r1 : max_pairs;  // runs clipped at max_pairs may start beyond it
       float acc[9] = {};
-      for (uint64_t cs = r0; cs < r1; cs += kWave) {
+      // the next needed chunk's words are loaded while the current one is
+      // summed, so a wave has one chunk of loads in flight behind its adds
+      auto needed = [&](uint64_t cs) {
         const uint64_t lo = o > cs ? o : cs, hi = e < cs + kWave ? e : cs + kWave;
-        const bool need = !big && lo < hi;
-        if (__ballot(need) == 0) continue;
+        return __ballot(!big && lo < hi) != 0;
+      };
+      auto load_chunk = [&](uint64_t cs, float* buf) {
         const uint32_t words = 9u * (uint32_t)min((uint64_t)kWave, r1 - cs);
         const float* src = gpart + 9 * cs;
 #pragma unroll
         for (uint32_t q = 0; q < 9; ++q) {
           const uint32_t i = lane + q * kWave;
-          if (i < words) stage[i] = src[i];
+          buf[q] = i < words ? src[i] : 0.f;
         }
+      };
+      uint64_t cs = r0;
+      while (cs < r1 && !needed(cs)) cs += kWave;
+      float cur[9];
+      if (cs < r1) load_chunk(cs, cur);
+      while (cs < r1) {
+        uint64_t nx = cs + kWave;
+        while (nx < r1 && !needed(nx)) nx += kWave;
+        const uint32_t words = 9u * (uint32_t)min((uint64_t)kWave, r1 - cs);
+#pragma unroll
+        for (uint32_t q = 0; q < 9; ++q) {
+          const uint32_t i = lane + q * kWave;
+          if (i < words) stage[i] = cur[q];
+        }
+        if (nx < r1) load_chunk(nx, cur);
         __builtin_amdgcn_wave_barrier();  // a wave's LDS operations complete in order
-        if (need)
-          for (uint64_t p = lo; p < hi; ++p) {
-            const float* v = stage + 9 * (uint32_t)(p - cs);
+        const uint64_t lo = o > cs ? o : cs, hi = e < cs + kWave ? e : cs + kWave;
+        if (!big && lo < hi) {
+          uint32_t p = (uint32_t)(lo - cs);
+          const uint32_t pe = (uint32_t)(hi - cs);
+          for (; p + 2 <= pe; p += 2) {  // two pairs per trip: 18 independent LDS reads in flight
+            const float* v = stage + 9 * p;
+            float x[18];
+#pragma unroll
+            for (int w = 0; w < 18; ++w) x[w] = v[w];
+#pragma unroll
+            for (int w = 0; w < 9; ++w) acc[w] += x[w] + x[9 + w];
+          }
+          if (p < pe) {
+            const float* v = stage + 9 * p;
 #pragma unroll
             for (int w = 0; w < 9; ++w) acc[w] += v[w];
           }
+        }
         __builtin_amdgcn_wave_barrier();
+        cs = nx;
       }
       if (k < n && !big) {
         float* dst = g9 + 9 * (size_t)slot;
@@ -1502,18 +1536,18 @@ __global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(Di
   if (valid) {
     P.qlen = load_quat(sc, id, P.q);
     for (int i = 0; i < 3; ++i) {
-      P.m[i] = sc.means[3 * (size_t)id + i];
-      P.s[i] = expf(sc.log_scales[3 * (size_t)id + i]);
+      P.m[i] = ld_global(sc.means + 3 * (size_t)id + i);
+      P.s[i] = expf(ld_global(sc.log_scales + 3 * (size_t)id + i));
     }
     const float4* sh4 = reinterpret_cast<const float4*>(sc.sh + 48 * (size_t)id);
     for (int j = 0; j < 12; ++j) {
-      const float4 u = sh4[j];
+      const float4 u = ld_global(sh4 + j);
       P.sh[4 * j] = u.x;
       P.sh[4 * j + 1] = u.y;
       P.sh[4 * j + 2] = u.z;
       P.sh[4 * j + 3] = u.w;
     }
-    op = 1.f / (1.f + expf(-sc.opacity[id]));
+    op = 1.f / (1.f + expf(-ld_global(sc.opacity + id)));
   } else {
     for (int i = 0; i < 3; ++i) {
       P.m[i] = 0.f;
@@ -1541,12 +1575,11 @@ __global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(Di
     const uint64_t m = masks[wi];
     if (m == 0) continue;
     if (!((m >> lane) & 1ull)) continue;
-    const uint32_t slot = offsets[wi] + mbcnt(m);
-    const float* gk = g9 + 9 * (size_t)slot;
+    const float* gk = g9 + 9 * (size_t)(offsets[wi] + mbcnt(m));
     float gv[9];
     bool nz = false;
     for (int j = 0; j < 9; ++j) {
-      gv[j] = gk[j];
+      gv[j] = ld_global(gk + j);
       nz = nz || gv[j] != 0.f;
     }
     if (!nz) continue;
@@ -1585,13 +1618,14 @@ __global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(Di
     gq[3] = (gqz - qz * dot) / P.qlen;
   }
   for (int i = 0; i < 3; ++i) {
-    sc.g_means[3 * (size_t)id + i] = gm[i];
-    sc.g_log_scales[3 * (size_t)id + i] = gls[i];
+    st_global(sc.g_means + 3 * (size_t)id + i, gm[i]);
+    st_global(sc.g_log_scales + 3 * (size_t)id + i, gls[i]);
   }
-  for (int i = 0; i < 4; ++i) sc.g_quats[4 * (size_t)id + i] = gq[i];
-  sc.g_opacity[id] = gopl;
+  st_global(reinterpret_cast<float4*>(sc.g_quats) + id, make_float4(gq[0], gq[1], gq[2], gq[3]));
+  st_global(sc.g_opacity + id, gopl);
   float4* gsh4 = reinterpret_cast<float4*>(sc.g_sh + 48 * (size_t)id);
-  for (int j = 0; j < 12; ++j) gsh4[j] = make_float4(gsh[4 * j], gsh[4 * j + 1], gsh[4 * j + 2], gsh[4 * j + 3]);
+  for (int j = 0; j < 12; ++j)
+    st_global(gsh4 + j, make_float4(gsh[4 * j], gsh[4 * j + 1], gsh[4 * j + 2], gsh[4 * j + 3]));
 }
 
 uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
@@ -1732,7 +1766,7 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
                                        uint64_t pairs, const uint32_t* soffs, const uint32_t* scnt,
                                        const uint32_t* d_count, uint32_t max_n, float* gpart, float* g9,
-                                       hipStream_t s) {
+                                       hipStream_t s, hipEvent_t mid) {
   // partials of pairs no pixel reached stay zero
   if (L.n_tiles > 0) {
     bool launched = false;
@@ -1756,6 +1790,10 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
     VKGS_DIFF_CASES(VKGS_CASE)
 #undef VKGS_CASE
     if (!launched) return hipErrorInvalidValue;
+  }
+  if (mid) {
+    const hipError_t e = hipEventRecord(mid, s);
+    if (e != hipSuccess) return e;
   }
   if (max_n > 0)
     diff_grad_reduce_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(soffs, scnt, gpart, d_count, max_n, pairs,

@@ -18,6 +18,22 @@ __device__ __forceinline__ float gauss_falloff(float q) {
   return __builtin_amdgcn_exp2f(q * kNegHalfLog2e);
 }
 
+// Load and store through a pointer known to address device (global) memory:
+// for a pointer read from memory (a field of a table in HBM) the compiler
+// cannot always infer the address space and falls back to flat accesses,
+// which also count against the LDS wait counter.
+__device__ __forceinline__ float ld_global(const float* p) { return *(const __attribute__((address_space(1))) float*)p; }
+__device__ __forceinline__ float4 ld_global(const float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = *(const __attribute__((address_space(1))) v4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_global(float* p, float v) { *(__attribute__((address_space(1))) float*)p = v; }
+__device__ __forceinline__ void st_global(float4* p, const float4& v) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  *(__attribute__((address_space(1))) v4*)p = v4{v.x, v.y, v.z, v.w};
+}
+
 // Number of set bits of m below this lane.
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -83,6 +83,11 @@ struct vkgs_diff_ctx {
   bool have_forward = false;
   bool grow_pairs = false;
   uint64_t need_pairs = 0;  // pair count of the last overflowing forward
+  // per-stage events and raster work counters (vkgs_diff_set_timing)
+  bool timing = false;
+  hipEvent_t ev[7] = {};
+  Buf work_f, work_b;
+  size_t work_words = 0;
 };
 
 namespace {
@@ -166,6 +171,10 @@ int vkgs_diff_destroy(vkgs_diff_ctx* c) {
                  &c->val64,      &c->vslot,    &c->perm_keys,    &c->perm_vals,  &c->perm_bbox_part,
                  &c->perm_bbox,  &c->perm_sort};
   for (Buf* b : bufs) b->release();
+  c->work_f.release();
+  c->work_b.release();
+  for (hipEvent_t e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   delete c;
   return VKGS_OK;
 }
@@ -186,7 +195,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   for (uint32_t i = 0; i < n_scenes; ++i) {
     const vkgs_diff_scene& a = scenes[i];
     if (a.n > 0 && (!a.means || !a.log_scales || !a.quats || !a.opacity_logits || !a.sh)) return VKGS_ERR_INVALID_ARG;
-    if (a.n > 0 && !aligned16(a.sh)) return VKGS_ERR_INVALID_ARG;
+    if (a.n > 0 && (!aligned16(a.sh) || !aligned16(a.quats))) return VKGS_ERR_INVALID_ARG;
     if (a.n > (1u << 28)) return VKGS_ERR_CAPACITY;
     DiffScene& d = c->h_scenes[i];
     d.means = a.means;
@@ -380,6 +389,16 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   // ---- forward ----
   uint32_t* misc = c->misc.as<uint32_t>();
   uint32_t* order = c->val.as<uint32_t>();
+  L.work = nullptr;
+  if (c->timing) {
+    const uint32_t wg_waves = rs0.tile_w * rs0.tile_h / kWave;
+    c->work_words = (size_t)L.n_tiles * (wg_waves ? wg_waves : 1u);
+    VKGS_HIP_CHECK(c->work_f.ensure(c->work_words * 4 + 4));
+    VKGS_HIP_CHECK(c->work_b.ensure(c->work_words * 4 + 4));
+    VKGS_HIP_CHECK(fill_u32_launch(c->work_f.as<uint32_t>(), 0u, c->work_words, s));
+    VKGS_HIP_CHECK(fill_u32_launch(c->work_b.as<uint32_t>(), 0u, c->work_words, s));
+    VKGS_HIP_CHECK(hipEventRecord(c->ev[0], s));
+  }
   VKGS_HIP_CHECK(fill_u32_launch(misc, 0u, 4, s));
   VKGS_HIP_CHECK(launch_diff_count(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(), misc + kMiscVisible,
                                    c->count_scan.p, s));
@@ -438,8 +457,14 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
                                c->ranges.as<uint32_t>(), s));
   VKGS_HIP_CHECK(launch_tile_order(c->ranges.as<uint32_t>(), (uint32_t)tiles, c->tile_keys.as<uint32_t>(),
                                    c->tile_order.as<uint32_t>(), c->tile_plan, c->tile_sort.p, s));
+  if (c->timing) {
+    VKGS_HIP_CHECK(hipEventRecord(c->ev[1], s));
+    L.work = c->work_f.as<uint32_t>();
+  }
   VKGS_HIP_CHECK(launch_diff_raster(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(), c->ranges.as<uint32_t>(),
                                     c->pslot.as<uint32_t>(), c->tile_order.as<uint32_t>(), true, s));
+  L.work = nullptr;
+  if (c->timing) VKGS_HIP_CHECK(hipEventRecord(c->ev[2], s));
   c->have_forward = true;
   return VKGS_OK;
 }
@@ -459,15 +484,18 @@ int vkgs_diff_backward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t
     if (a.n != d.n) return VKGS_ERR_INVALID_ARG;
     if (d.n > 0 && (!a.grad_means || !a.grad_log_scales || !a.grad_quats || !a.grad_opacity_logits || !a.grad_sh))
       return VKGS_ERR_INVALID_ARG;
-    if (d.n > 0 && !aligned16(a.grad_sh)) return VKGS_ERR_INVALID_ARG;
+    if (d.n > 0 && (!aligned16(a.grad_sh) || !aligned16(a.grad_quats))) return VKGS_ERR_INVALID_ARG;
     d.g_means = a.grad_means;
     d.g_log_scales = a.grad_log_scales;
     d.g_quats = a.grad_quats;
     d.g_opacity = a.grad_opacity_logits;
     d.g_sh = a.grad_sh;
   }
+  const bool timed = c->timing && c->work_words > 0;
+  if (timed) VKGS_HIP_CHECK(hipEventRecord(c->ev[3], s));
   VKGS_HIP_CHECK(upload_tables(c, s));
-  const DiffLaunch& L = c->launch;
+  DiffLaunch L = c->launch;
+  L.work = timed ? c->work_b.as<uint32_t>() : nullptr;
   uint32_t* misc = c->misc.as<uint32_t>();
   // per-pair partials: gpart holds max_pairs, the bound of the live count the
   // kernels read on the device (no host read-back, so a graph can capture this)
@@ -475,9 +503,46 @@ int vkgs_diff_backward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t
                                              c->ranges.as<uint32_t>(), c->pslot.as<uint32_t>(),
                                              c->tile_order.as<uint32_t>(), c->max_pairs, c->soffs.as<uint32_t>(),
                                              c->scnt.as<uint32_t>(), misc + kMiscVisible, c->cap_items,
-                                             c->gpart.as<float>(), c->g9.as<float>(), s));
+                                             c->gpart.as<float>(), c->g9.as<float>(), s, timed ? c->ev[4] : nullptr));
+  if (timed) VKGS_HIP_CHECK(hipEventRecord(c->ev[5], s));
+  L.work = nullptr;
   VKGS_HIP_CHECK(launch_diff_project_backward(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(),
                                               c->g9.as<float>(), s));
+  if (timed) VKGS_HIP_CHECK(hipEventRecord(c->ev[6], s));
+  return VKGS_OK;
+}
+
+int vkgs_diff_set_timing(vkgs_diff_ctx* c, int enable) {
+  if (!c) return VKGS_ERR_INVALID_ARG;
+  if (enable && !c->ev[0])
+    for (hipEvent_t& e : c->ev) VKGS_HIP_CHECK(hipEventCreate(&e));
+  c->timing = enable != 0;
+  c->work_words = 0;  // valid again after the next timed forward
+  return VKGS_OK;
+}
+
+int vkgs_diff_stage_ms(vkgs_diff_ctx* c, float ms[8]) {
+  if (!c || !ms || !c->timing || c->work_words == 0) return VKGS_ERR_INVALID_ARG;
+  VKGS_HIP_CHECK(hipEventSynchronize(c->ev[6]));
+  const int pairs[7][2] = {{0, 1}, {1, 2}, {0, 2}, {3, 4}, {4, 5}, {5, 6}, {3, 6}};
+  for (int i = 0; i < 7; ++i) VKGS_HIP_CHECK(hipEventElapsedTime(&ms[i], c->ev[pairs[i][0]], c->ev[pairs[i][1]]));
+  ms[7] = 0.f;
+  return VKGS_OK;
+}
+
+int vkgs_diff_raster_work(vkgs_diff_ctx* c, uint64_t* forward_entries, uint64_t* backward_entries) {
+  if (!c || !c->timing || c->work_words == 0) return VKGS_ERR_INVALID_ARG;
+  std::vector<uint32_t> w(c->work_words);
+  VKGS_HIP_CHECK(hipDeviceSynchronize());
+  const Buf* bufs[2] = {&c->work_f, &c->work_b};
+  uint64_t* outs[2] = {forward_entries, backward_entries};
+  for (int k = 0; k < 2; ++k) {
+    if (!outs[k]) continue;
+    VKGS_HIP_CHECK(hipMemcpy(w.data(), bufs[k]->p, c->work_words * 4, hipMemcpyDeviceToHost));
+    uint64_t sum = 0;
+    for (uint32_t x : w) sum += x;
+    *outs[k] = sum;
+  }
   return VKGS_OK;
 }
 

@@ -71,6 +71,11 @@ struct DiffLaunch {
   // launch choices (host side): 8x8 tiles two per wave (RasterShape::pairs),
   // backward LDS batch depth on 8x8 tiles, resident-wave cap of the rasters
   uint32_t pairs, lds_batch, waves_per_cu;
+  // optional work counter of a raster launch: work[i * waves + w] = list
+  // entries wave w of workgroup i stepped through (two-tile kernels: one
+  // word per tile, its own entries), so 64 x the sum counts the (pixel,
+  // entry) evaluations executed
+  uint32_t* work;
 };
 
 // Spatial processing order of every scene's splats: keys[g] = scene index in
@@ -136,7 +141,8 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
                                        uint64_t pairs, const uint32_t* soffs, const uint32_t* scnt,
                                        const uint32_t* d_count, uint32_t max_n, float* gpart, float* g9,
-                                       hipStream_t s);
+                                       hipStream_t s, hipEvent_t mid = nullptr);
+// (mid, optional: recorded between the raster backward and the reduction.)
 // Writes (overwrites) every parameter gradient of every scene.
 hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
                                         const float* g9, hipStream_t s);

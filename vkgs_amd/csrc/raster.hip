@@ -31,13 +31,12 @@
 #include "common.h"
 #include "device.h"
 #include "kernels.h"
+#include "stage.h"
 
 #include <mutex>
 
 namespace vkgs {
 namespace {
-
-__device__ __forceinline__ uint32_t pack16(int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
 __global__ __launch_bounds__(256) void bin_count_kernel(const float4* __restrict__ inst,
                                                         const uint32_t* __restrict__ order,
@@ -204,27 +203,6 @@ __global__ void tile_order_keys_kernel(const uint32_t* __restrict__ ranges, uint
   const uint32_t m = (e >= 3u ? v >> (e - 3u) : v << (3u - e)) & 7u;
   keys[t] = 255u - (8u * e + m);
   vals[t] = t;
-}
-
-// A raster record restaged for one tile (see kFrameScale): the scaled inverse
-// rot_scale rows with their offsets at the corner (X0, Y0) of the tile's
-// 64 x 64 pixel block, -log2(opacity), the colour and the pixel rectangle.  A
-// lane with no record stages one that covers no pixel (offsets +inf).
-__device__ __forceinline__ void stage_record(const float4& p0, const float4& p1, const float4& p2, bool valid,
-                                             float X0, float Y0, float4* s0, float4* s1, float4* s2) {
-  if (valid) {
-    const float i00 = p0.z * kFrameScale, i01 = p0.w * kFrameScale;
-    const float i10 = p1.x * kFrameScale, i11 = p1.y * kFrameScale;
-    const float dx = X0 - p0.x, dy = Y0 - p0.y;
-    *s0 = make_float4(i00, i01, fmaf(i00, dx, i01 * dy), -__builtin_amdgcn_logf(p1.z));
-    *s1 = make_float4(i10, i11, fmaf(i10, dx, i11 * dy), p1.w);
-    *s2 = p2;
-  } else {
-    const float inf = __builtin_inff(), empty = __uint_as_float(pack16(1, 0));
-    *s0 = make_float4(0.f, 0.f, inf, inf);
-    *s1 = make_float4(0.f, 0.f, inf, empty);
-    *s2 = make_float4(0.f, 0.f, 0.f, empty);
-  }
 }
 
 // One (pixel, record) step of front-to-back compositing; (lx, ly) is the pixel

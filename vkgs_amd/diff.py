@@ -40,6 +40,9 @@ _SIGS = {
     "vkgs_diff_backward": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P]),
     "vkgs_diff_info": (C.c_int, [_P, _P, _P, _P]),
     "vkgs_diff_check": (C.c_int, [_P, _P, _P]),
+    "vkgs_diff_set_timing": (C.c_int, [_P, C.c_int]),
+    "vkgs_diff_stage_ms": (C.c_int, [_P, _P]),
+    "vkgs_diff_raster_work": (C.c_int, [_P, _P, _P]),
     "vkgs_diff_autotune": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P, _P, _P]),
 }
 
@@ -358,6 +361,36 @@ class GraphedStep:
         """Synchronous: raises VkgsError (VKGS_ERR_OVERFLOW) if any replay since the last check
         overflowed its pair list."""
         check(_L().vkgs_diff_check(self.ctx.h, None, _stream()), "vkgs_diff_check")
+
+
+DIFF_STAGES = ("forward_prep", "forward_raster", "forward_total", "backward_raster", "backward_reduce",
+               "backward_project", "backward_total")
+
+
+def profile_step(scenes, views, grad_fn, tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False, config=None):
+    """One eager forward + backward of render_batch_flat's form with per-stage HIP events and the
+    raster kernels' work counters (vkgs_diff_set_timing): returns {"ms": {stage: ms}, "forward_entries",
+    "backward_entries"} (list entries stepped through, summed over waves; 64 x each = executed (pixel,
+    entry) evaluations).  grad_fn(flat) -> dL/d(flat).  Synchronous; the parameters' .grad receive
+    the step's gradients (accumulated as autograd does)."""
+    vs, params = _batch_args(scenes, views)
+    cfg = _config(tile, t_min, spatial_order, config)
+    c = _Ctx()
+    check(_L().vkgs_diff_set_timing(c.h, 1), "vkgs_diff_set_timing")
+    for attempt in range(2):
+        flat = _RenderBatch.apply((vs, len(scenes), cfg, True, c), *params)
+        flat.backward(grad_fn(flat.detach()))
+        st = _L().vkgs_diff_check(c.h, None, _stream())
+        if st == 0:
+            break
+        if st != 6 or attempt == 1:  # the retry is sized for the whole list
+            check(st, "vkgs_diff_check")
+    ms = (C.c_float * 8)()
+    check(_L().vkgs_diff_stage_ms(c.h, ms), "vkgs_diff_stage_ms")
+    fe, be = C.c_uint64(), C.c_uint64()
+    check(_L().vkgs_diff_raster_work(c.h, C.byref(fe), C.byref(be)), "vkgs_diff_raster_work")
+    return {"ms": dict(zip(DIFF_STAGES, list(ms))), "forward_entries": int(fe.value),
+            "backward_entries": int(be.value)}
 
 
 def view_images(flat, spans):
