@@ -27,6 +27,10 @@ if HERE not in sys.path:
 METRIC = "FPS @6M splats 1600x900; batched vs plain fwd+bwd views/s; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# Without packed FP32 instructions (this build disables them) a wave64 FP32 instruction issues once per
+# 4 cycles per SIMD: 256 CUs x 4 SIMDs x 16 lanes x 2 (FMA) x 2.4 GHz
+FP32_ISSUE_TFLOPS = 78.6
+PAIRS_ALWAYS, PAIRS_NEVER = 4, 8  # vkgs_raster_config.flags
 # FP32 operations per (pixel, splat) evaluation of the reference fragment and blend, exp counted as
 # 1: eigenframe coordinates, falloff, alpha, and the blend of colour and alpha
 RASTER_FLOPS_PER_EVAL = 20
@@ -269,10 +273,12 @@ def diff_roofline(profs):
     f_tf = RASTER_FLOPS_PER_EVAL * f_eval / (ms["forward_raster"] * 1e-3) / 1e12
     return {"kernel": "diff_raster_backward_pair_kernel / diff_raster_backward_kernel", "bound": "valu",
             "achieved": round(b_tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(b_tf / FP32_PEAK_TFLOPS, 4), "traffic": None, "avg_ms": round(ms["backward_raster"], 4),
+            "frac": round(b_tf / FP32_PEAK_TFLOPS, 4), "frac_of_issue_ceiling": round(b_tf / FP32_ISSUE_TFLOPS, 4),
+            "traffic": None, "avg_ms": round(ms["backward_raster"], 4),
             "work": f"{BACKWARD_OPS_PER_EVAL} FP32 ops (FMA, exp, rcp = 1) x {b_eval:.4g} executed (pixel, entry) "
                     f"evaluations over {len(profs)} call(s)",
             "forward_raster": {"achieved": round(f_tf, 3), "frac": round(f_tf / FP32_PEAK_TFLOPS, 4),
+                               "frac_of_issue_ceiling": round(f_tf / FP32_ISSUE_TFLOPS, 4),
                                "avg_ms": round(ms["forward_raster"], 4),
                                "work": f"{RASTER_FLOPS_PER_EVAL} FP32 ops x {f_eval:.4g} executed evaluations"},
             "stage_ms": {k: round(v, 4) for k, v in ms.items()}}
@@ -427,11 +433,19 @@ def main(argv=None) -> int:
     own_bytes = b_proj + 96.0 * V + 32.0 * K + 20.0 * K + 8.0 * T
     own_gbs = own_bytes / t_ps / 1e9
 
-    raster_name = "rasterize_kernel" if r_variant == 0 else "rasterize_wave_kernel"
+    # the raster kernel this configuration launches (8x8 tiles go two per wave from 8192 tiles up
+    # unless a pair flag says otherwise)
+    cfg = r.tuned if r.tuned is not None else None
+    flags = cfg.flags if cfg is not None else 0
+    pairs_form = (tw, th) == (8, 8) and r_variant == 0 and not (flags & PAIRS_NEVER) and (
+        (flags & PAIRS_ALWAYS) or T >= 8192)
+    raster_name = ("rasterize_pair_kernel" if pairs_form else
+                   "rasterize_kernel" if r_variant == 0 else "rasterize_wave_kernel")
     if t_raster >= t_proj:
         # vector fp32 work (no stage is a dense contraction, MFMA is unused)
         roof = {"kernel": raster_name, "bound": "valu", "achieved": round(raster_tflops, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(raster_tflops / FP32_PEAK_TFLOPS, 4),
+                "frac_of_issue_ceiling": round(raster_tflops / FP32_ISSUE_TFLOPS, 4),
                 "traffic": load_traffic(raster_name, wl.name, f"{tw}x{th}"), "avg_ms": stage_ms["raster"],
                 "work": f"{RASTER_FLOPS_PER_EVAL} FP32 ops (exp = 1) x {p_exec:.4g} executed (pixel, splat) "
                         f"evaluations (64 x {wave_entries} wave steps; whole lists: {p_eval:.4g})",
@@ -461,7 +475,9 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"plain forward {wl.name}: {wl.n} splats, {wl.width}x{wl.height}, orbit r={wl.radius} m",
                    "splats": n, "width": wl.width, "height": wl.height, "visible": V, "pairs": K,
-                   "tile": f"{tw}x{th}", "t_min": args.t_min, "graph": True},
+                   "tile": f"{tw}x{th}", "raster_kernel": raster_name,
+                   "waves_per_cu": (cfg.waves_per_cu if cfg is not None else 0) or "max",
+                   "t_min": args.t_min, "graph": True},
         "frame_ms": frame_ms,
         "stages_ms": stage_ms,
         "roofline": roof,
