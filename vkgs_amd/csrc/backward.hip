@@ -705,17 +705,19 @@ constexpr float kInvScale2 = 1.3862943611198906f;
 //   dL/dC = sum dL/dq d d^T = M'^-1 S M'^-T,  S = -1/2 [[u4, u5], [u5, u6]],
 //   dL/dcentre = sum dL/dq (-2 C d) = -2 / kFrameScale^2 M'^T s,  s = -1/2 (u7, u8),
 // written as (colour 3, opacity, C00, 2 C01, C11, centre x, centre y).  The
-// opacity is recovered from the staged -log2(opacity) in a.w.
+// opacity's reciprocal is recovered from the staged -log2(opacity) in a.w.
 __device__ __forceinline__ void pair_words(const float u[9], const float4& a, const float4& b,
                                            float* __restrict__ out) {
-  const float opacity = __builtin_amdgcn_exp2f(-a.w);
+  // 1 / opacity = exp2(log2(1 / opacity)), the staged a.w; reciprocals are the
+  // hardware's (1 ulp), well inside the gradients' tolerance
+  const float inv_opacity = __builtin_amdgcn_exp2f(a.w);
   out[0] = u[0];
   out[1] = u[1];
   out[2] = u[2];
-  out[3] = opacity > 0.f ? u[3] / opacity : 0.f;
+  out[3] = inv_opacity < __builtin_inff() ? u[3] * inv_opacity : 0.f;
   // M'^-1 = adj(M') / det; det = ab(c^2 + s^2)-like products of one sign, no cancellation
   const float det = a.x * b.y - a.y * b.x;
-  const float id = 1.f / det;
+  const float id = __builtin_amdgcn_rcpf(det);
   const float A00 = b.y * id, A01 = -a.y * id, A10 = -b.x * id, A11 = a.x * id;
   const float p = -0.5f * u[4], r = -0.5f * u[5], q = -0.5f * u[6];
   const float t00 = A00 * p + A01 * r, t01 = A00 * r + A01 * q;  // row 0 of A S
@@ -793,7 +795,6 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
   // behind starts at 0 and the alpha behind at 1, and both follow the blend
   // r <- alpha * x + (1 - alpha) * r, so their g-weighted sum does too
   float acc = g.w;
-  const float gw2 = 2.f * g.w;
   uint32_t walked = 0;  // entries this wave stepped through (work counter)
 
   // traversal index i (0 = nearest) maps to list position end - 1 - i; walk i downwards
@@ -837,10 +838,12 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
       const float om = 1.f - alpha;
       T = T * __builtin_amdgcn_rcpf(om);  // transmittance in front of entry i
       const float ta = T * alpha;
-      // dL/dalpha = T * (g . (colour, 2 alpha) - g . (behind))
+      // dL/dalpha = T * (g . (colour, 2 alpha) - g . (behind)) and the blend of
+      // g . (colour, alpha) into acc, sharing y = g.w alpha + g . colour - acc
       const float gc = fmaf(g.z, c.z, fmaf(g.y, c.y, g.x * c.x));
-      const float dalpha = T * (fmaf(gw2, alpha, gc) - acc);
-      acc = fmaf(alpha, fmaf(g.w, alpha, gc), om * acc);
+      const float y = fmaf(g.w, alpha, gc - acc);
+      const float dalpha = T * fmaf(g.w, alpha, y);
+      acc = fmaf(alpha, y, acc);
       const float rr = raw > kAlphaMax ? 0.f : dalpha * raw;
       const float dpx = rr * ux, dpy = rr * uy;
       float s0 = half_row_sum(g.x * ta);
@@ -1031,7 +1034,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
 // diff_raster_backward_kernel: a pixel outside the square or past its last
 // entry gets raw alpha 0, so its terms are zero and T and acc stay.
 struct PixState {
-  float T, acc, gw2;
+  float T, acc;
   float4 g;
   uint32_t last;
 };
@@ -1065,8 +1068,9 @@ __device__ __forceinline__ PixTerms bwd_pixel(PixState& ps, const PixCover& pc, 
   PixTerms r;
   r.ta = ps.T * alpha;
   const float gc = fmaf(ps.g.z, c.z, fmaf(ps.g.y, c.y, ps.g.x * c.x));
-  const float dalpha = ps.T * (fmaf(ps.gw2, alpha, gc) - ps.acc);
-  ps.acc = fmaf(alpha, fmaf(ps.g.w, alpha, gc), om * ps.acc);
+  const float y = fmaf(ps.g.w, alpha, gc - ps.acc);
+  const float dalpha = ps.T * fmaf(ps.g.w, alpha, y);
+  ps.acc = fmaf(alpha, y, ps.acc);
   r.rr = raw > kAlphaMax ? 0.f : dalpha * raw;
   r.dpx = r.rr * pc.ux;
   r.dpy = r.rr * pc.uy;
@@ -1106,7 +1110,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
   const bool half_last = (__lane_id() & 7u) == 7u;  // the last lane of each half row of 8
   const uint32_t row = (uint32_t)l >> 3;           // half-row slot 0..3 of the tile
 
-  PixState s0 = {1.f, 0.f, 0.f, make_float4(0.f, 0.f, 0.f, 0.f), 0u}, s1 = s0;
+  PixState s0 = {1.f, 0.f, make_float4(0.f, 0.f, 0.f, 0.f), 0u}, s1 = s0;
   if (in0) {
     const size_t pix = (size_t)py0 * W + px;
     s0.last = view.last[pix];
@@ -1121,9 +1125,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
   }
   // g . (colour, alpha) composited behind the current entry (as in the one-tile kernel)
   s0.acc = s0.g.w;
-  s0.gw2 = 2.f * s0.g.w;
   s1.acc = s1.g.w;
-  s1.gw2 = 2.f * s1.g.w;
   const uint32_t max_half = half_max(max(s0.last, s1.last));
   const uint32_t max_a = __builtin_amdgcn_readlane(max_half, 0), max_b = __builtin_amdgcn_readlane(max_half, kPairHalf);
   const uint32_t max_all = max(max_a, max_b);
@@ -1243,7 +1245,10 @@ __global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* _
     // run in the chunk.  Chunks that only long runs need are skipped.
     {
       const uint32_t last = __builtin_amdgcn_readfirstlane(min(n - base, (uint32_t)kWave) - 1);
-      const uint64_t r0 = ((uint64_t)lane_u32((uint32_t)(o >> 32), 0) << 32) | lane_u32((uint32_t)o, 0);
+      // chunks start at an even pair index, so the pairs summed two at a time
+      // below are (even q, q + 1) whatever the wave's first slot: the sum's
+      // order depends on the slot's run alone
+      const uint64_t r0 = (((uint64_t)lane_u32((uint32_t)(o >> 32), 0) << 32) | lane_u32((uint32_t)o, 0)) & ~1ull;
       const uint64_t e = o + c;
       uint64_t r1 = ((uint64_t)lane_u32((uint32_t)(e >> 32), last) << 32) | lane_u32((uint32_t)e, last);
       r1 = r1 < max_pairs ? r1 : max_pairs;  // runs clipped at max_pairs may start beyond it
@@ -1282,6 +1287,12 @@ __global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* _
         if (!big && lo < hi) {
           uint32_t p = (uint32_t)(lo - cs);
           const uint32_t pe = (uint32_t)(hi - cs);
+          if (p & 1u) {  // an odd first pair alone (cs is even)
+            const float* v = stage + 9 * p;
+#pragma unroll
+            for (int w = 0; w < 9; ++w) acc[w] += v[w];
+            ++p;
+          }
           for (; p + 2 <= pe; p += 2) {  // two pairs per trip: 18 independent LDS reads in flight
             const float* v = stage + 9 * p;
             float x[18];
