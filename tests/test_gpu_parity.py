@@ -348,6 +348,23 @@ def test_renderer_full_size_matches_oracle(name, width, height):
     sc = _workload_scene(name)
     cam = S.orbit_camera(width, height, wl.radius)
     print("full size", name, width, height, flush=True)
+    _check_frame_against_oracle(sc, cam, width, height)
+
+
+def test_renderer_beyond_the_reference_capacity():
+    """2^23 + 4099 splats, more than the reference's MAX_SPLAT_COUNT = 2^23 (engine.cc:1653, past which
+    it overflows its buffers silently): the renderer holds the oracle's bar there too."""
+    n = (1 << 23) + 4099
+    _SCENES.clear()
+    sc = S.gaussian_cloud(n, seed=23, pos_std=2.0, logscale=(-4.5, 0.4))
+    cam = S.orbit_camera(800, 600, 9.0)
+    print("beyond capacity", n, flush=True)
+    _check_frame_against_oracle(sc, cam, 800, 600)
+
+
+def _check_frame_against_oracle(sc, cam, width, height):
+    """Visible count, sorted keys and ids, every tile's list bit-exact against the oracle; the image
+    within 1/255 and PSNR >= 50 dB of the oracle's back-to-front composite."""
     r = P.Renderer(sc.n)
     r.load_buffer(sc.ply_buffer(), sc.n)
     for _ in range(3):  # the first frame may grow the pair storage
