@@ -200,6 +200,17 @@ typedef struct vkgs_raster_config {
  * the same and refreshed every 32 forwards. */
 #define VKGS_DIFF_SPATIAL_ORDER 1u
 
+/* vkgs_raster_config.flags bit (differentiable path): when a batch's views
+ * hold more tiles together than one view does by a radix pass (for instance
+ * five 1600x900 views on 8x8 tiles: 113,000 tiles, 17 bits, against 22,600,
+ * 15 bits), the forward sorts view-major keys: the depth key carries the view
+ * index above a 25-bit integer depth (views: at most 128), so pairs are
+ * emitted view by view in depth order, and the tile sort orders the tile
+ * within the view alone, one pass fewer over all pairs.  The tile lists, and
+ * so images and gradients, are the same bit for bit as with global tile keys;
+ * this bit keeps the global keys (for checking and timing). */
+#define VKGS_DIFF_GLOBAL_TILE_KEYS 16u
+
 /* vkgs_raster_config.flags bit: rasterize each splat as the reference's
  * fixed-function pipeline does (two triangles per quad, index pattern
  * 4i + {0,1,2,2,1,3} (engine.cc:607-616), vertices snapped to 1/256 pixel,

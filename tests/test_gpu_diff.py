@@ -340,6 +340,46 @@ def test_full_batch_workload_equals_single_views():
             assert rel < 1e-5, (si, k, rel)
 
 
+def test_view_major_tile_keys_match_global_keys():
+    """Five 1600x900 views on 8x8 tiles hold 113,000 tiles (17-bit global tile ids, three radix
+    passes) but at most 22,600 per view (15 bits, two passes), so the batch sorts view-major keys:
+    the view above a 25-bit integer depth, then the tile within the view alone.  The tile lists are
+    the ones global tile keys give (VKGS_DIFF_GLOBAL_TILE_KEYS), so images and gradients are the same
+    bit for bit, in the default and in the spatial processing order; and the images equal one-view
+    renders bit for bit."""
+    sa = S.gaussian_cloud(40000, seed=61, pos_std=1.5, logscale=(-3.0, 0.5))
+    sb = S.gaussian_cloud(9000, seed=62, pos_std=1.0, logscale=(-2.5, 0.4))
+    cams = [S.orbit_camera(1600, 900, 4.0 + 0.4 * i, theta=0.7 * i) for i in range(4)]
+    views = [(0, c) for c in cams] + [(1, S.orbit_camera(1600, 900, 3.5, phi=1.0))]
+    rng = np.random.default_rng(8)
+    weights = [torch.tensor(rng.uniform(0.5, 1.5, (900, 1600, 4)), dtype=torch.float32, device="cuda")
+               for _ in views]
+    def run(spatial, flags):
+        cfg = D._config(t_min=1.0 / 8192.0, spatial_order=spatial)
+        cfg.flags |= flags
+        params = [D.scene_tensors(sa), D.scene_tensors(sb)]
+        flat, spans = D.render_batch_flat(params, views, config=cfg)
+        imgs = D.view_images(flat, spans)
+        sum((img * w).sum() for img, w in zip(imgs, weights)).backward()
+        return ([img.detach().cpu().numpy() for img in imgs],
+                [{k: p[k].grad.detach().cpu().numpy() for k in NAMES} for p in params])
+
+    for spatial in (False, True):
+        local_i, local_g = run(spatial, 0)
+        glob_i, glob_g = run(spatial, D.GLOBAL_TILE_KEYS)
+        print("view-major vs global keys, spatial", spatial, "pairs", D.last_info["pairs"], flush=True)
+        for a, b in zip(local_i, glob_i):
+            np.testing.assert_array_equal(a, b)
+        for ga, gb in zip(local_g, glob_g):
+            for k in NAMES:
+                np.testing.assert_array_equal(ga[k], gb[k])
+        if not spatial:
+            params = [D.scene_tensors(sa), D.scene_tensors(sb)]
+            for (si, cam), img in zip(views, local_i):
+                one = D.render(**params[si], camera=cam).detach().cpu().numpy()
+                np.testing.assert_array_equal(img, one)
+
+
 def cam_t64(cam):
     Pm, Vm, eye = S.camera_arrays(cam)
     return (torch.tensor(Pm.reshape(4, 4).T, dtype=torch.float64),

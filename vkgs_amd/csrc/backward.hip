@@ -401,7 +401,7 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
       val[slot] = slot;
     }
     scnt[slot] = r.z;
-    key[slot] = __float_as_uint(omz);
+    key[slot] = L.local_tile_bits ? (v << kDepthRankBits) | depth_rank_key(omz) : __float_as_uint(omz);
   }
 }
 
@@ -498,8 +498,9 @@ __global__ __launch_bounds__(256) void diff_bin_emit_kernel(DiffLaunch L, const 
         r = rect[k];
         slot = order[k];
         q0 = soffs[slot];
-        const DiffView& view = L.views[kview[k]];
-        key_base = view.tile_base;
+        const uint32_t v = kview[k];
+        const DiffView& view = L.views[v];
+        key_base = L.local_tile_bits ? v << L.local_tile_bits : view.tile_base;
         tiles_x = view.fc.tiles_x;
         if (o + c > max_pairs) atomicOr(status, 1u);
       }
@@ -536,8 +537,9 @@ __global__ __launch_bounds__(256) void diff_bin_emit_packed_kernel(DiffLaunch L,
         slot = (uint32_t)w;
         o = offs[k];
         q0 = soffs[slot];
-        const DiffView& view = L.views[range_of(vslot, L.n_views, slot)];
-        key_base = view.tile_base;
+        const uint32_t v = range_of(vslot, L.n_views, slot);
+        const DiffView& view = L.views[v];
+        key_base = L.local_tile_bits ? v << L.local_tile_bits : view.tile_base;
         tiles_x = view.fc.tiles_x;
         if (o + c > max_pairs) atomicOr(status, 1u);
       }

@@ -221,7 +221,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   c->h_view_blocks.assign(n_views + 1, 0u);
   c->h_view_tiles.assign(n_views + 1, 0u);
   uint64_t blocks = 0, tiles = 0, pixels = 0;
-  uint32_t max_tiles_x = 1, max_tiles_y = 1;
+  uint32_t max_tiles_x = 1, max_tiles_y = 1, max_view_tiles = 1;
   for (uint32_t v = 0; v < n_views; ++v) {
     const vkgs_diff_view& a = views[v];
     if (a.scene >= n_scenes || !a.rgba) return VKGS_ERR_INVALID_ARG;
@@ -245,6 +245,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
     pixels += (uint64_t)rs.width * rs.height;
     max_tiles_x = rs.tiles_x > max_tiles_x ? rs.tiles_x : max_tiles_x;
     max_tiles_y = rs.tiles_y > max_tiles_y ? rs.tiles_y : max_tiles_y;
+    if (rs.tiles_x * rs.tiles_y > max_view_tiles) max_view_tiles = rs.tiles_x * rs.tiles_y;
     if (blocks * kCompactBlock > (1ull << 30) || tiles > (1u << 24)) return VKGS_ERR_CAPACITY;
   }
   c->h_view_blocks[n_views] = (uint32_t)blocks;
@@ -375,6 +376,17 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   L.lds_batch = rs0.lds_batch;
   L.waves_per_cu = rs0.waves_per_cu;
   c->cap_items = cap;
+  // view-major local tile keys when they save a tile-sort pass and the view
+  // index fits above the 25-bit depth key (128 views) and the local tile key
+  const uint32_t view_bits = (uint32_t)bits_for(n_views);
+  const uint32_t global_bits = (uint32_t)bits_for((uint32_t)tiles);
+  const uint32_t local_bits = bits_for(max_view_tiles) > 1 ? (uint32_t)bits_for(max_view_tiles) : 1u;
+  const bool local = !(cfg->flags & VKGS_DIFF_GLOBAL_TILE_KEYS) && view_bits + kDepthRankBits <= 32 &&
+                     view_bits + local_bits <= 31 && sort_passes(local_bits) < sort_passes(global_bits);
+  L.local_tile_bits = local ? local_bits : 0u;
+  const uint32_t depth_bits = local ? kDepthRankBits + view_bits : kDepthKeyBits;
+  const uint32_t depth_narrow = local ? kDepthRankNarrowPass : kDepthNarrowPass;
+  const uint32_t tile_bits = local ? local_bits : global_bits;
 
   // ---- processing order (when due) ----
   if (use_perm && c->perm_age == kPermRefresh) {
@@ -414,14 +426,14 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
                                        misc + kMiscSlotPairs, c->scan_storage.p, s));
   if (packed) {
     VKGS_HIP_CHECK(sort_kv64_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(),
-                                    c->val64.as<uint64_t>(), kDepthKeyBits, c->depth_sort.p, s, kDepthNarrowPass));
+                                    c->val64.as<uint64_t>(), depth_bits, c->depth_sort.p, s, depth_narrow));
     VKGS_HIP_CHECK(launch_diff_view_slots(L, c->counts.as<uint32_t>(), misc + kMiscVisible, c->vslot.as<uint32_t>(),
                                           s));
     VKGS_HIP_CHECK(scan_rect_counts_launch(c->scan, c->val64.as<uint64_t>(), rb, c->offs.as<uint32_t>(),
                                            misc + kMiscVisible, misc + kMiscPairs, c->scan_storage.p, s));
   } else {
-    VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, kDepthKeyBits,
-                                  c->depth_sort.p, s, kDepthNarrowPass));
+    VKGS_HIP_CHECK(sort_kv_launch(c->depth_plan, misc + kMiscVisible, c->key.as<uint32_t>(), order, depth_bits,
+                                  c->depth_sort.p, s, depth_narrow));
     VKGS_HIP_CHECK(launch_diff_bin_gather(c->rc.as<uint4>(), order, misc + kMiscVisible, cap,
                                           c->rect.as<TileRect>(), c->cnt.as<uint32_t>(), c->kview.as<uint32_t>(), s));
     VKGS_HIP_CHECK(scan_exclusive_launch(c->scan, c->cnt.as<uint32_t>(), c->offs.as<uint32_t>(),
@@ -430,7 +442,6 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   // with an odd number of tile-sort passes the pairs are emitted into the
   // sort's alternate buffers, so the sorted result lands in pkeys / pvals
   // without a final copy
-  const uint32_t tile_bits = (uint32_t)bits_for((uint32_t)tiles);
   const bool odd = sort_passes(tile_bits) % 2 == 1;
   uint32_t *ek = c->pkeys.as<uint32_t>(), *ev = c->pvals.as<uint32_t>();
   if (odd) sort_alt_buffers(c->pair_plan, c->pair_sort.p, &ek, &ev);
@@ -454,7 +465,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   VKGS_HIP_CHECK(fold_status_launch(misc + kMiscStatus, misc + kMiscPairs, misc + kMiscSticky,
                                     misc + kMiscStickyPairs, s));
   VKGS_HIP_CHECK(launch_ranges(c->pkeys.as<uint32_t>(), misc + kMiscPairs, c->max_pairs, (uint32_t)tiles,
-                               c->ranges.as<uint32_t>(), s));
+                               c->ranges.as<uint32_t>(), s, false, local ? L.view_tiles : nullptr, local_bits));
   VKGS_HIP_CHECK(launch_tile_order(c->ranges.as<uint32_t>(), (uint32_t)tiles, c->tile_keys.as<uint32_t>(),
                                    c->tile_order.as<uint32_t>(), c->tile_plan, c->tile_sort.p, s));
   if (c->timing) {

@@ -71,6 +71,13 @@ struct DiffLaunch {
   // launch choices (host side): 8x8 tiles two per wave (RasterShape::pairs),
   // backward LDS batch depth on 8x8 tiles, resident-wave cap of the rasters
   uint32_t pairs, lds_batch, waves_per_cu;
+  // view-major local tile keys (0: off).  On: the depth key is view << 25 |
+  // depth_rank_key(1 - z), so pairs are emitted view by view in depth order,
+  // and a pair's tile key is view << local_tile_bits | the tile in its view;
+  // the tile sort then orders the low local_tile_bits bits alone (stable, so
+  // each (view, tile) run stays contiguous and in depth order), one radix
+  // pass fewer than a sort on the global tile id when views are many.
+  uint32_t local_tile_bits;
   // optional work counter of a raster launch: work[i * waves + w] = list
   // entries wave w of workgroup i stepped through (two-tile kernels: one
   // word per tile, its own entries), so 64 x the sum counts the (pixel,
@@ -96,8 +103,8 @@ hipError_t launch_diff_count(const DiffLaunch& L, uint32_t* counts, uint64_t* ma
 size_t diff_count_scan_bytes(uint32_t n_blocks);
 // Per visible item at its slot offsets[w] + (visible lanes of wave w before
 // it): raster record rec[3 slot..], (tile rect x, tile rect y, tile count,
-// view) in rc[slot], the tile count again in scnt[slot], key = bits(1 - z),
-// val = slot.
+// view) in rc[slot], the tile count again in scnt[slot], key = bits(1 - z)
+// (with local tile keys: view << 25 | depth_rank_key(1 - z)), val = slot.
 hipError_t launch_diff_write(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks, float4* rec,
                              uint4* rc, uint32_t* scnt, uint32_t* key, uint32_t* val, hipStream_t s);
 // Same with the tile rectangle packed into the depth sort's value:

@@ -103,9 +103,12 @@ hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, c
                                   uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear = nullptr,
                                   uint32_t n_clear = 0);
 // [start, end) per tile from the tile-sorted keys; `cleared`: the ranges were
-// already zeroed (by the emit launch).
+// already zeroed (by the emit launch).  Keys are tile ids, or with view_tiles
+// view << local_bits | tile in the view, for tile view_tiles[view] + tile in
+// the view (each key's pairs contiguous, the keys in any order).
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
-                         uint32_t* ranges, hipStream_t s, bool cleared = false);
+                         uint32_t* ranges, hipStream_t s, bool cleared = false, const uint32_t* view_tiles = nullptr,
+                         uint32_t local_bits = 0);
 // Tiles ordered longest list first, by a log-spaced bucket of the list
 // length (stable by tile id within a bucket); one 8-bit sort pass.
 hipError_t launch_tile_order(const uint32_t* ranges, uint32_t tiles, uint32_t* keys, uint32_t* order,

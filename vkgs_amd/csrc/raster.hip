@@ -156,7 +156,9 @@ __global__ void ranges_fill_kernel(uint32_t* __restrict__ ranges, uint32_t n2) {
 // twice.  Positions before 0 or at K compare as kNoTile.
 __global__ __launch_bounds__(256) void ranges_mark_kernel(const uint32_t* __restrict__ keys,
                                                           const uint32_t* __restrict__ d_pairs, uint64_t max_pairs,
-                                                          uint32_t* __restrict__ ranges) {
+                                                          uint32_t* __restrict__ ranges,
+                                                          const uint32_t* __restrict__ view_tiles,
+                                                          uint32_t local_bits) {
   constexpr int kRows = 4;
   constexpr uint32_t kNoTile = 0xffffffffu;  // tile ids are below 2^24
   const uint64_t K = min((uint64_t)*d_pairs, max_pairs);
@@ -181,9 +183,11 @@ __global__ __launch_bounds__(256) void ranges_mark_kernel(const uint32_t* __rest
       if (lane == 0) prev = row_prev;
       if (lane == kWave - 1) next = row_next;
       const uint64_t i = base + r * kWave + lane;
-      if (i < K) {
-        if (prev != t[r]) ranges[2 * t[r]] = (uint32_t)i;
-        if (next != t[r]) ranges[2 * t[r] + 1] = (uint32_t)(i + 1);
+      if (i < K && (prev != t[r] || next != t[r])) {
+        // a view << local_bits | local tile key names tile view_tiles[view] + local tile
+        const uint32_t g = view_tiles ? view_tiles[t[r] >> local_bits] + (t[r] & ((1u << local_bits) - 1u)) : t[r];
+        if (prev != t[r]) ranges[2 * g] = (uint32_t)i;
+        if (next != t[r]) ranges[2 * g + 1] = (uint32_t)(i + 1);
       }
     }
   }
@@ -755,10 +759,12 @@ hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, c
 }
 
 hipError_t launch_ranges(const uint32_t* keys, const uint32_t* d_pairs, uint64_t max_pairs, uint32_t tiles,
-                         uint32_t* ranges, hipStream_t s, bool cleared) {
+                         uint32_t* ranges, hipStream_t s, bool cleared, const uint32_t* view_tiles,
+                         uint32_t local_bits) {
   if (!cleared) ranges_fill_kernel<<<grid_for(2ull * tiles, 256, 1u << 20), 256, 0, s>>>(ranges, 2 * tiles);
   if (max_pairs > 0)
-    ranges_mark_kernel<<<grid_for(max_pairs, 1024, 2048), 256, 0, s>>>(keys, d_pairs, max_pairs, ranges);
+    ranges_mark_kernel<<<grid_for(max_pairs, 1024, 2048), 256, 0, s>>>(keys, d_pairs, max_pairs, ranges, view_tiles,
+                                                                       local_bits);
   return hipGetLastError();
 }
 

@@ -34,6 +34,16 @@ hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_
 // longer (6M splats: 0.168 -> 0.161 ms).
 constexpr uint32_t kDepthKeyBits = 30;
 constexpr uint32_t kDepthNarrowPass = 2;
+
+// The same order as an integer: k = fl(1 - z) * 2^24, exact because every
+// fl(1 - z) with z in [0, 1] is a multiple of 2^-24 (so k in [0, 2^24] orders
+// and ties as the float bits do), in 25 bits, which leaves room above it for
+// a view index in a 32-bit key.
+constexpr uint32_t kDepthRankBits = 25;
+constexpr uint32_t kDepthRankNarrowPass = 0;
+__host__ __device__ inline uint32_t depth_rank_key(float one_minus_z) {
+  return (uint32_t)(one_minus_z * 16777216.f);
+}
 // Radix passes of a key_bits-bit sort (8 bits per pass).
 uint32_t sort_passes(uint32_t key_bits);
 // The storage's alternate (key, value) buffers of a 4-byte-value plan.

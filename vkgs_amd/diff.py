@@ -102,6 +102,7 @@ def _stream() -> int:
 
 
 SPATIAL_ORDER = 1  # VKGS_DIFF_SPATIAL_ORDER
+GLOBAL_TILE_KEYS = 16  # VKGS_DIFF_GLOBAL_TILE_KEYS
 
 
 def _config(tile=(8, 8), t_min=1.0 / 8192.0, spatial_order=False, config=None):
