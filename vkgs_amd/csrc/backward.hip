@@ -130,65 +130,73 @@ __device__ __forceinline__ void load_inputs(const DiffScene& sc, uint32_t id, Sp
   in.opacity = 1.f / (1.f + expf(-sc.opacity[id]));
 }
 
-// projection.comp:80-179 for one view; fills the 48-byte instance.
-__device__ void diff_project(const FrameConst& fc, const SplatInputs& in, float4 out[3]) {
-  float S[9];
-  for (int i = 0; i < 9; ++i) S[i] = in.S[i];
-  float p4[4] = {in.p[0], in.p[1], in.p[2], 1.f};
-  const float d0 = p4[0] - fc.eye_model[0], d1 = p4[1] - fc.eye_model[1], d2 = p4[2] - fc.eye_model[2];
-  const float dl = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
-  const float dx = d0 / dl, dy = d1 / dl, dz = d2 / dl;
-  float m3[9], t4[4];
-  m3_from_m4(fc.M, m3);
-  m3_sandwich(m3, S);
-  m4_mulv(fc.M, p4, t4);
-  m3_from_m4(fc.V, m3);
-  m3_sandwich(m3, S);
-  m4_mulv(fc.V, t4, p4);
-  const float x = p4[0], y = p4[1], z = p4[2];
-  const float r = sqrtf((x * x + y * y) + z * z);
-  float J[9] = {-1.f / z, 0.f, (-2.f * x) / r, 0.f, -1.f / z, (-2.f * y) / r, (x / z) / z, (y / z) / z,
-                (-2.f * z) / r};
-  m3_sandwich(J, S);
-  const float Ps[4] = {fc.P[0], fc.P[1], fc.P[4], fc.P[5]};
-  const float S2[4] = {S[0], S[1], S[3], S[4]};
-  float cov2[4];
-  m2_mul(Ps, S2, cov2);
-  m2_mul(cov2, Ps, cov2);
-  cov2[0] = cov2[0] + fc.inv_w2;
-  cov2[3] = cov2[3] + fc.inv_h2;
-  const float a = cov2[0], b = cov2[3], c = cov2[2];
-  const float D = sqrtf((a - b) * (a - b) + (4.f * c) * c);
-  const float s0 = sqrtf(0.5f * ((a + b) + D));
-  const float s1 = sqrtf(0.5f * ((a + b) - D));
-  const float sin2t = (2.f * c) / D;
-  const float cos2t = (a - b) / D;
+// projection.comp:80-179 for one view; fills the 48-byte instance.  The
+// differentiable path is held to float64, not to the reference's rounding,
+// so this form contracts multiply-adds into FMAs, uses the hardware
+// reciprocal, square root and reciprocal square root (about 1 ulp), maps the
+// mean and the covariance to view space with the one product V M formed on
+// the host, and forms only the 2x2 block of J S' J^T that the shader reads
+// (its third row of J never reaches the output).  Same quantities as the
+// reference's op order to float rounding; NaN for the same degenerate splats.
+__device__ void diff_project(const DiffView& view, const SplatInputs& in, float4 out[3]) {
+#pragma clang fp contract(fast)
+  const FrameConst& fc = view.fc;
+  const float* W = view.vm;  // column-major
+  float pv[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) pv[r] = W[r] * in.p[0] + W[4 + r] * in.p[1] + W[8 + r] * in.p[2] + W[12 + r];
+  const float x = pv[0], y = pv[1], z = pv[2];
+  // S' = W3 S W3^T; S symmetric, so S[3k + c] = S(k, c); WS(r, c) = sum_k W3(r, k) S(k, c)
+  float WS[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) WS[3 * r + c] = W[r] * in.S[c] + W[4 + r] * in.S[3 + c] + W[8 + r] * in.S[6 + c];
+  auto sp = [&](int r, int c) { return WS[3 * r] * W[c] + WS[3 * r + 1] * W[4 + c] + WS[3 * r + 2] * W[8 + c]; };
+  const float s00 = sp(0, 0), s01 = sp(0, 1), s02 = sp(0, 2), s11 = sp(1, 1), s12 = sp(1, 2), s22 = sp(2, 2);
+  // rows 0 and 1 of J: (-1/z, 0, x/z^2) and (0, -1/z, y/z^2)
+  const float iz = __builtin_amdgcn_rcpf(z);
+  const float ja = -iz, jb = (x * iz) * iz, jc = (y * iz) * iz;
+  const float c00 = ja * (ja * s00 + 2.f * jb * s02) + jb * jb * s22;
+  const float c01 = ja * (ja * s01 + jc * s02 + jb * s12) + jb * jc * s22;
+  const float c11 = ja * (ja * s11 + 2.f * jc * s12) + jc * jc * s22;
+  // cov2 = Ps S2 Ps with Ps = [[P0, P4], [P1, P5]]
+  const float P0 = fc.P[0], P1 = fc.P[1], P4 = fc.P[4], P5 = fc.P[5];
+  const float T00 = P0 * c00 + P4 * c01, T01 = P0 * c01 + P4 * c11;
+  const float T10 = P1 * c00 + P5 * c01, T11 = P1 * c01 + P5 * c11;
+  const float a = (T00 * P0 + T01 * P1) + fc.inv_w2;
+  const float b = (T10 * P4 + T11 * P5) + fc.inv_h2;
+  const float c = T00 * P4 + T01 * P5;
+  const float D = __builtin_amdgcn_sqrtf((a - b) * (a - b) + (4.f * c) * c);
+  const float s0 = __builtin_amdgcn_sqrtf(0.5f * ((a + b) + D));
+  const float s1 = __builtin_amdgcn_sqrtf(0.5f * ((a + b) - D));
+  const float iD = __builtin_amdgcn_rcpf(D);
+  const float sin2t = (2.f * c) * iD;
+  const float cos2t = (a - b) * iD;
   float ct = (1.f + cos2t) * 0.5f;
   float st = (1.f - cos2t) * 0.5f;
   ct = ct < 0.f ? 0.f : ct;
   st = st < 0.f ? 0.f : st;
-  float cos_t = sqrtf(ct);
-  float sin_t = copysignf(sqrtf(st), sin2t);
+  float cos_t = __builtin_amdgcn_sqrtf(ct);
+  float sin_t = copysignf(__builtin_amdgcn_sqrtf(st), sin2t);
   if (sin2t != sin2t || cos2t != cos2t) {
     cos_t = __builtin_nanf("");
     sin_t = __builtin_nanf("");
   }
-  m4_mulv(fc.P, p4, t4);
-  const float nx = t4[0] / t4[3], ny = t4[1] / t4[3], nz = t4[2] / t4[3];
+  float t4[4];
+  m4_mulv(fc.P, pv, t4);
+  const float iw = __builtin_amdgcn_rcpf(t4[3]);
+  const float nx = t4[0] * iw, ny = t4[1] * iw, nz = t4[2] * iw;
+  const float d0 = in.p[0] - fc.eye_model[0], d1 = in.p[1] - fc.eye_model[1], d2 = in.p[2] - fc.eye_model[2];
+  const float idl = __builtin_amdgcn_rsqf(d0 * d0 + d1 * d1 + d2 * d2);
   float basis[16];
-  sh_basis(dx, dy, dz, basis);
+  sh_basis(d0 * idl, d1 * idl, d2 * idl, basis);
   float color[3];
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
     float acc = 0.f;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      float dot = basis[4 * j + 0] * in.sh[ch * 16 + 4 * j + 0];
-      dot = dot + basis[4 * j + 1] * in.sh[ch * 16 + 4 * j + 1];
-      dot = dot + basis[4 * j + 2] * in.sh[ch * 16 + 4 * j + 2];
-      dot = dot + basis[4 * j + 3] * in.sh[ch * 16 + 4 * j + 3];
-      acc = (j == 0) ? dot : acc + dot;
-    }
+    for (int j = 0; j < 16; ++j) acc += basis[j] * in.sh[ch * 16 + j];
     color[ch] = glsl_max(acc + 0.5f, 0.f);
   }
   out[0] = make_float4(nx, ny, nz, 0.f);
@@ -372,7 +380,7 @@ __global__ __launch_bounds__(kCompactBlock) void diff_write_kernel(DiffLaunch L,
     float omz;
     (void)rank_visible(view.fc.PVM, in.p[0], in.p[1], in.p[2], &omz);
     float4 o[3];
-    diff_project(view.fc, in, o);
+    diff_project(view, in, o);
     const float f[12] = {o[0].x, o[0].y, o[0].z, o[0].w, o[1].x, o[1].y, o[1].z, o[1].w, o[2].x, o[2].y, o[2].z, o[2].w};
     Splat2D sp = {};
     uint4 r = make_uint4(0u, 0u, 0u, v);
@@ -1341,16 +1349,19 @@ __global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* _
   }
 }
 
-// Row-major 3x3 helpers.
+// Row-major 3x3 helpers (of the projection backward: FMA-contracted).
 __device__ __forceinline__ void mm3(const float* A, const float* B, float* C) {
+#pragma clang fp contract(fast)
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
 }
 __device__ __forceinline__ void mm3_tn(const float* A, const float* B, float* C) {  // A^T B
+#pragma clang fp contract(fast)
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) C[3 * r + c] = A[r] * B[c] + A[3 + r] * B[3 + c] + A[6 + r] * B[6 + c];
 }
 __device__ __forceinline__ void mm3_nt(const float* A, const float* B, float* C) {  // A B^T
+#pragma clang fp contract(fast)
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[3 * c] + A[3 * r + 1] * B[3 * c + 1] + A[3 * r + 2] * B[3 * c + 2];
 }
@@ -1368,25 +1379,25 @@ struct SplatState {
 
 // Chains the 9 gradient words gv of one view (colour 3, opacity 1, conic 3,
 // centre 2) through that view's projection and adds the result to the
-// view-summed dL/dSigma (GSig), dL/dmean (gm) and dL/dSH (gsh).
-__device__ __forceinline__ void chain_view(const FrameConst& fc, const SplatState& P, const float gv[9],
+// view-summed dL/dSigma (GSig), dL/dmean (gm) and dL/dSH (gsh).  The forward
+// quantities are recomputed in the forward's arithmetic (FMA contraction,
+// hardware reciprocals, the view's V M product).
+__device__ __forceinline__ void chain_view(const DiffView& view, const SplatState& P, const float gv[9],
                                            float GSig[9], float gm[3], float gsh[48]) {
+#pragma clang fp contract(fast)
+  const FrameConst& fc = view.fc;
   // ---- recompute the forward (row-major) ----
-  float V3[9], M3[9], Wm[9];  // W = V3 * M3 from the column-major matrices
+  float Wm[9];  // W = (V M)_3x3, row-major
   for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) {
-      V3[3 * r + c] = fc.V[4 * c + r];
-      M3[3 * r + c] = fc.M[4 * c + r];
-    }
-  mm3(V3, M3, Wm);
-  float p4[4] = {P.m[0], P.m[1], P.m[2], 1.f}, t4[4], pv[4];
-  m4_mulv(fc.M, p4, t4);
-  m4_mulv(fc.V, t4, pv);
+    for (int c = 0; c < 3; ++c) Wm[3 * r + c] = view.vm[4 * c + r];
+  float pv[4];
+  for (int r = 0; r < 4; ++r)
+    pv[r] = view.vm[r] * P.m[0] + view.vm[4 + r] * P.m[1] + view.vm[8 + r] * P.m[2] + view.vm[12 + r];
   const float x = pv[0], y = pv[1], z = pv[2];
   float WS[9], Sp[9];
   mm3(Wm, P.Sig, WS);
   mm3_nt(WS, Wm, Sp);
-  const float iz = 1.f / z, iz2 = iz * iz;
+  const float iz = __builtin_amdgcn_rcpf(z), iz2 = iz * iz;
   const float J2[6] = {-iz, 0.f, x * iz2, 0.f, -iz, y * iz2};
   float JS[6];  // J2 Sp (2x3)
   for (int r = 0; r < 2; ++r)
@@ -1406,7 +1417,8 @@ __device__ __forceinline__ void chain_view(const FrameConst& fc, const SplatStat
   const float hw = 0.5f * (float)fc.width, hh = 0.5f * (float)fc.height;
   const float cp00 = cov[0] * hw * hw, cp01 = cov[1] * hw * hh, cp10 = cov[2] * hh * hw, cp11 = cov[3] * hh * hh;
   const float det = cp00 * cp11 - cp01 * cp10;
-  const float Q00 = cp11 / det, Q01 = -cp01 / det, Q10 = -cp10 / det, Q11 = cp00 / det;
+  const float idet = __builtin_amdgcn_rcpf(det);
+  const float Q00 = cp11 * idet, Q01 = -cp01 * idet, Q10 = -cp10 * idet, Q11 = cp00 * idet;
 
   // ---- conic -> cov2d (pixels) -> cov2d (NDC) -> S2 ----
   const float GQ00 = gv[4], GQ01 = 0.5f * gv[5], GQ11 = gv[6];
@@ -1453,7 +1465,7 @@ __device__ __forceinline__ void chain_view(const FrameConst& fc, const SplatStat
     float clip[4];
     float pv4[4] = {x, y, z, 1.f};
     m4_mulv(fc.P, pv4, clip);
-    const float iw = 1.f / clip[3];
+    const float iw = __builtin_amdgcn_rcpf(clip[3]);
     const float ndx = clip[0] * iw, ndy = clip[1] * iw;
     const float gx = gv[7] * hw, gy = gv[8] * hh;  // d/d ndc
     for (int j = 0; j < 3; ++j) {
@@ -1466,8 +1478,8 @@ __device__ __forceinline__ void chain_view(const FrameConst& fc, const SplatStat
 
   // ---- colour: SH coefficients and view direction ----
   const float d0 = P.m[0] - fc.eye_model[0], d1 = P.m[1] - fc.eye_model[1], d2 = P.m[2] - fc.eye_model[2];
-  const float dl = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
-  const float dx = d0 / dl, dy = d1 / dl, dz = d2 / dl;
+  const float idl = __builtin_amdgcn_rsqf(d0 * d0 + d1 * d1 + d2 * d2);
+  const float dx = d0 * idl, dy = d1 * idl, dz = d2 * idl;
   float basis[16];
   sh_basis(dx, dy, dz, basis);
   float gcol[3];
@@ -1521,9 +1533,9 @@ __device__ __forceinline__ void chain_view(const FrameConst& fc, const SplatStat
     gdx += gb[15] * -3.f * C30 * (XX - YY);
     gdy += gb[15] * 6.f * C30 * X * Y;
     const float dd = gdx * X + gdy * Y + gdz * Z;
-    gm[0] += (gdx - X * dd) / dl;
-    gm[1] += (gdy - Y * dd) / dl;
-    gm[2] += (gdz - Z * dd) / dl;
+    gm[0] += (gdx - X * dd) * idl;
+    gm[1] += (gdy - Y * dd) * idl;
+    gm[2] += (gdz - Z * dd) * idl;
   }
 }
 
@@ -1598,7 +1610,7 @@ __global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(Di
     if (!nz) continue;
     any = true;
     gop += gv[3];
-    chain_view(view.fc, P, gv, GSig, gm, gsh);
+    chain_view(view, P, gv, GSig, gm, gsh);
   }
   if (!valid) return;
 

@@ -229,6 +229,7 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
     if (!make_raster_shape(a.camera.screen_size[0], a.camera.screen_size[1], cfg, &rs)) return VKGS_ERR_INVALID_ARG;
     DiffView& d = c->h_views[v];
     make_frame_const(&a.camera, a.model, &d.fc);
+    m4_mul(d.fc.V, d.fc.M, d.vm);
     d.fc.tile_w = rs.tile_w;
     d.fc.tile_h = rs.tile_h;
     d.fc.tiles_x = rs.tiles_x;

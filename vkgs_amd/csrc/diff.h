@@ -48,6 +48,7 @@ struct DiffScene {
 
 struct DiffView {
   FrameConst fc;            // camera, model, screen and tile shape
+  float vm[16];             // V * M (column-major): model space to view space in one product
   uint32_t scene;
   uint32_t block_base;      // first compaction block of this view
   uint32_t tile_base;       // first global tile of this view
