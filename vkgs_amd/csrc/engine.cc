@@ -29,7 +29,9 @@ using namespace vkgs;
 
 namespace {
 
-enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscWords = 64 };
+// misc words; the frame-constant launch zeroes the first four every frame
+// (kMiscKeyMax: the frame's largest depth key, for the depth sort)
+enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscKeyMax = 3, kMiscWords = 64 };
 constexpr int kEvents = 7;
 constexpr int kStages = 8;
 
@@ -98,7 +100,7 @@ struct vkgs_renderer {
   uint64_t max_pairs = 0;
   DevBuf rec, rect, cnt, offs, scan_storage, pkeys, pvals, psort, ranges, tile_keys, tile_order, tile_sort;
   SortPlan tile_plan{};
-  DevBuf sorted_ids, list_ids;
+  DevBuf sorted_ids, sorted_keys, list_ids;
   DevBuf work;  // list entries composited per raster wave (timed frames)
   SortPlan depth_plan{}, pair_plan{};
   ScanPlan scan{};
@@ -348,21 +350,22 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
     e = launch_fused_write_packed(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
                                   r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(),
                                   r->key.as<uint32_t>(), r->val64.as<uint64_t>(), r->ids.as<uint32_t>(),
-                                  r->counts.as<uint32_t>(), s);
+                                  r->counts.as<uint32_t>(), misc + kMiscKeyMax, s);
   else
     e = launch_fused_write(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
                            r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(), r->rc.as<uint4>(),
                            r->key.as<uint32_t>(), r->val.as<uint32_t>(), r->ids.as<uint32_t>(),
-                           r->counts.as<uint32_t>(), s);
+                           r->counts.as<uint32_t>(), misc + kMiscKeyMax, s);
   if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[2], s);
-  // keys are bits(1 - z) <= 0x3f800000: 30 significant bits
+  // keys are k = fl(1 - z) * 2^24 <= 2^24: 25 bits, the passes above the
+  // frame's largest k skipped on the device
   if (r->packed)
     e = sort_kv64_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val64.as<uint64_t>(),
-                         kDepthKeyBits, r->sort_storage.p, s, kDepthNarrowPass);
+                         kDepthRankBits, r->sort_storage.p, s, kDepthRankNarrowLast, misc + kMiscKeyMax);
   else
     e = sort_kv_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val.as<uint32_t>(),
-                       kDepthKeyBits, r->sort_storage.p, s, kDepthNarrowPass);
+                       kDepthRankBits, r->sort_storage.p, s, kDepthRankNarrowLast, misc + kMiscKeyMax);
   if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[3], s);
   hipEvent_t marks[2] = {r->ev[4], r->ev[5]};
@@ -424,7 +427,7 @@ int vkgs_renderer_destroy(vkgs_renderer* r) {
                     &r->counts,     &r->rc,         &r->key,      &r->val,        &r->val64,     &r->ids,
                     &r->sort_storage, &r->rec,      &r->rect,     &r->cnt,        &r->offs,      &r->scan_storage,
                     &r->pkeys,      &r->pvals,      &r->psort,    &r->ranges,     &r->tile_keys, &r->tile_order,
-                    &r->tile_sort,  &r->sorted_ids, &r->list_ids};
+                    &r->tile_sort,  &r->sorted_ids, &r->sorted_keys, &r->list_ids};
   for (DevBuf* b : bufs) b->release();
   if (r->capture_stream) hipStreamDestroy(r->capture_stream);
   for (int i = 0; i < kEvents; ++i)
@@ -623,9 +626,15 @@ int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const 
   else
     VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val.as<uint32_t>(), nullptr, misc + kMiscVisible,
                                      cap, r->sorted_ids.as<uint32_t>(), nullptr));
+  // the sort ran on the integer keys fl(1 - z) * 2^24: hand back rank.comp:39's float bits
+  if (d_keys) {
+    VKGS_HIP_CHECK(r->sorted_keys.ensure((size_t)cap * 4));
+    VKGS_HIP_CHECK(launch_depth_key_bits(r->key.as<uint32_t>(), misc + kMiscVisible, cap,
+                                         r->sorted_keys.as<uint32_t>(), nullptr));
+  }
   VKGS_HIP_CHECK(hipDeviceSynchronize());
   *d_sorted_ids = r->sorted_ids.as<uint32_t>();
-  if (d_keys) *d_keys = r->key.as<uint32_t>();
+  if (d_keys) *d_keys = r->sorted_keys.as<uint32_t>();
   return VKGS_OK;
 }
 

@@ -37,19 +37,20 @@ hipError_t launch_projection(const FrameConst& fc, const float* pos, const float
 // including the screen and tile shape), in two launches: visible counts per
 // workgroup + their scan (-> *d_count), then per visible splat at its slot:
 // raster record rec[3 slot..], (tile rect x, tile rect y, tile count, 0) in
-// rc[slot], key = bits(1 - z), val = slot, ids[slot] = splat id.
+// rc[slot], key = depth_rank_key(1 - z), val = slot, ids[slot] = splat id,
+// and *key_max raised to the largest key (the caller zeroes it per frame).
 hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t n, uint32_t* d_count,
                               uint32_t* counts, hipStream_t s);
 hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
                               const uint16_t* sh, uint32_t n, float4* rec, uint4* rc, uint32_t* key, uint32_t* val,
-                              uint32_t* ids, const uint32_t* counts, hipStream_t s);
+                              uint32_t* ids, const uint32_t* counts, uint32_t* key_max, hipStream_t s);
 
 // Same, with the tile rectangle packed into the sort value instead of rc:
 // val64[slot] = slot | packed rect << 32 (frame tiles must fit rect_bits).
 hipError_t launch_fused_write_packed(const FrameConst* d_fc, const float* pos, const float* cov3d,
                                      const float* opacity, const uint16_t* sh, uint32_t n, float4* rec,
                                      uint32_t* key, uint64_t* val64, uint32_t* ids, const uint32_t* counts,
-                                     hipStream_t s);
+                                     uint32_t* key_max, hipStream_t s);
 
 // Copies a FrameConst passed by value into device memory (stream-ordered)
 // and zeroes n_clear words at clear (optional).
@@ -167,5 +168,9 @@ hipError_t run_draw_packed(const uint64_t* val64, const uint32_t* d_count, uint3
 // src[idx[idx2[k] * idx_stride]] for k < *d_count.
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uint32_t* idx2, const uint32_t* d_count,
                              uint32_t max_n, uint32_t* out, hipStream_t s, uint32_t idx_stride = 1);
+// out[i] = the float bits of fl(1 - z) from the integer depth key k[i] =
+// fl(1 - z) * 2^24 (depth_key_bits) for i < *d_count.
+hipError_t launch_depth_key_bits(const uint32_t* k, const uint32_t* d_count, uint32_t max_n, uint32_t* out,
+                                 hipStream_t s);
 
 }  // namespace vkgs

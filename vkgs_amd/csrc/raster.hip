@@ -635,6 +635,13 @@ __global__ void gather_u32_kernel(const uint32_t* __restrict__ src, const uint32
   }
 }
 
+__global__ void depth_key_bits_kernel(const uint32_t* __restrict__ k, const uint32_t* __restrict__ d_count,
+                                      uint32_t max_n, uint32_t* __restrict__ out) {
+  const uint32_t n = min(*d_count, max_n);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = depth_key_bits(k[i]);
+}
+
 uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
   uint64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;
@@ -838,6 +845,12 @@ hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, const uin
                              uint32_t max_n, uint32_t* out, hipStream_t s, uint32_t idx_stride) {
   if (max_n == 0) return hipSuccess;
   gather_u32_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(src, idx, idx2, idx_stride, d_count, max_n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_depth_key_bits(const uint32_t* k, const uint32_t* d_count, uint32_t max_n, uint32_t* out,
+                                 hipStream_t s) {
+  depth_key_bits_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(k, d_count, max_n, out);
   return hipGetLastError();
 }
 

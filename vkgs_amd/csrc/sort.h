@@ -22,9 +22,13 @@ SortPlan sort_plan(uint32_t max_n, uint32_t val_bytes = 4);
 // Stable ascending sort of min(*d_count, max_n) pairs (d_count == nullptr:
 // max_n) on key bits [0, key_bits), result in (keys, vals).  Passes take 8
 // bits each from bit 0 up, except pass narrow_pass, which takes the
-// remainder when key_bits is not a multiple of 8.
+// remainder when key_bits is not a multiple of 8.  d_key_max (optional): a
+// device word >= every key; the passes then run in pairs (the first alone
+// for an odd count), and a later pair whose bits are zero in *d_key_max
+// returns at once, so a sort spends passes on the bits the keys use.
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
-                          uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass = 0);
+                          uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass = 0,
+                          const uint32_t* d_key_max = nullptr);
 
 // Depth keys are the bits of fl(1 - z), z in [0, 1]: every such value is a
 // multiple of 2^-24, so a distant splat's low mantissa bits are zero and
@@ -44,6 +48,18 @@ constexpr uint32_t kDepthRankNarrowPass = 0;
 __host__ __device__ inline uint32_t depth_rank_key(float one_minus_z) {
   return (uint32_t)(one_minus_z * 16777216.f);
 }
+// The float bits of fl(1 - z) back from k (exact: k <= 2^24).
+__host__ __device__ inline uint32_t depth_key_bits(uint32_t k) {
+  const float f = (float)k * (1.f / 16777216.f);
+  uint32_t b;
+  __builtin_memcpy(&b, &f, 4);
+  return b;
+}
+// The renderer's depth sort on k alone: passes of 8, 8, 8, 1 bits with the
+// frame's largest k on the device (d_key_max), so that a frame whose k stay
+// below 2^16 (every visible splat farther than 256 near-plane distances,
+// about) sorts in two passes instead of four.
+constexpr uint32_t kDepthRankNarrowLast = 3;
 // Radix passes of a key_bits-bit sort (8 bits per pass).
 uint32_t sort_passes(uint32_t key_bits);
 // The storage's alternate (key, value) buffers of a 4-byte-value plan.
@@ -55,7 +71,8 @@ hipError_t sort_kv_from_alt_launch(const SortPlan& plan, const uint32_t* d_count
                                    uint32_t key_bits, void* storage, hipStream_t stream);
 // Same with 64-bit values (plan from sort_plan(max_n, 8)).
 hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
-                            uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass = 0);
+                            uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass = 0,
+                            const uint32_t* d_key_max = nullptr);
 
 struct ScanPlan {
   uint32_t max_n = 0;

@@ -88,10 +88,22 @@ __device__ __forceinline__ void block_range(uint32_t n, uint32_t& begin, uint32_
   end = (uint32_t)(b + chunk < n ? b + chunk : n);
 }
 
+// A pass of a pair of passes that the keys' largest value makes unneeded:
+// every key's bits from skip_shift up are zero, so that pair would keep the
+// order (a stable sort on an all-zero digit), and both of its passes return
+// at once.  The pass count still has its parity, so the result lands where
+// the full sort would put it.
+__device__ __forceinline__ bool pass_skipped(const uint32_t* __restrict__ d_key_max, uint32_t skip_shift) {
+  return d_key_max && skip_shift < 32 && (*d_key_max >> skip_shift) == 0u;
+}
+
 __global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restrict__ keys,
                                                          const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                          uint32_t shift, uint32_t bits,
-                                                         uint32_t* __restrict__ hist) {
+                                                         uint32_t* __restrict__ hist,
+                                                         const uint32_t* __restrict__ d_key_max,
+                                                         uint32_t skip_shift) {
+  if (pass_skipped(d_key_max, skip_shift)) return;
   // a histogram needs no ranking: one LDS atomic per key, into one of 16
   // copies of the histogram chosen by lane (lane l counts into copy l % 16),
   // bin d of copy j at word 16 d + j.  Depth keys crowd into few digit
@@ -134,7 +146,10 @@ __global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restr
 // Exclusive scan of one digit's row hist[d][0..blocks) in place (one workgroup
 // per digit); digit_total[d] receives the row sum.
 __global__ __launch_bounds__(kThreads) void sort_scan_rows(uint32_t* __restrict__ hist, uint32_t blocks,
-                                                           uint32_t* __restrict__ digit_total) {
+                                                           uint32_t* __restrict__ digit_total,
+                                                           const uint32_t* __restrict__ d_key_max,
+                                                           uint32_t skip_shift) {
+  if (pass_skipped(d_key_max, skip_shift)) return;
   __shared__ uint32_t s_tmp[kWaves];
   uint32_t* row = hist + (size_t)blockIdx.x * blocks;
   uint32_t carry = 0;
@@ -165,7 +180,9 @@ __global__ __launch_bounds__(kThreads) void sort_downsweep(const uint32_t* __res
                                                            const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                            uint32_t shift, const uint32_t* __restrict__ hist,
                                                            const uint32_t* __restrict__ digit_total,
-                                                           uint32_t raw_hist) {
+                                                           uint32_t raw_hist, const uint32_t* __restrict__ d_key_max,
+                                                           uint32_t skip_shift) {
+  if (pass_skipped(d_key_max, skip_shift)) return;
   constexpr uint32_t bits = kBits;
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_wcnt[kWaves][256];
@@ -426,7 +443,8 @@ namespace {
 
 template <typename V>
 hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, V* vals, uint32_t key_bits,
-                       void* storage, hipStream_t stream, bool input_in_alt = false, uint32_t narrow_pass = 0) {
+                       void* storage, hipStream_t stream, bool input_in_alt = false, uint32_t narrow_pass = 0,
+                       const uint32_t* d_key_max = nullptr) {
   if (key_bits > 32) key_bits = 32;
   char* base = static_cast<char*>(storage);
   uint32_t* alt_k = reinterpret_cast<uint32_t*>(base + plan.off_alt_key);
@@ -450,16 +468,28 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
   const uint32_t n_pass = (key_bits + 7) / 8;
   const uint32_t narrow = key_bits - 8 * (n_pass - 1);
   const uint32_t narrow_at = narrow_pass < n_pass ? narrow_pass : n_pass - 1;
+  // with d_key_max the passes go in pairs that keep the pass count's parity
+  // (the first pass alone when the count is odd); a later pair runs only if
+  // *d_key_max has a bit at or above its first pass's shift
+  uint32_t pass_shift[4];
+  for (uint32_t sh = 0, p = 0; p < n_pass; ++p) {
+    pass_shift[p] = sh;
+    sh += p == narrow_at ? narrow : 8;
+  }
   for (uint32_t shift = 0, pass = 0; shift < key_bits; ++pass) {
     const uint32_t bits = pass == narrow_at ? narrow : 8;
-    sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist);
-    if (!raw) sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total);
+    const uint32_t lead = n_pass % 2 == 0 ? pass & ~1u : (pass == 0 ? 0u : ((pass - 1) & ~1u) + 1u);
+    const uint32_t skip_shift = d_key_max && lead > 0 ? pass_shift[lead] : 32u;
+    sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist, d_key_max,
+                                                       skip_shift);
+    if (!raw)
+      sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total, d_key_max, skip_shift);
     switch (bits) {
 #define VKGS_DOWNSWEEP(B)                                                                          \
   case B:                                                                                          \
     sort_downsweep<B, V><<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, \
                                                                plan.max_n, shift, hist, digit_total, \
-                                                               raw ? 1u : 0u);                     \
+                                                               raw ? 1u : 0u, d_key_max, skip_shift); \
     break;
       VKGS_DOWNSWEEP(1)
       VKGS_DOWNSWEEP(2)
@@ -486,8 +516,9 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
 }  // namespace
 
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
-                          uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass) {
-  return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass);
+                          uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass,
+                          const uint32_t* d_key_max) {
+  return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass, d_key_max);
 }
 
 uint32_t sort_passes(uint32_t key_bits) { return ((key_bits > 32 ? 32 : key_bits) + 7) / 8; }
@@ -504,8 +535,9 @@ hipError_t sort_kv_from_alt_launch(const SortPlan& plan, const uint32_t* d_count
 }
 
 hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
-                            uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass) {
-  return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass);
+                            uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass,
+                            const uint32_t* d_key_max) {
+  return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass, d_key_max);
 }
 
 ScanPlan scan_plan(uint32_t max_n) {
