@@ -365,13 +365,13 @@ def test_renderer_beyond_the_reference_capacity():
 @pytest.mark.parametrize("near", [False, True])
 def test_depth_sort_skips_the_passes_above_the_largest_key(near):
     """The renderer sorts the integer depth key k = fl(1 - z) * 2^24 (the order and ties of rank.comp's
-    float key) in passes of 8, 8, 8 and 1 bits and skips the last two when the frame's largest k is below
-    2^16, i.e. every visible splat is farther than about 256 near-plane distances.  A cloud 4-12 m from a
+    float key) in passes of 7, 8, 8 and 2 bits and skips the last two when the frame's largest k is below
+    2^15, i.e. every visible splat is farther than about 512 near-plane distances.  A cloud 8-16 m from a
     camera with a 0.01 m near plane sorts in two passes; the same cloud with 200 splats 0.05-2 m in front
     of the camera in four.  Both against the oracle: keys (handed back as rank.comp's float bits), sorted
     ids, tile lists bit-exact, image within 1/255 and PSNR >= 50 dB."""
     sc = S.gaussian_cloud(30000, seed=71, pos_std=1.0, logscale=(-3.5, 0.4))
-    cam = S.orbit_camera(320, 240, 8.0)
+    cam = S.orbit_camera(320, 240, 12.0)
     if near:
         eye = S.camera_arrays(cam)[2][:3].astype(np.float64)
         rng = np.random.default_rng(72)
@@ -384,7 +384,7 @@ def test_depth_sort_skips_the_passes_above_the_largest_key(near):
     _, okey, _ = oracle.rank(cam, np.ascontiguousarray(sc.rows[:, 0:3], np.float32))
     k_max = int((np.asarray(okey, np.uint32).view(np.float32).astype(np.float64) * 2.0 ** 24).max())
     print("largest depth key", k_max, flush=True)
-    assert (k_max >= 1 << 16) == near
+    assert (k_max >= 1 << 15) == near
     _check_frame_against_oracle(sc, cam, 320, 240)
 
 

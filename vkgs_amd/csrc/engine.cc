@@ -30,7 +30,7 @@ using namespace vkgs;
 namespace {
 
 // misc words; the frame-constant launch zeroes the first four every frame
-// (kMiscKeyMax: the frame's largest depth key, for the depth sort)
+// (kMiscKeyMax: the frame's largest depth key, found by the depth sort)
 enum : uint32_t { kMiscVisible = 0, kMiscPairs = 1, kMiscStatus = 2, kMiscKeyMax = 3, kMiscWords = 64 };
 constexpr int kEvents = 7;
 constexpr int kStages = 8;
@@ -350,22 +350,22 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
     e = launch_fused_write_packed(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
                                   r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(),
                                   r->key.as<uint32_t>(), r->val64.as<uint64_t>(), r->ids.as<uint32_t>(),
-                                  r->counts.as<uint32_t>(), misc + kMiscKeyMax, s);
+                                  r->counts.as<uint32_t>(), s);
   else
     e = launch_fused_write(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
                            r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(), r->rc.as<uint4>(),
                            r->key.as<uint32_t>(), r->val.as<uint32_t>(), r->ids.as<uint32_t>(),
-                           r->counts.as<uint32_t>(), misc + kMiscKeyMax, s);
+                           r->counts.as<uint32_t>(), s);
   if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[2], s);
-  // keys are k = fl(1 - z) * 2^24 <= 2^24: 25 bits, the passes above the
-  // frame's largest k skipped on the device
+  // keys are k = fl(1 - z) * 2^24 <= 2^24: 25 bits; the sort's first pass
+  // finds the frame's largest k and the passes above its bits return at once
   if (r->packed)
     e = sort_kv64_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val64.as<uint64_t>(),
-                         kDepthRankBits, r->sort_storage.p, s, kDepthRankNarrowLast, misc + kMiscKeyMax);
+                         kDepthRankBits, r->sort_storage.p, s, 0, misc + kMiscKeyMax, kDepthRankFirstBits);
   else
     e = sort_kv_launch(r->depth_plan, misc + kMiscVisible, r->key.as<uint32_t>(), r->val.as<uint32_t>(),
-                       kDepthRankBits, r->sort_storage.p, s, kDepthRankNarrowLast, misc + kMiscKeyMax);
+                       kDepthRankBits, r->sort_storage.p, s, 0, misc + kMiscKeyMax, kDepthRankFirstBits);
   if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[3], s);
   hipEvent_t marks[2] = {r->ev[4], r->ev[5]};

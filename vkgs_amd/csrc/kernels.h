@@ -37,20 +37,19 @@ hipError_t launch_projection(const FrameConst& fc, const float* pos, const float
 // including the screen and tile shape), in two launches: visible counts per
 // workgroup + their scan (-> *d_count), then per visible splat at its slot:
 // raster record rec[3 slot..], (tile rect x, tile rect y, tile count, 0) in
-// rc[slot], key = depth_rank_key(1 - z), val = slot, ids[slot] = splat id,
-// and *key_max raised to the largest key (the caller zeroes it per frame).
+// rc[slot], key = depth_rank_key(1 - z), val = slot, ids[slot] = splat id.
 hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t n, uint32_t* d_count,
                               uint32_t* counts, hipStream_t s);
 hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
                               const uint16_t* sh, uint32_t n, float4* rec, uint4* rc, uint32_t* key, uint32_t* val,
-                              uint32_t* ids, const uint32_t* counts, uint32_t* key_max, hipStream_t s);
+                              uint32_t* ids, const uint32_t* counts, hipStream_t s);
 
 // Same, with the tile rectangle packed into the sort value instead of rc:
 // val64[slot] = slot | packed rect << 32 (frame tiles must fit rect_bits).
 hipError_t launch_fused_write_packed(const FrameConst* d_fc, const float* pos, const float* cov3d,
                                      const float* opacity, const uint16_t* sh, uint32_t n, float4* rec,
                                      uint32_t* key, uint64_t* val64, uint32_t* ids, const uint32_t* counts,
-                                     uint32_t* key_max, hipStream_t s);
+                                     hipStream_t s);
 
 // Copies a FrameConst passed by value into device memory (stream-ordered)
 // and zeroes n_clear words at clear (optional).

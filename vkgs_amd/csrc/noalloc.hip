@@ -22,9 +22,9 @@ namespace {
 
 // sort_plan(VKGS_SORT_NOALLOC_MAX_N, 4): alternate keys and values (4 B each
 // per pair), 256 histogram words per block for at most 1024 blocks, 256 digit
-// totals, each region aligned to 256 bytes.
-constexpr size_t kScratchBytes =
-    (size_t)VKGS_SORT_NOALLOC_MAX_N * 8 + (size_t)256 * 1024 * 4 + (size_t)256 * 4 + (size_t)4 * 256;
+// totals, one word per block, each region aligned to 256 bytes.
+constexpr size_t kScratchBytes = (size_t)VKGS_SORT_NOALLOC_MAX_N * 8 + (size_t)256 * 1024 * 4 + (size_t)256 * 4 +
+                                 (size_t)1024 * 4 + (size_t)5 * 256;
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 

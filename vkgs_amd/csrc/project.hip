@@ -274,9 +274,8 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
     const float* __restrict__ opacity, const uint16_t* __restrict__ sh, uint32_t n,
     const uint32_t* __restrict__ offsets, float4* __restrict__ rec, uint4* __restrict__ rc,
     uint32_t* __restrict__ key, uint32_t* __restrict__ val, uint64_t* __restrict__ val64,
-    uint32_t* __restrict__ ids, uint32_t* __restrict__ key_max) {
+    uint32_t* __restrict__ ids) {
   __shared__ uint32_t s_w[kCompactBlock / kWave];
-  __shared__ uint32_t s_kmax;
   const FrameConst& fc = *fcp;
   const uint32_t id = blockIdx.x * kCompactBlock + threadIdx.x;
   bool vis = false;
@@ -284,16 +283,7 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
   if (id < n)
     vis = rank_visible(fc.PVM, position[3 * (size_t)id], position[3 * (size_t)id + 1], position[3 * (size_t)id + 2],
                        &omz);
-  if (threadIdx.x == 0) s_kmax = 0u;
-  const uint32_t slot = compact_slot(vis, offsets, s_w);  // (its barrier orders the s_kmax clear)
-  // the depth key: k = fl(1 - z) * 2^24, an integer in the order and ties of
-  // rank.comp:39's float key; the frame's largest k lets the depth sort skip
-  // the passes over bits no key uses
-  const uint32_t k = vis ? depth_rank_key(omz) : 0u;
-  uint32_t wk = k;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) wk = max(wk, (uint32_t)__shfl_xor((int)wk, d, kWave));
-  if (__lane_id() == 0 && wk != 0u) atomicMax(&s_kmax, wk);
+  const uint32_t slot = compact_slot(vis, offsets, s_w);
   if (vis) {
     float4 o[3];
     project_splat(fc, position, cov3d, opacity, sh, id, o);
@@ -323,13 +313,10 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
       rc[slot] = r;
       val[slot] = slot;
     }
-    key[slot] = k;
+    // k = fl(1 - z) * 2^24: an integer in the order and ties of rank.comp:39's float key
+    key[slot] = depth_rank_key(omz);
     ids[slot] = id;
   }
-  __syncthreads();
-  // one device atomic per workgroup, and only while it can raise the maximum
-  if (threadIdx.x == 0 && s_kmax != 0u && s_kmax > __atomic_load_n(key_max, __ATOMIC_RELAXED))
-    atomicMax(key_max, s_kmax);
 }
 
 // -------------------------------------------------------------- inverse ---
@@ -459,22 +446,22 @@ hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t
 
 hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
                               const uint16_t* sh, uint32_t n, float4* rec, uint4* rc, uint32_t* key, uint32_t* val,
-                              uint32_t* ids, const uint32_t* counts, uint32_t* key_max, hipStream_t s) {
+                              uint32_t* ids, const uint32_t* counts, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t nb = ceil_div(n, kCompactBlock);
   fused_write_kernel<false><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, cov3d, opacity, sh, n, counts, rec, rc, key, val,
-                                                         nullptr, ids, key_max);
+                                                         nullptr, ids);
   return hipGetLastError();
 }
 
 hipError_t launch_fused_write_packed(const FrameConst* d_fc, const float* pos, const float* cov3d,
                                      const float* opacity, const uint16_t* sh, uint32_t n, float4* rec,
                                      uint32_t* key, uint64_t* val64, uint32_t* ids, const uint32_t* counts,
-                                     uint32_t* key_max, hipStream_t s) {
+                                     hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t nb = ceil_div(n, kCompactBlock);
   fused_write_kernel<true><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, cov3d, opacity, sh, n, counts, rec, nullptr, key,
-                                                        nullptr, val64, ids, key_max);
+                                                        nullptr, val64, ids);
   return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ namespace vkgs {
 struct SortPlan {
   uint32_t max_n = 0;
   uint32_t blocks = 1;  // fixed grid; each block sorts a contiguous chunk
-  size_t off_alt_key = 0, off_alt_val = 0, off_hist = 0, off_digit_total = 0;
+  size_t off_alt_key = 0, off_alt_val = 0, off_hist = 0, off_digit_total = 0, off_block_max = 0;
   size_t bytes = 0;
 };
 
@@ -22,13 +22,16 @@ SortPlan sort_plan(uint32_t max_n, uint32_t val_bytes = 4);
 // Stable ascending sort of min(*d_count, max_n) pairs (d_count == nullptr:
 // max_n) on key bits [0, key_bits), result in (keys, vals).  Passes take 8
 // bits each from bit 0 up, except pass narrow_pass, which takes the
-// remainder when key_bits is not a multiple of 8.  d_key_max (optional): a
-// device word >= every key; the passes then run in pairs (the first alone
-// for an odd count), and a later pair whose bits are zero in *d_key_max
-// returns at once, so a sort spends passes on the bits the keys use.
+// remainder when key_bits is not a multiple of 8.  d_key_max (optional, a
+// device word; used by four-pass sorts): the first two passes store the
+// largest key there, and the last two, as a pair, return at once when that
+// maximum has no bit at or above the third pass's shift, so such a sort
+// spends passes on the bits its keys use (the result lands in place either
+// way).  first_bits (1-7, optional): the first pass takes that many bits, the
+// middle ones 8 and the last the remainder (narrow_pass then unused).
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
                           uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass = 0,
-                          const uint32_t* d_key_max = nullptr);
+                          uint32_t* d_key_max = nullptr, uint32_t first_bits = 0);
 
 // Depth keys are the bits of fl(1 - z), z in [0, 1]: every such value is a
 // multiple of 2^-24, so a distant splat's low mantissa bits are zero and
@@ -55,11 +58,12 @@ __host__ __device__ inline uint32_t depth_key_bits(uint32_t k) {
   __builtin_memcpy(&b, &f, 4);
   return b;
 }
-// The renderer's depth sort on k alone: passes of 8, 8, 8, 1 bits with the
-// frame's largest k on the device (d_key_max), so that a frame whose k stay
-// below 2^16 (every visible splat farther than 256 near-plane distances,
-// about) sorts in two passes instead of four.
-constexpr uint32_t kDepthRankNarrowLast = 3;
+// The renderer's depth sort on k alone: passes of 7, 8, 8, 2 bits with a
+// d_key_max word, so that a frame whose k stay below 2^15 (every visible
+// splat farther than about 512 near-plane distances) sorts in two passes
+// instead of four.  k's low bits are evenly spread: the 7-bit first digit
+// writes runs twice as long as an 8-bit one.
+constexpr uint32_t kDepthRankFirstBits = 7;
 // Radix passes of a key_bits-bit sort (8 bits per pass).
 uint32_t sort_passes(uint32_t key_bits);
 // The storage's alternate (key, value) buffers of a 4-byte-value plan.
@@ -72,7 +76,7 @@ hipError_t sort_kv_from_alt_launch(const SortPlan& plan, const uint32_t* d_count
 // Same with 64-bit values (plan from sort_plan(max_n, 8)).
 hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
                             uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass = 0,
-                            const uint32_t* d_key_max = nullptr);
+                            uint32_t* d_key_max = nullptr, uint32_t first_bits = 0);
 
 struct ScanPlan {
   uint32_t max_n = 0;

@@ -97,13 +97,22 @@ __device__ __forceinline__ bool pass_skipped(const uint32_t* __restrict__ d_key_
   return d_key_max && skip_shift < 32 && (*d_key_max >> skip_shift) == 0u;
 }
 
+// The keys' maximum for a sort given d_key_max, without atomics: the first
+// pass's blocks store their own maxima in block_max (max_mode 1), and block 0
+// of the second pass reduces them into *key_max (max_mode 2), before any pass
+// that tests it.
+enum : uint32_t { kMaxNone = 0, kMaxPerBlock = 1, kMaxReduce = 2 };
+
 __global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restrict__ keys,
                                                          const uint32_t* __restrict__ d_count, uint32_t max_n,
                                                          uint32_t shift, uint32_t bits,
                                                          uint32_t* __restrict__ hist,
                                                          const uint32_t* __restrict__ d_key_max,
-                                                         uint32_t skip_shift) {
+                                                         uint32_t skip_shift, uint32_t max_mode,
+                                                         uint32_t* __restrict__ block_max,
+                                                         uint32_t* __restrict__ key_max) {
   if (pass_skipped(d_key_max, skip_shift)) return;
+  __shared__ uint32_t s_wmax[kWaves];
   // a histogram needs no ranking: one LDS atomic per key, into one of 16
   // copies of the histogram chosen by lane (lane l counts into copy l % 16),
   // bin d of copy j at word 16 d + j.  Depth keys crowd into few digit
@@ -121,18 +130,38 @@ __global__ __launch_bounds__(kThreads) void sort_upsweep(const uint32_t* __restr
   uint32_t begin, end;
   block_range(n, begin, end);
   const uint32_t mask = (1u << bits) - 1u;
+  uint32_t kmax = 0;
   for (uint32_t base = begin; base < end; base += kTile) {
     uint32_t k[kIpt];
 #pragma unroll
     for (int r = 0; r < kIpt; ++r) {
       const uint32_t i = base + r * kThreads + tid;
-      k[r] = i < end ? keys[i] : 0u;
+      k[r] = i < end ? keys[i] : 0u;  // (0 for the padding: no effect on the maximum)
+      kmax = max(kmax, k[r]);
     }
 #pragma unroll
     for (int r = 0; r < kIpt; ++r)
       if (base + r * kThreads + tid < end) atomicAdd(&h[((k[r] >> shift) & mask) * kCopies], 1u);
   }
+  if (max_mode == kMaxReduce && blockIdx.x == 0) {
+    kmax = 0;
+    for (uint32_t b = tid; b < gridDim.x; b += kThreads) kmax = max(kmax, block_max[b]);
+  }
+  if (max_mode != kMaxNone) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, d, kWave));
+    if (__lane_id() == 0) s_wmax[tid / kWave] = kmax;
+  }
   __syncthreads();
+  if (max_mode != kMaxNone && tid == 0) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) m = max(m, s_wmax[i]);
+    if (max_mode == kMaxPerBlock)
+      block_max[blockIdx.x] = m;
+    else if (blockIdx.x == 0)
+      *key_max = m;
+  }
   if (tid <= mask) {
     // thread d reads its bin's copies rotated by d, so the lanes' reads fall
     // in distinct banks
@@ -435,6 +464,8 @@ SortPlan sort_plan(uint32_t max_n, uint32_t val_bytes) {
   off = align_up(off + (size_t)256 * p.blocks * 4, 256);
   p.off_digit_total = off;
   off = align_up(off + 256 * 4, 256);
+  p.off_block_max = off;
+  off = align_up(off + (size_t)p.blocks * 4, 256);
   p.bytes = off;
   return p;
 }
@@ -444,7 +475,7 @@ namespace {
 template <typename V>
 hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, V* vals, uint32_t key_bits,
                        void* storage, hipStream_t stream, bool input_in_alt = false, uint32_t narrow_pass = 0,
-                       const uint32_t* d_key_max = nullptr) {
+                       uint32_t* d_key_max = nullptr, uint32_t first_bits = 0) {
   if (key_bits > 32) key_bits = 32;
   char* base = static_cast<char*>(storage);
   uint32_t* alt_k = reinterpret_cast<uint32_t*>(base + plan.off_alt_key);
@@ -465,31 +496,43 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
   const bool raw = plan.blocks <= kRawHistBlocks;
   // the narrow digit (when key_bits is not a multiple of 8) goes in pass
   // narrow_pass, the first by default
-  const uint32_t n_pass = (key_bits + 7) / 8;
-  const uint32_t narrow = key_bits - 8 * (n_pass - 1);
-  const uint32_t narrow_at = narrow_pass < n_pass ? narrow_pass : n_pass - 1;
-  // with d_key_max the passes go in pairs that keep the pass count's parity
-  // (the first pass alone when the count is odd); a later pair runs only if
-  // *d_key_max has a bit at or above its first pass's shift
-  uint32_t pass_shift[4];
-  for (uint32_t sh = 0, p = 0; p < n_pass; ++p) {
-    pass_shift[p] = sh;
-    sh += p == narrow_at ? narrow : 8;
+  // digit widths: 8 bits each except the narrow pass's remainder, or with
+  // first_bits (1-7) the first pass that many and the last the remainder
+  uint32_t width[5];
+  uint32_t n_pass;
+  if (first_bits > 0 && first_bits < 8 && key_bits > first_bits) {
+    n_pass = 1 + (key_bits - first_bits + 7) / 8;
+    for (uint32_t p = 0; p < n_pass; ++p) width[p] = p == 0 ? first_bits : 8u;
+    width[n_pass - 1] = key_bits - first_bits - 8 * (n_pass - 2);
+  } else {
+    n_pass = (key_bits + 7) / 8;
+    const uint32_t narrow_at = narrow_pass < n_pass ? narrow_pass : n_pass - 1;
+    for (uint32_t p = 0; p < n_pass; ++p) width[p] = p == narrow_at ? key_bits - 8 * (n_pass - 1) : 8u;
   }
+  // with d_key_max and four passes, passes 0 and 1 find the keys' maximum and
+  // passes 2 and 3 run only if it has a bit at or above pass 2's shift (as a
+  // pair, so the result stays where four passes put it)
+  const bool skipping = d_key_max && n_pass == 4;
+  uint32_t* block_max = reinterpret_cast<uint32_t*>(base + plan.off_block_max);
+  const uint32_t pair_shift = n_pass == 4 ? width[0] + width[1] : 32u;  // shift of pass 2
   for (uint32_t shift = 0, pass = 0; shift < key_bits; ++pass) {
-    const uint32_t bits = pass == narrow_at ? narrow : 8;
-    const uint32_t lead = n_pass % 2 == 0 ? pass & ~1u : (pass == 0 ? 0u : ((pass - 1) & ~1u) + 1u);
-    const uint32_t skip_shift = d_key_max && lead > 0 ? pass_shift[lead] : 32u;
-    sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist, d_key_max,
-                                                       skip_shift);
+    const uint32_t bits = width[pass];
+    const uint32_t skip_shift = skipping && pass >= 2 ? pair_shift : 32u;
+    // the first two passes find the keys' maximum that the later pair tests
+    const uint32_t max_mode = !skipping ? kMaxNone : pass == 0 ? kMaxPerBlock : pass == 1 ? kMaxReduce : kMaxNone;
+    sort_upsweep<<<plan.blocks, kThreads, 0, stream>>>(src_k, d_count, plan.max_n, shift, bits, hist,
+                                                       skipping ? d_key_max : nullptr, skip_shift, max_mode,
+                                                       block_max, d_key_max);
     if (!raw)
-      sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total, d_key_max, skip_shift);
+      sort_scan_rows<<<1u << bits, kThreads, 0, stream>>>(hist, plan.blocks, digit_total,
+                                                          skipping ? d_key_max : nullptr, skip_shift);
     switch (bits) {
 #define VKGS_DOWNSWEEP(B)                                                                          \
   case B:                                                                                          \
     sort_downsweep<B, V><<<plan.blocks, kThreads, 0, stream>>>(src_k, src_v, dst_k, dst_v, d_count, \
                                                                plan.max_n, shift, hist, digit_total, \
-                                                               raw ? 1u : 0u, d_key_max, skip_shift); \
+                                                               raw ? 1u : 0u, skipping ? d_key_max : nullptr, \
+                                                               skip_shift);                        \
     break;
       VKGS_DOWNSWEEP(1)
       VKGS_DOWNSWEEP(2)
@@ -517,8 +560,9 @@ hipError_t sort_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* 
 
 hipError_t sort_kv_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint32_t* vals,
                           uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass,
-                          const uint32_t* d_key_max) {
-  return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass, d_key_max);
+                          uint32_t* d_key_max, uint32_t first_bits) {
+  return sort_launch<uint32_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass, d_key_max,
+                               first_bits);
 }
 
 uint32_t sort_passes(uint32_t key_bits) { return ((key_bits > 32 ? 32 : key_bits) + 7) / 8; }
@@ -536,8 +580,9 @@ hipError_t sort_kv_from_alt_launch(const SortPlan& plan, const uint32_t* d_count
 
 hipError_t sort_kv64_launch(const SortPlan& plan, const uint32_t* d_count, uint32_t* keys, uint64_t* vals,
                             uint32_t key_bits, void* storage, hipStream_t stream, uint32_t narrow_pass,
-                            const uint32_t* d_key_max) {
-  return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass, d_key_max);
+                            uint32_t* d_key_max, uint32_t first_bits) {
+  return sort_launch<uint64_t>(plan, d_count, keys, vals, key_bits, storage, stream, false, narrow_pass, d_key_max,
+                               first_bits);
 }
 
 ScanPlan scan_plan(uint32_t max_n) {
