@@ -39,6 +39,9 @@ RASTER_FLOPS_PER_EVAL = 20
 # clamp, transmittance recovery T / (1 - alpha), dL/dalpha, the behind-colour update and the opacity
 # and centre terms 25; the nine per-entry words (colour 3, conic 3) 6; their sum over the pixels 9
 BACKWARD_OPS_PER_EVAL = 49
+# the renderer's depth sort (integer keys fl(1 - z) * 2^24 in digits of 7, 8, 8, 2 bits) skips its last two
+# passes when the frame's largest key is below this
+DEPTH_PAIR_SKIP_BELOW = 1 << 15
 
 
 def parse_args(argv=None):
@@ -429,8 +432,12 @@ def main(argv=None) -> int:
     b_emit, b_sort, b_ranges = 12.0 * K, 24.0 * K * passes, 8.0 * K + 8.0 * T
     ps_gbs = (b_proj + b_emit + b_sort + b_ranges) / t_ps / 1e9
     # bytes this design moves by its own algorithm: depth sort over V with 12-B (key, slot | packed tile
-    # rectangle) items, tile sort over K 8-B pairs, emit and ranges
-    own_bytes = b_proj + 96.0 * V + 32.0 * K + 20.0 * K + 8.0 * T
+    # rectangle) items in 2 or 4 passes (the integer depth key's last two passes run only when the frame's
+    # largest key reaches 2^15), tile sort over K 8-B pairs, emit and ranges
+    _, fkeys = r.sorted_ids()
+    k_max = int((fkeys[:V].view(np.float32).astype(np.float64) * 2.0 ** 24).max()) if V else 0
+    depth_passes = 4 if k_max >= DEPTH_PAIR_SKIP_BELOW else 2
+    own_bytes = b_proj + 24.0 * depth_passes * V + 32.0 * K + 20.0 * K + 8.0 * T
     own_gbs = own_bytes / t_ps / 1e9
 
     # the raster kernel this configuration launches (8x8 tiles go two per wave from 8192 tiles up
@@ -486,6 +493,7 @@ def main(argv=None) -> int:
         "projection_sort": {"ms": round(t_ps * 1e3, 4), "bytes": round(b_proj + b_emit + b_sort + b_ranges),
                             "passes": passes, "gbs": round(ps_gbs, 1),
                             "own_bytes": round(own_bytes), "own_gbs": round(own_gbs, 1),
+                            "depth_key_max": k_max, "depth_passes": depth_passes,
                             "work": "12N + 172V + 12K + 24K*P + 8K + 8T (P radix passes of a (tile|depth) key)"},
     }
 
