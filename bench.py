@@ -14,6 +14,7 @@ CPU oracle timed on the host cores as `cpu_baseline`.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -50,6 +51,8 @@ def parse_args(argv=None):
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c3_6m")
+    ap.add_argument("--resolution", default="", metavar="WxH",
+                    help="render the plain-forward workload at this size instead of its own (e.g. 1920x1080)")
     ap.add_argument("--tile", default="auto", help="WxH, or auto (vkgs_renderer_autotune)")
     ap.add_argument("--t-min", type=float, default=1.0 / 8192.0)
     ap.add_argument("--triangles", action="store_true",
@@ -333,6 +336,9 @@ def main(argv=None) -> int:
     from vkgs_amd import scene as S
 
     wl = S.WORKLOADS[args.workload]
+    if args.resolution:
+        rw, rh = (int(v) for v in args.resolution.lower().split("x"))
+        wl = dataclasses.replace(wl, width=rw, height=rh)
     log(f"generating {wl.name}: {wl.n} splats")
     t0 = time.perf_counter()
     sc = wl.make()
