@@ -380,6 +380,38 @@ def test_view_major_tile_keys_match_global_keys():
                 np.testing.assert_array_equal(img, one)
 
 
+@pytest.mark.parametrize("n_views", [128, 129])
+def test_view_major_keys_at_the_view_limit(n_views):
+    """View-major depth keys hold the view index above the 25-bit depth: 128 views fill the 32-bit key
+    (the largest view index in bits 25-31), 129 views do not fit and the batch keeps global tile keys.
+    64x48 views on 8x8 tiles (48 tiles each, 6 bits, one radix pass; 6,144-6,192 tiles together, 13 bits,
+    two passes), so view-major keys are chosen whenever they fit.  Either way the lists are the global
+    keys' lists: images and gradients bit-identical to VKGS_DIFF_GLOBAL_TILE_KEYS, and every image equal
+    to its one-view render."""
+    sc, _ = small_scene(3000, 63, 64, 48)
+    cams = [S.orbit_camera(64, 48, 3.5 + 0.01 * i, phi=0.5 + 0.004 * i, theta=0.049 * i) for i in range(n_views)]
+    views = [(0, c) for c in cams]
+    w = torch.tensor(np.random.default_rng(10).uniform(0.5, 1.5, (n_views * 48 * 64, 4)), dtype=torch.float32,
+                     device="cuda")
+    runs = []
+    for flags in (0, D.GLOBAL_TILE_KEYS):
+        cfg = D._config(t_min=1.0 / 8192.0)
+        cfg.flags |= flags
+        p = D.scene_tensors(sc)
+        flat, spans = D.render_batch_flat([p], views, config=cfg)
+        (flat * w).sum().backward()
+        runs.append((flat.detach().cpu().numpy(), {k: p[k].grad.detach().cpu().numpy() for k in NAMES}, spans))
+    print("views", n_views, "pairs", D.last_info["pairs"], flush=True)
+    (fa, ga, spans), (fb, gb, _) = runs
+    np.testing.assert_array_equal(fa, fb)
+    for k in NAMES:
+        np.testing.assert_array_equal(ga[k], gb[k])
+    p = D.scene_tensors(sc, requires_grad=False)
+    for (o, h, wd), cam in list(zip(spans, cams))[::16] + [(spans[-1], cams[-1])]:
+        one = D.render(**p, camera=cam).cpu().numpy()
+        np.testing.assert_array_equal(fa[o:o + h * wd].reshape(h, wd, 4), one)
+
+
 def cam_t64(cam):
     Pm, Vm, eye = S.camera_arrays(cam)
     return (torch.tensor(Pm.reshape(4, 4).T, dtype=torch.float64),
