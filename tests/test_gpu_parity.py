@@ -287,7 +287,15 @@ def test_autotune_keeps_the_image(c1):
     assert r.tuned.waves_per_cu in (0, 24, 16, 12, 8)
     print("renderer autotune", tw, th, r.tuned.flags, r.tuned.waves_per_cu, f"{ms:.3f} ms", flush=True)
     after = r.render(cam).cpu().numpy()
-    np.testing.assert_array_equal(before, after)  # compositing per pixel does not depend on the tile shape
+    if (tw, th) == DEFAULT_TILE and not (r.tuned.flags & 8):  # VKGS_RASTER_PAIRS_NEVER
+        # the default form (8x8 tiles two per wave at this size), wave caps aside: bit-identical
+        np.testing.assert_array_equal(before, after)
+    else:
+        # another wave layout stops compositing at other points: a pixel may add a few splats past
+        # T < t_min, each weighted by T, so the images differ by less than t_min x max(colour, 1)
+        d = float(np.abs(before - after).max())
+        print("autotuned form differs by", d, flush=True)
+        assert d < (1.0 / 8192.0) * max(1.0, float(before[..., :3].max()))
     r.close()
 
 
