@@ -201,3 +201,33 @@ def test_bench_refuses_multi_gpu():
                          text=True, timeout=120)
     assert out.returncode == 2
     assert "one GPU" in out.stdout
+
+
+def test_integer_depth_key_orders_and_ties_as_the_float_key():
+    """The renderer's depth sort and the batch's view-major keys sort k = fl(1 - z) * 2^24 instead of the
+    float bits of fl(1 - z) (rank.comp:39).  For z in [0, 1] (the visible range) every fl(1 - z) is a
+    multiple of 2^-24 in [0, 1], so k is an exact integer <= 2^24 (25 bits); a stable sort of k gives the
+    float key's order and ties, and k * 2^-24 gives its bits back.  Checked on uniform z, on z crowded
+    near 0 and near 1 (where the spacing of fl(1 - z) changes), on the edge values, and on the oracle's
+    keys of a scene."""
+    import oracle
+
+    rng = np.random.default_rng(3)
+    f32 = np.float32
+    edges = np.array([0.0, -0.0, 1.0, 0.5, 2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26, 1 - 2.0 ** -24,
+                      np.nextafter(f32(0.5), f32(0)), np.nextafter(f32(0.5), f32(1)),
+                      np.nextafter(f32(1), f32(0)), np.nextafter(f32(0), f32(1))], f32)
+    z = np.concatenate([rng.uniform(0, 1, 400_000).astype(f32), (rng.uniform(0, 1, 100_000) ** 12).astype(f32),
+                        (1 - rng.uniform(0, 1, 100_000) ** 12).astype(f32), np.repeat(edges, 3)])
+    one_minus_z = (f32(1) - z).astype(f32)  # fl(1 - z), as the kernels compute it
+    k = (one_minus_z * f32(2 ** 24)).astype(np.uint32)  # depth_rank_key
+    assert np.array_equal(k.astype(np.float64), one_minus_z.astype(np.float64) * 2.0 ** 24)  # exact
+    assert k.max() <= 1 << 24
+    bits = one_minus_z.view(np.uint32)
+    np.testing.assert_array_equal(np.argsort(k, kind="stable"), np.argsort(bits, kind="stable"))
+    np.testing.assert_array_equal((k.astype(f32) * f32(2.0 ** -24)).view(np.uint32), bits)  # depth_key_bits
+    # the oracle's rank keys (float bits of 1 - z of visible splats) of a scene with near and far splats
+    sc = S.gaussian_cloud(50_000, seed=4, pos_std=2.0)
+    _, okey, _ = oracle.rank(S.orbit_camera(640, 480, 3.0), np.ascontiguousarray(sc.rows[:, 0:3], f32))
+    ok = (np.asarray(okey, np.uint32).view(f32) * f32(2 ** 24)).astype(np.uint32)
+    np.testing.assert_array_equal(np.argsort(ok, kind="stable"), np.argsort(okey, kind="stable"))
