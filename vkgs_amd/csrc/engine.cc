@@ -349,12 +349,12 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
   if (r->packed)
     e = launch_fused_write_packed(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
                                   r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(),
-                                  r->key.as<uint32_t>(), r->val64.as<uint64_t>(), r->ids.as<uint32_t>(),
+                                  r->key.as<uint32_t>(), r->val64.as<uint64_t>(), nullptr,
                                   r->counts.as<uint32_t>(), s);
   else
     e = launch_fused_write(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
                            r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(), r->rc.as<uint4>(),
-                           r->key.as<uint32_t>(), r->val.as<uint32_t>(), r->ids.as<uint32_t>(),
+                           r->key.as<uint32_t>(), r->val.as<uint32_t>(), nullptr,
                            r->counts.as<uint32_t>(), s);
   if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[2], s);
@@ -619,6 +619,9 @@ int vkgs_renderer_sorted(vkgs_renderer* r, const uint32_t** d_sorted_ids, const 
   const uint32_t cap = r->frame_n ? r->frame_n : 1;
   VKGS_HIP_CHECK(r->sorted_ids.ensure((size_t)cap * 4));
   uint32_t* misc = r->misc.as<uint32_t>();
+  // the frame writes no slot -> splat id table; rebuild it from its offsets
+  VKGS_HIP_CHECK(launch_compact_ids(r->fc.as<FrameConst>(), r->position.as<float>(), r->frame_n,
+                                    r->counts.as<uint32_t>(), r->ids.as<uint32_t>(), nullptr));
   // sorted slots: the depth sort's values (low words of the packed values)
   if (r->packed)
     VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->val64.as<uint32_t>(), nullptr, misc + kMiscVisible,
@@ -644,7 +647,9 @@ int vkgs_renderer_tile_lists(vkgs_renderer* r, const uint32_t** d_ranges, const 
   const uint64_t cap = r->max_pairs ? r->max_pairs : 1;
   VKGS_HIP_CHECK(r->list_ids.ensure((size_t)cap * 4));
   uint32_t* misc = r->misc.as<uint32_t>();
-  // pairs hold slots; slot -> splat id
+  // pairs hold slots; slot -> splat id (the table rebuilt from the frame's offsets)
+  VKGS_HIP_CHECK(launch_compact_ids(r->fc.as<FrameConst>(), r->position.as<float>(), r->frame_n,
+                                    r->counts.as<uint32_t>(), r->ids.as<uint32_t>(), nullptr));
   VKGS_HIP_CHECK(launch_gather_u32(r->ids.as<uint32_t>(), r->pvals.as<uint32_t>(), nullptr, misc + kMiscPairs,
                                    (uint32_t)cap, r->list_ids.as<uint32_t>(), nullptr));
   VKGS_HIP_CHECK(hipDeviceSynchronize());

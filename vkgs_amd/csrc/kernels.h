@@ -37,9 +37,14 @@ hipError_t launch_projection(const FrameConst& fc, const float* pos, const float
 // including the screen and tile shape), in two launches: visible counts per
 // workgroup + their scan (-> *d_count), then per visible splat at its slot:
 // raster record rec[3 slot..], (tile rect x, tile rect y, tile count, 0) in
-// rc[slot], key = depth_rank_key(1 - z), val = slot, ids[slot] = splat id.
+// rc[slot], key = depth_rank_key(1 - z), val = slot, ids[slot] = splat id
+// (ids optional: launch_compact_ids rebuilds them from the frame's offsets).
 hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t n, uint32_t* d_count,
                               uint32_t* counts, hipStream_t s);
+// ids[slot] = the splat id of each visible slot of the last fused frame
+// (its frame constants d_fc and its scanned counts), rebuilt on request.
+hipError_t launch_compact_ids(const FrameConst* d_fc, const float* pos, uint32_t n, const uint32_t* counts,
+                              uint32_t* ids, hipStream_t s);
 hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
                               const uint16_t* sh, uint32_t n, float4* rec, uint4* rc, uint32_t* key, uint32_t* val,
                               uint32_t* ids, const uint32_t* counts, hipStream_t s);

@@ -242,11 +242,14 @@ __device__ __forceinline__ uint32_t compact_slot(bool vis, const uint32_t* __res
   return before + mbcnt(m);
 }
 
-__global__ __launch_bounds__(kCompactBlock) void rank_write_kernel(FrameConst fc, const float* __restrict__ position,
+// key (optional): the rank key per slot; index: the splat id per slot.
+template <typename FC>
+__global__ __launch_bounds__(kCompactBlock) void rank_write_kernel(FC fcarg, const float* __restrict__ position,
                                                                    uint32_t n, const uint32_t* __restrict__ offsets,
                                                                    uint32_t* __restrict__ key,
                                                                    uint32_t* __restrict__ index) {
   __shared__ uint32_t s_w[kCompactBlock / kWave];
+  const FrameConst& fc = fcref(fcarg);
   const uint32_t id = blockIdx.x * kCompactBlock + threadIdx.x;
   bool vis = false;
   float omz = 0.f;
@@ -255,8 +258,8 @@ __global__ __launch_bounds__(kCompactBlock) void rank_write_kernel(FrameConst fc
                        &omz);
   const uint32_t slot = compact_slot(vis, offsets, s_w);
   if (vis) {
-    key[slot] = __float_as_uint(omz);  // rank.comp:39
-    index[slot] = id;                  // rank.comp:40
+    if (key) key[slot] = __float_as_uint(omz);  // rank.comp:39
+    index[slot] = id;                           // rank.comp:40
   }
 }
 
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
     }
     // k = fl(1 - z) * 2^24: an integer in the order and ties of rank.comp:39's float key
     key[slot] = depth_rank_key(omz);
-    ids[slot] = id;
+    if (ids) ids[slot] = id;
   }
 }
 
@@ -415,7 +418,7 @@ hipError_t launch_rank(const FrameConst& fc, const float* pos, uint32_t n, uint3
   visible_count_kernel<FrameConst><<<nb, kCompactBlock, 0, s>>>(fc, pos, n, counts);
   hipError_t e = compact_scan(counts, nb, d_count, s);
   if (e != hipSuccess) return e;
-  rank_write_kernel<<<nb, kCompactBlock, 0, s>>>(fc, pos, n, counts, key, index);
+  rank_write_kernel<FrameConst><<<nb, kCompactBlock, 0, s>>>(fc, pos, n, counts, key, index);
   return hipGetLastError();
 }
 
@@ -442,6 +445,14 @@ hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t
   const uint32_t nb = ceil_div(n, kCompactBlock);
   visible_count_kernel<const FrameConst*><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, n, counts);
   return compact_scan(counts, nb, d_count, s);
+}
+
+hipError_t launch_compact_ids(const FrameConst* d_fc, const float* pos, uint32_t n, const uint32_t* counts,
+                              uint32_t* ids, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t nb = ceil_div(n, kCompactBlock);
+  rank_write_kernel<const FrameConst*><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, n, counts, nullptr, ids);
+  return hipGetLastError();
 }
 
 hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
