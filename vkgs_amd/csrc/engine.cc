@@ -350,12 +350,12 @@ hipError_t record_frame(vkgs_renderer* r, float* out, hipStream_t s, bool timed)
     e = launch_fused_write_packed(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
                                   r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(),
                                   r->key.as<uint32_t>(), r->val64.as<uint64_t>(), nullptr,
-                                  r->counts.as<uint32_t>(), s);
+                                  r->counts.as<uint32_t>(), misc + kMiscVisible, s);
   else
     e = launch_fused_write(r->fc.as<FrameConst>(), r->position.as<float>(), r->cov3d.as<float>(),
                            r->opacity.as<float>(), r->sh.as<uint16_t>(), n, r->rec.as<float4>(), r->rc.as<uint4>(),
                            r->key.as<uint32_t>(), r->val.as<uint32_t>(), nullptr,
-                           r->counts.as<uint32_t>(), s);
+                           r->counts.as<uint32_t>(), misc + kMiscVisible, s);
   if (e != hipSuccess) return e;
   if (timed) hipEventRecord(r->ev[2], s);
   // keys are k = fl(1 - z) * 2^24 <= 2^24: 25 bits; the sort's first pass

@@ -11,14 +11,18 @@
 namespace vkgs {
 
 constexpr uint32_t kCompactBlock = 256;  // splats per workgroup in rank/fused compaction
+constexpr uint32_t kOffsetGroup = 16;   // compaction workgroups per group sum of the fused count
 
 // parse_ply.comp
 hipError_t launch_parse_ply(const void* d_ply, uint32_t n, float* pos, float* cov3d, float* opacity, uint16_t* sh,
                             hipStream_t s);
 
 // Bytes of the `counts` storage of launch_rank / launch_fused_count: one word
-// per 256 splats plus the partials of their scan.
+// per 256 splats, the partials of their scan, and one group sum per
+// kOffsetGroup words.
 size_t compact_storage_bytes(uint32_t n);
+// The group sums inside counts for n splats (*groups = their number).
+uint32_t* compact_group_sums(uint32_t* counts, uint32_t n, uint32_t* groups);
 
 // rank.comp with ordered compaction; counts is compact_storage_bytes(n) bytes.
 hipError_t launch_rank(const FrameConst& fc, const float* pos, uint32_t n, uint32_t* d_count, uint32_t* key,
@@ -35,7 +39,10 @@ hipError_t launch_projection(const FrameConst& fc, const float* pos, const float
 
 // Fused cull + project + compaction (frame constants read from device memory,
 // including the screen and tile shape), in two launches: visible counts per
-// workgroup + their scan (-> *d_count), then per visible splat at its slot:
+// group of kOffsetGroup workgroups with the offsets inside each group, then
+// per visible splat at its slot (the group sums before its group + its
+// workgroup's offset in the group + its rank; the write's workgroup 0 stores
+// the total in *d_count):
 // raster record rec[3 slot..], (tile rect x, tile rect y, tile count, 0) in
 // rc[slot], key = depth_rank_key(1 - z), val = slot, ids[slot] = splat id
 // (ids optional: launch_compact_ids rebuilds them from the frame's offsets).
@@ -47,14 +54,14 @@ hipError_t launch_compact_ids(const FrameConst* d_fc, const float* pos, uint32_t
                               uint32_t* ids, hipStream_t s);
 hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
                               const uint16_t* sh, uint32_t n, float4* rec, uint4* rc, uint32_t* key, uint32_t* val,
-                              uint32_t* ids, const uint32_t* counts, hipStream_t s);
+                              uint32_t* ids, const uint32_t* counts, uint32_t* d_count, hipStream_t s);
 
 // Same, with the tile rectangle packed into the sort value instead of rc:
 // val64[slot] = slot | packed rect << 32 (frame tiles must fit rect_bits).
 hipError_t launch_fused_write_packed(const FrameConst* d_fc, const float* pos, const float* cov3d,
                                      const float* opacity, const uint16_t* sh, uint32_t n, float4* rec,
                                      uint32_t* key, uint64_t* val64, uint32_t* ids, const uint32_t* counts,
-                                     hipStream_t s);
+                                     uint32_t* d_count, hipStream_t s);
 
 // Copies a FrameConst passed by value into device memory (stream-ordered)
 // and zeroes n_clear words at clear (optional).

@@ -230,6 +230,112 @@ __global__ __launch_bounds__(kCompactBlock) void visible_count_kernel(FC fcarg, 
   }
 }
 
+// The fused path's counts without a scan launch: one workgroup per group of
+// kOffsetGroup compaction workgroups (kOffsetGroup * kCompactBlock splats)
+// writes, for each compaction workgroup b of its group, the visible splats of
+// the group's workgroups before b (counts[b]), and the group's total
+// (group_sums[group]).  Each splat's slot is then the group sums before its
+// group + counts[b] + its rank inside b.
+constexpr uint32_t kGroupThreads = 512;
+constexpr uint32_t kGroupIters = kOffsetGroup * kCompactBlock / kGroupThreads;
+static_assert(kOffsetGroup <= kWave && kGroupThreads % kCompactBlock == 0, "group count layout");
+
+template <typename FC>
+__global__ __launch_bounds__(kGroupThreads) void visible_group_count_kernel(FC fcarg,
+                                                                            const float* __restrict__ position,
+                                                                            uint32_t n, uint32_t nb,
+                                                                            uint32_t* __restrict__ counts,
+                                                                            uint32_t* __restrict__ group_sums) {
+  constexpr uint32_t kWaves = kGroupThreads / kWave;
+  constexpr uint32_t kPerIter = kGroupThreads / kCompactBlock;  // compaction workgroups per iteration
+  constexpr uint32_t kWavesPer = kCompactBlock / kWave;         // waves per compaction workgroup
+  __shared__ uint32_t s_c[kGroupIters * kWaves];                // visible splats of wave w in iteration it
+  const FrameConst& fc = fcref(fcarg);
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t base = blockIdx.x * kOffsetGroup * kCompactBlock + threadIdx.x;
+  float px[kGroupIters], py[kGroupIters], pz[kGroupIters];
+#pragma unroll
+  for (uint32_t it = 0; it < kGroupIters; ++it) {
+    const uint32_t id = base + it * kGroupThreads;
+    const uint32_t j = id < n ? id : 0u;
+    px[it] = position[3 * (size_t)j];
+    py[it] = position[3 * (size_t)j + 1];
+    pz[it] = position[3 * (size_t)j + 2];
+  }
+#pragma unroll
+  for (uint32_t it = 0; it < kGroupIters; ++it) {
+    float omz;
+    const bool vis = base + it * kGroupThreads < n && rank_visible(fc.PVM, px[it], py[it], pz[it], &omz);
+    const uint64_t m = __ballot(vis);
+    if (__lane_id() == 0) s_c[it * kWaves + w] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    const uint32_t j = threadIdx.x;  // compaction workgroup j of the group
+    uint32_t v = 0;
+    if (j < kOffsetGroup) {
+      const uint32_t* c = s_c + (j / kPerIter) * kWaves + (j % kPerIter) * kWavesPer;
+#pragma unroll
+      for (uint32_t i = 0; i < kWavesPer; ++i) v += c[i];
+    }
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)inc, d, kWave);
+      if ((int)j >= d) inc += t;
+    }
+    const uint32_t b = blockIdx.x * kOffsetGroup + j;
+    if (j < kOffsetGroup && b < nb) counts[b] = inc - v;
+    if (j == kOffsetGroup - 1) group_sums[blockIdx.x] = inc;
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v += (uint32_t)__shfl_xor((int)v, d, kWave);
+  return v;
+}
+
+// Slot of this lane's splat if visible, with the workgroup's offset formed
+// from visible_group_count_kernel's output: the group sums before this
+// workgroup's group (summed by the whole workgroup) plus counts[blockIdx.x].
+__device__ __forceinline__ uint32_t compact_slot_grouped(bool vis, const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ group_sums, uint32_t* s_w,
+                                                         uint32_t* s_off) {
+  const uint32_t b = blockIdx.x, g = b / kOffsetGroup;
+  uint32_t acc = threadIdx.x == 0 ? counts[b] : 0u;
+  for (uint32_t i = threadIdx.x; i < g; i += kCompactBlock) acc += group_sums[i];
+  acc = wave_sum_u32(acc);
+  const uint64_t m = __ballot(vis);
+  const uint32_t w = threadIdx.x / kWave;
+  if (__lane_id() == 0) {
+    s_w[w] = (uint32_t)__popcll(m);
+    s_off[w] = acc;
+  }
+  __syncthreads();
+  uint32_t before = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kCompactBlock / kWave; ++i) before += s_off[i];
+  for (uint32_t i = 0; i < w; ++i) before += s_w[i];
+  return before + mbcnt(m);
+}
+
+// The total of the group sums (every visible splat) into *d_count: workgroup
+// 0 of the fused write, which runs after the count kernel.
+__device__ __forceinline__ void write_visible_total(const uint32_t* __restrict__ group_sums, uint32_t groups,
+                                                    uint32_t* s_tot, uint32_t* __restrict__ d_count) {
+  uint32_t acc = 0;
+  for (uint32_t i = threadIdx.x; i < groups; i += kCompactBlock) acc += group_sums[i];
+  acc = wave_sum_u32(acc);
+  if (__lane_id() == 0) s_tot[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t i = 0; i < kCompactBlock / kWave; ++i) t += s_tot[i];
+    *d_count = t;
+  }
+}
+
 // Slot of this lane's splat if visible; offsets[] is the exclusive scan of the
 // per-workgroup counts.
 __device__ __forceinline__ uint32_t compact_slot(bool vis, const uint32_t* __restrict__ offsets, uint32_t* s_w) {
@@ -243,12 +349,16 @@ __device__ __forceinline__ uint32_t compact_slot(bool vis, const uint32_t* __res
 }
 
 // key (optional): the rank key per slot; index: the splat id per slot.
+// group_sums (optional): offsets and group_sums are visible_group_count_kernel's
+// output instead of the scanned offsets.
 template <typename FC>
 __global__ __launch_bounds__(kCompactBlock) void rank_write_kernel(FC fcarg, const float* __restrict__ position,
                                                                    uint32_t n, const uint32_t* __restrict__ offsets,
                                                                    uint32_t* __restrict__ key,
-                                                                   uint32_t* __restrict__ index) {
+                                                                   uint32_t* __restrict__ index,
+                                                                   const uint32_t* __restrict__ group_sums) {
   __shared__ uint32_t s_w[kCompactBlock / kWave];
+  __shared__ uint32_t s_off[kCompactBlock / kWave];
   const FrameConst& fc = fcref(fcarg);
   const uint32_t id = blockIdx.x * kCompactBlock + threadIdx.x;
   bool vis = false;
@@ -256,7 +366,8 @@ __global__ __launch_bounds__(kCompactBlock) void rank_write_kernel(FC fcarg, con
   if (id < n)
     vis = rank_visible(fc.PVM, position[3 * (size_t)id], position[3 * (size_t)id + 1], position[3 * (size_t)id + 2],
                        &omz);
-  const uint32_t slot = compact_slot(vis, offsets, s_w);
+  const uint32_t slot = group_sums ? compact_slot_grouped(vis, offsets, group_sums, s_w, s_off)
+                                   : compact_slot(vis, offsets, s_w);
   if (vis) {
     if (key) key[slot] = __float_as_uint(omz);  // rank.comp:39
     index[slot] = id;                           // rank.comp:40
@@ -277,16 +388,20 @@ __global__ __launch_bounds__(kCompactBlock) void fused_write_kernel(
     const float* __restrict__ opacity, const uint16_t* __restrict__ sh, uint32_t n,
     const uint32_t* __restrict__ offsets, float4* __restrict__ rec, uint4* __restrict__ rc,
     uint32_t* __restrict__ key, uint32_t* __restrict__ val, uint64_t* __restrict__ val64,
-    uint32_t* __restrict__ ids) {
+    uint32_t* __restrict__ ids, const uint32_t* __restrict__ group_sums, uint32_t groups,
+    uint32_t* __restrict__ d_count) {
   __shared__ uint32_t s_w[kCompactBlock / kWave];
+  __shared__ uint32_t s_off[kCompactBlock / kWave];
+  __shared__ uint32_t s_tot[kCompactBlock / kWave];
   const FrameConst& fc = *fcp;
+  if (blockIdx.x == 0) write_visible_total(group_sums, groups, s_tot, d_count);
   const uint32_t id = blockIdx.x * kCompactBlock + threadIdx.x;
   bool vis = false;
   float omz = 0.f;
   if (id < n)
     vis = rank_visible(fc.PVM, position[3 * (size_t)id], position[3 * (size_t)id + 1], position[3 * (size_t)id + 2],
                        &omz);
-  const uint32_t slot = compact_slot(vis, offsets, s_w);
+  const uint32_t slot = compact_slot_grouped(vis, offsets, group_sums, s_w, s_off);
   if (vis) {
     float4 o[3];
     project_splat(fc, position, cov3d, opacity, sh, id, o);
@@ -400,7 +515,15 @@ hipError_t launch_parse_ply(const void* d_ply, uint32_t n, float* pos, float* co
 
 size_t compact_storage_bytes(uint32_t n) {
   const uint32_t nb = ceil_div(n ? n : 1, kCompactBlock);
-  return align_up((size_t)nb * 4, 256) + scan_plan(nb).bytes;
+  return align_up((size_t)nb * 4, 256) + scan_plan(nb).bytes +
+         align_up((size_t)ceil_div(nb, kOffsetGroup) * 4, 256);
+}
+
+uint32_t* compact_group_sums(uint32_t* counts, uint32_t n, uint32_t* groups) {
+  const uint32_t nb = ceil_div(n ? n : 1, kCompactBlock);
+  if (groups) *groups = ceil_div(nb, kOffsetGroup);
+  return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(counts) + align_up((size_t)nb * 4, 256) +
+                                     scan_plan(nb).bytes);
 }
 
 // Exclusive scan of the per-workgroup visible counts in place; the scan's
@@ -418,7 +541,7 @@ hipError_t launch_rank(const FrameConst& fc, const float* pos, uint32_t n, uint3
   visible_count_kernel<FrameConst><<<nb, kCompactBlock, 0, s>>>(fc, pos, n, counts);
   hipError_t e = compact_scan(counts, nb, d_count, s);
   if (e != hipSuccess) return e;
-  rank_write_kernel<FrameConst><<<nb, kCompactBlock, 0, s>>>(fc, pos, n, counts, key, index);
+  rank_write_kernel<FrameConst><<<nb, kCompactBlock, 0, s>>>(fc, pos, n, counts, key, index, nullptr);
   return hipGetLastError();
 }
 
@@ -443,36 +566,43 @@ hipError_t launch_fused_count(const FrameConst* d_fc, const float* pos, uint32_t
                               uint32_t* counts, hipStream_t s) {
   if (n == 0) return fill_u32_launch(d_count, 0u, 1, s);
   const uint32_t nb = ceil_div(n, kCompactBlock);
-  visible_count_kernel<const FrameConst*><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, n, counts);
-  return compact_scan(counts, nb, d_count, s);
+  uint32_t groups;
+  uint32_t* gs = compact_group_sums(counts, n, &groups);
+  visible_group_count_kernel<const FrameConst*><<<groups, kGroupThreads, 0, s>>>(d_fc, pos, n, nb, counts, gs);
+  return hipGetLastError();
 }
 
 hipError_t launch_compact_ids(const FrameConst* d_fc, const float* pos, uint32_t n, const uint32_t* counts,
                               uint32_t* ids, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t nb = ceil_div(n, kCompactBlock);
-  rank_write_kernel<const FrameConst*><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, n, counts, nullptr, ids);
+  const uint32_t* gs = compact_group_sums(const_cast<uint32_t*>(counts), n, nullptr);
+  rank_write_kernel<const FrameConst*><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, n, counts, nullptr, ids, gs);
   return hipGetLastError();
 }
 
 hipError_t launch_fused_write(const FrameConst* d_fc, const float* pos, const float* cov3d, const float* opacity,
                               const uint16_t* sh, uint32_t n, float4* rec, uint4* rc, uint32_t* key, uint32_t* val,
-                              uint32_t* ids, const uint32_t* counts, hipStream_t s) {
+                              uint32_t* ids, const uint32_t* counts, uint32_t* d_count, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t nb = ceil_div(n, kCompactBlock);
+  uint32_t groups;
+  const uint32_t* gs = compact_group_sums(const_cast<uint32_t*>(counts), n, &groups);
   fused_write_kernel<false><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, cov3d, opacity, sh, n, counts, rec, rc, key, val,
-                                                         nullptr, ids);
+                                                         nullptr, ids, gs, groups, d_count);
   return hipGetLastError();
 }
 
 hipError_t launch_fused_write_packed(const FrameConst* d_fc, const float* pos, const float* cov3d,
                                      const float* opacity, const uint16_t* sh, uint32_t n, float4* rec,
                                      uint32_t* key, uint64_t* val64, uint32_t* ids, const uint32_t* counts,
-                                     hipStream_t s) {
+                                     uint32_t* d_count, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t nb = ceil_div(n, kCompactBlock);
+  uint32_t groups;
+  const uint32_t* gs = compact_group_sums(const_cast<uint32_t*>(counts), n, &groups);
   fused_write_kernel<true><<<nb, kCompactBlock, 0, s>>>(d_fc, pos, cov3d, opacity, sh, n, counts, rec, nullptr, key,
-                                                        nullptr, val64, ids);
+                                                        nullptr, val64, ids, gs, groups, d_count);
   return hipGetLastError();
 }
 
