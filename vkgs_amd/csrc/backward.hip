@@ -1222,129 +1222,96 @@ __device__ __forceinline__ float wave_total_last(float x) {
   return x;
 }
 
-// g9[slot] = sum of the partials of the slot's pairs, which sit at
-// soffs[slot] .. soffs[slot] + scnt[slot] - 1 (contiguous, slots in order, so
-// reads and writes both stream).  A lane sums a run of up to kRunSerial pairs
-// itself (from coalesced LDS-staged chunks); longer runs are summed by the
-// whole wave.  Every visible slot is written once (zeros for a splat without
-// pairs), so g9 needs no zero fill.
-__global__ __launch_bounds__(256) void diff_grad_reduce_kernel(const uint32_t* __restrict__ soffs,
-                                                               const uint32_t* __restrict__ scnt,
-                                                               const float* __restrict__ gpart,
-                                                               const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                               uint64_t max_pairs, float* __restrict__ g9) {
-  __shared__ float s_stage[256 / kWave][9 * kWave];  // per wave: the partials of 64 consecutive pairs
-  const uint32_t n = min(*d_count, max_n);
+// Sum of the partials of one slot's pairs, which sit at o .. o + c - 1 of
+// gpart (9 words each; a slot's pairs are numbered contiguously in slot
+// order), for every lane of a wave at once; lanes whose slots are consecutive
+// have adjacent runs, which together span [r0, r1) (r0 even).  Runs of up to
+// kRunSerial pairs: the span is staged through LDS 64 pairs at a time with
+// coalesced loads (lane i reads words i, i + 64, ...) and each lane adds the
+// part of its run in the chunk (chunks that only long runs need are skipped).
+// Longer runs are summed by the whole wave.  Chunks start at even pair
+// indices, so the pairs summed two at a time are (even q, q + 1): a slot's sum
+// depends on its run alone, whatever the wave's other runs.
+__device__ __forceinline__ void sum_slot_runs(const float* __restrict__ gpart, uint64_t o, uint32_t c, uint64_t r0,
+                                              uint64_t r1, float* stage, float acc[9]) {
   const uint32_t lane = __lane_id();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  float* stage = s_stage[threadIdx.x / kWave];
-  for (uint32_t base = wave * kWave; base < n; base += stride) {  // wave-uniform
-    const uint32_t k = base + lane;
-    uint32_t c = 0, slot = k;
-    uint64_t o = 0;
-    if (k < n) {
-      o = soffs[k];
-      c = scnt[k];
-      if (o + c > max_pairs) c = o < max_pairs ? (uint32_t)(max_pairs - o) : 0u;
+  const bool big = c > kRunSerial;
+  const uint64_t e = o + c;
+#pragma unroll
+  for (int w = 0; w < 9; ++w) acc[w] = 0.f;
+  // the next needed chunk's words are loaded while the current one is
+  // summed, so a wave has one chunk of loads in flight behind its adds
+  auto needed = [&](uint64_t cs) {
+    const uint64_t lo = o > cs ? o : cs, hi = e < cs + kWave ? e : cs + kWave;
+    return __ballot(!big && lo < hi) != 0;
+  };
+  auto load_chunk = [&](uint64_t cs, float* buf) {
+    const uint32_t words = 9u * (uint32_t)min((uint64_t)kWave, r1 - cs);
+    const float* src = gpart + 9 * cs;
+#pragma unroll
+    for (uint32_t q = 0; q < 9; ++q) {
+      const uint32_t i = lane + q * kWave;
+      buf[q] = i < words ? src[i] : 0.f;
     }
-    const bool big = c > kRunSerial;
-    // runs of up to kRunSerial pairs: the wave's runs are contiguous, [r0,
-    // r1); it is staged through LDS 64 pairs at a time with coalesced loads
-    // (lane i reads words i, i + 64, ...) and each lane adds the part of its
-    // run in the chunk.  Chunks that only long runs need are skipped.
-    {
-      const uint32_t last = __builtin_amdgcn_readfirstlane(min(n - base, (uint32_t)kWave) - 1);
-      // chunks start at an even pair index, so the pairs summed two at a time
-      // below are (even q, q + 1) whatever the wave's first slot: the sum's
-      // order depends on the slot's run alone
-      const uint64_t r0 = (((uint64_t)lane_u32((uint32_t)(o >> 32), 0) << 32) | lane_u32((uint32_t)o, 0)) & ~1ull;
-      const uint64_t e = o + c;
-      uint64_t r1 = ((uint64_t)lane_u32((uint32_t)(e >> 32), last) << 32) | lane_u32((uint32_t)e, last);
-      r1 = r1 < max_pairs ? r1 : max_pairs;  // runs clipped at max_pairs may start beyond it
-      float acc[9] = {};
-      // the next needed chunk's words are loaded while the current one is
-      // summed, so a wave has one chunk of loads in flight behind its adds
-      auto needed = [&](uint64_t cs) {
-        const uint64_t lo = o > cs ? o : cs, hi = e < cs + kWave ? e : cs + kWave;
-        return __ballot(!big && lo < hi) != 0;
-      };
-      auto load_chunk = [&](uint64_t cs, float* buf) {
-        const uint32_t words = 9u * (uint32_t)min((uint64_t)kWave, r1 - cs);
-        const float* src = gpart + 9 * cs;
+  };
+  uint64_t cs = r0;
+  while (cs < r1 && !needed(cs)) cs += kWave;
+  float cur[9];
+  if (cs < r1) load_chunk(cs, cur);
+  while (cs < r1) {
+    uint64_t nx = cs + kWave;
+    while (nx < r1 && !needed(nx)) nx += kWave;
+    const uint32_t words = 9u * (uint32_t)min((uint64_t)kWave, r1 - cs);
 #pragma unroll
-        for (uint32_t q = 0; q < 9; ++q) {
-          const uint32_t i = lane + q * kWave;
-          buf[q] = i < words ? src[i] : 0.f;
-        }
-      };
-      uint64_t cs = r0;
-      while (cs < r1 && !needed(cs)) cs += kWave;
-      float cur[9];
-      if (cs < r1) load_chunk(cs, cur);
-      while (cs < r1) {
-        uint64_t nx = cs + kWave;
-        while (nx < r1 && !needed(nx)) nx += kWave;
-        const uint32_t words = 9u * (uint32_t)min((uint64_t)kWave, r1 - cs);
+    for (uint32_t q = 0; q < 9; ++q) {
+      const uint32_t i = lane + q * kWave;
+      if (i < words) stage[i] = cur[q];
+    }
+    if (nx < r1) load_chunk(nx, cur);
+    __builtin_amdgcn_wave_barrier();  // a wave's LDS operations complete in order
+    const uint64_t lo = o > cs ? o : cs, hi = e < cs + kWave ? e : cs + kWave;
+    if (!big && lo < hi) {
+      uint32_t p = (uint32_t)(lo - cs);
+      const uint32_t pe = (uint32_t)(hi - cs);
+      if (p & 1u) {  // an odd first pair alone (cs is even)
+        const float* v = stage + 9 * p;
 #pragma unroll
-        for (uint32_t q = 0; q < 9; ++q) {
-          const uint32_t i = lane + q * kWave;
-          if (i < words) stage[i] = cur[q];
-        }
-        if (nx < r1) load_chunk(nx, cur);
-        __builtin_amdgcn_wave_barrier();  // a wave's LDS operations complete in order
-        const uint64_t lo = o > cs ? o : cs, hi = e < cs + kWave ? e : cs + kWave;
-        if (!big && lo < hi) {
-          uint32_t p = (uint32_t)(lo - cs);
-          const uint32_t pe = (uint32_t)(hi - cs);
-          if (p & 1u) {  // an odd first pair alone (cs is even)
-            const float* v = stage + 9 * p;
-#pragma unroll
-            for (int w = 0; w < 9; ++w) acc[w] += v[w];
-            ++p;
-          }
-          for (; p + 2 <= pe; p += 2) {  // two pairs per trip: 18 independent LDS reads in flight
-            const float* v = stage + 9 * p;
-            float x[18];
-#pragma unroll
-            for (int w = 0; w < 18; ++w) x[w] = v[w];
-#pragma unroll
-            for (int w = 0; w < 9; ++w) acc[w] += x[w] + x[9 + w];
-          }
-          if (p < pe) {
-            const float* v = stage + 9 * p;
-#pragma unroll
-            for (int w = 0; w < 9; ++w) acc[w] += v[w];
-          }
-        }
-        __builtin_amdgcn_wave_barrier();
-        cs = nx;
+        for (int w = 0; w < 9; ++w) acc[w] += v[w];
+        ++p;
       }
-      if (k < n && !big) {
-        float* dst = g9 + 9 * (size_t)slot;
+      for (; p + 2 <= pe; p += 2) {  // two pairs per trip: 18 independent LDS reads in flight
+        const float* v = stage + 9 * p;
+        float x[18];
 #pragma unroll
-        for (int w = 0; w < 9; ++w) dst[w] = acc[w];
+        for (int w = 0; w < 18; ++w) x[w] = v[w];
+#pragma unroll
+        for (int w = 0; w < 9; ++w) acc[w] += x[w] + x[9 + w];
+      }
+      if (p < pe) {
+        const float* v = stage + 9 * p;
+#pragma unroll
+        for (int w = 0; w < 9; ++w) acc[w] += v[w];
       }
     }
-    uint64_t pending = __ballot(big);
-    while (pending) {
-      const uint32_t src_lane = (uint32_t)__builtin_ctzll(pending);
-      pending &= pending - 1;
-      const uint32_t bc = lane_u32(c, src_lane), bslot = lane_u32(slot, src_lane);
-      const uint64_t bo = ((uint64_t)lane_u32((uint32_t)(o >> 32), src_lane) << 32) | lane_u32((uint32_t)o, src_lane);
-      float acc[9] = {};
-      for (uint32_t i = lane; i < bc; i += kWave) {
-        const float* src = gpart + 9 * (bo + i);
+    __builtin_amdgcn_wave_barrier();
+    cs = nx;
+  }
+  uint64_t pending = __ballot(big);
+  while (pending) {
+    const uint32_t src_lane = (uint32_t)__builtin_ctzll(pending);
+    pending &= pending - 1;
+    const uint32_t bc = lane_u32(c, src_lane);
+    const uint64_t bo = ((uint64_t)lane_u32((uint32_t)(o >> 32), src_lane) << 32) | lane_u32((uint32_t)o, src_lane);
+    float t[9] = {};
+    for (uint32_t i = lane; i < bc; i += kWave) {
+      const float* src = gpart + 9 * (bo + i);
 #pragma unroll
-        for (int w = 0; w < 9; ++w) acc[w] += src[w];
-      }
+      for (int w = 0; w < 9; ++w) t[w] += src[w];
+    }
 #pragma unroll
-      for (int w = 0; w < 9; ++w) acc[w] = wave_total_last(acc[w]);
-      if (lane == kWave - 1) {
-        float* dst = g9 + 9 * (size_t)bslot;
-#pragma unroll
-        for (int w = 0; w < 9; ++w) dst[w] = acc[w];
-      }
+    for (int w = 0; w < 9; ++w) {
+      const float tot = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_total_last(t[w])), kWave - 1));
+      if (lane == src_lane) acc[w] = tot;
     }
   }
 }
@@ -1539,14 +1506,19 @@ __device__ __forceinline__ void chain_view(const DiffView& view, const SplatStat
   }
 }
 
-// One lane per scene splat: visits every view of the scene, reads the view's
+// One lane per scene splat: visits every view of the scene, sums the view's
 // 9 gradient words at the splat's slot (found from the forward's visibility
-// ballots and per-wave offsets) and chains them; then writes all 59 parameter
-// gradients of the splat once (zeros for a splat no view touched).
-__global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(DiffLaunch L,
-                                                                              const uint32_t* __restrict__ offsets,
-                                                                              const uint64_t* __restrict__ masks,
-                                                                              const float* __restrict__ g9) {
+// ballots and per-wave offsets) from the partials of the slot's pairs and
+// chains them; then writes all 59 parameter gradients of the splat once
+// (zeros for a splat no view touched).  The slots of a wave's visible lanes
+// in one view are consecutive, so their pair runs are adjacent and the wave
+// streams them (sum_slot_runs).
+__global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(
+    DiffLaunch L, const uint32_t* __restrict__ offsets, const uint64_t* __restrict__ masks,
+    const uint32_t* __restrict__ soffs, const uint32_t* __restrict__ scnt, const float* __restrict__ gpart,
+    uint64_t max_pairs) {
+  __shared__ float s_stage[kCompactBlock / kWave][9 * kWave];  // per wave: the partials of 64 consecutive pairs
+  float* stage = s_stage[threadIdx.x / kWave];
   const uint32_t sb = blockIdx.x;
   const DiffScene& sc = L.scenes[range_of(L.scene_blocks, L.n_scenes, sb)];
   const uint32_t lb = sb - sc.block_base;
@@ -1594,23 +1566,70 @@ __global__ __launch_bounds__(kCompactBlock) void diff_project_backward_kernel(Di
   float GSig[9] = {}, gm[3] = {}, gsh[48] = {};
   float gop = 0.f;
   bool any = false;
-  for (uint32_t vi = sc.view_begin; vi < sc.view_end; ++vi) {
-    const DiffView& view = L.views[L.scene_views[vi]];
-    const size_t wi = (size_t)(view.block_base + lb) * kMaskWords + w;
-    const uint64_t m = masks[wi];
-    if (m == 0) continue;
-    if (!((m >> lane) & 1ull)) continue;
-    const float* gk = g9 + 9 * (size_t)(offsets[wi] + mbcnt(m));
-    float gv[9];
-    bool nz = false;
-    for (int j = 0; j < 9; ++j) {
-      gv[j] = ld_global(gk + j);
-      nz = nz || gv[j] != 0.f;
+  // The scene's views 64 at a time: lane j reads view j's visibility ballot
+  // and compaction offset for this wave at once; the views this wave has
+  // visible splats in are then visited in order, the next one's run bounds
+  // (soffs, scnt) loaded while the current one is summed and chained.
+  for (uint32_t g0 = sc.view_begin; g0 < sc.view_end; g0 += kWave) {
+    uint64_t vm_lane = 0;
+    uint32_t voff_lane = 0;
+    if (g0 + lane < sc.view_end) {
+      const size_t wi = (size_t)(L.views[L.scene_views[g0 + lane]].block_base + lb) * kMaskWords + w;
+      vm_lane = masks[wi];
+      voff_lane = offsets[wi];
     }
-    if (!nz) continue;
-    any = true;
-    gop += gv[3];
-    chain_view(view, P, gv, GSig, gm, gsh);
+    uint64_t todo = __ballot(vm_lane != 0);
+    if (todo == 0) continue;
+    // run bounds of this lane's slot in view j of the group (c = 0 where not visible)
+    auto bounds = [&](uint32_t j, uint64_t& m, uint64_t& o, uint32_t& c) {
+      m = ((uint64_t)lane_u32((uint32_t)(vm_lane >> 32), j) << 32) | lane_u32((uint32_t)vm_lane, j);
+      const uint32_t base = lane_u32(voff_lane, j);
+      o = 0;
+      c = 0;
+      if ((m >> lane) & 1ull) {
+        const uint32_t slot = base + mbcnt(m);
+        o = soffs[slot];
+        c = scnt[slot];
+      }
+    };
+    uint32_t j = (uint32_t)__builtin_ctzll(todo);
+    todo &= todo - 1;
+    uint64_t m, o;
+    uint32_t c;
+    bounds(j, m, o, c);
+    while (true) {
+      const bool more = todo != 0;
+      const uint32_t jn = more ? (uint32_t)__builtin_ctzll(todo) : 0u;
+      uint64_t mn = 0, on = 0;
+      uint32_t cn = 0;
+      if (more) {
+        todo &= todo - 1;
+        bounds(jn, mn, on, cn);
+      }
+      const bool vis = (m >> lane) & 1ull;
+      if (o + c > max_pairs) c = o < max_pairs ? (uint32_t)(max_pairs - o) : 0u;
+      // the span of the wave's runs: from its first visible lane's run start
+      // (rounded down to even) to its last visible lane's run end
+      const uint32_t first = (uint32_t)__builtin_ctzll(m), lastl = 63u - (uint32_t)__builtin_clzll(m);
+      const uint64_t r0 = (((uint64_t)lane_u32((uint32_t)(o >> 32), first) << 32) | lane_u32((uint32_t)o, first)) & ~1ull;
+      const uint64_t e = o + c;
+      uint64_t r1 = ((uint64_t)lane_u32((uint32_t)(e >> 32), lastl) << 32) | lane_u32((uint32_t)e, lastl);
+      r1 = r1 < max_pairs ? r1 : max_pairs;  // runs clipped at max_pairs may start beyond it
+      float gv[9];
+      sum_slot_runs(gpart, o, c, r0, r1, stage, gv);
+      bool nz = false;
+      for (int k = 0; k < 9; ++k) nz = nz || gv[k] != 0.f;
+      if (vis && nz) {
+        any = true;
+        gop += gv[3];
+        chain_view(L.views[L.scene_views[g0 + j]], P, gv, GSig, gm, gsh);
+      }
+      if (!more) break;
+      j = jn;
+      m = mn;
+      o = on;
+      c = cn;
+    }
   }
   if (!valid) return;
 
@@ -1789,9 +1808,7 @@ void launch_backward_8x8(const DiffLaunch& L, const float4* rec, const uint32_t*
 
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
-                                       uint64_t pairs, const uint32_t* soffs, const uint32_t* scnt,
-                                       const uint32_t* d_count, uint32_t max_n, float* gpart, float* g9,
-                                       hipStream_t s, hipEvent_t mid) {
+                                       float* gpart, hipStream_t s) {
   // partials of pairs no pixel reached stay zero
   if (L.n_tiles > 0) {
     bool launched = false;
@@ -1816,20 +1833,15 @@ hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, c
 #undef VKGS_CASE
     if (!launched) return hipErrorInvalidValue;
   }
-  if (mid) {
-    const hipError_t e = hipEventRecord(mid, s);
-    if (e != hipSuccess) return e;
-  }
-  if (max_n > 0)
-    diff_grad_reduce_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(soffs, scnt, gpart, d_count, max_n, pairs,
-                                                                       g9);
   return hipGetLastError();
 }
 
 hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
-                                        const float* g9, hipStream_t s) {
+                                        const uint32_t* soffs, const uint32_t* scnt, const float* gpart,
+                                        uint64_t max_pairs, hipStream_t s) {
   if (L.n_scene_blocks == 0) return hipSuccess;
-  diff_project_backward_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, g9);
+  diff_project_backward_kernel<<<L.n_scene_blocks, kCompactBlock, 0, s>>>(L, offsets, masks, soffs, scnt, gpart,
+                                                                          max_pairs);
   return hipGetLastError();
 }
 

@@ -53,7 +53,7 @@ struct vkgs_diff_ctx {
   Buf tables;
   // per-item buffers
   uint32_t cap_items = 0;
-  Buf counts, count_scan, rec, rc, key, val, val64, vslot, masks, kview, g9, misc, rect, cnt, offs, scnt, soffs,
+  Buf counts, count_scan, rec, rc, key, val, val64, vslot, masks, kview, misc, rect, cnt, offs, scnt, soffs,
       scan_storage, depth_sort;
   // pairs
   uint64_t max_pairs = 0;
@@ -164,7 +164,7 @@ int vkgs_diff_destroy(vkgs_diff_ctx* c) {
   if (!c) return VKGS_ERR_INVALID_ARG;
   (void)hipDeviceSynchronize();
   Buf* bufs[] = {&c->tables,     &c->counts,   &c->count_scan,   &c->rec,        &c->rc,         &c->key,
-                 &c->val,        &c->masks,    &c->kview,        &c->g9,         &c->misc,       &c->rect,
+                 &c->val,        &c->masks,    &c->kview,        &c->misc,       &c->rect,
                  &c->cnt,        &c->offs,     &c->scnt,         &c->soffs,      &c->scan_storage, &c->depth_sort,
                  &c->pkeys,      &c->pvals,    &c->pair_sort,    &c->ranges,     &c->final_t,    &c->last,
                  &c->tile_keys,  &c->tile_order, &c->tile_sort,  &c->pslot,      &c->gpart,
@@ -300,7 +300,6 @@ int vkgs_diff_forward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t 
   }
   ENSURE(key, (size_t)cap * 4);
   ENSURE(masks, (size_t)(blocks ? blocks : 1) * kMaskWords * 8);
-  ENSURE(g9, (size_t)cap * 36);
   const void* old_misc = c->misc.p;
   ENSURE(misc, 256);
   if (e == hipSuccess && c->misc.p != old_misc) e = fill_u32_launch(c->misc.as<uint32_t>(), 0u, 64, s);
@@ -508,18 +507,19 @@ int vkgs_diff_backward(vkgs_diff_ctx* c, const vkgs_diff_scene* scenes, uint32_t
   VKGS_HIP_CHECK(upload_tables(c, s));
   DiffLaunch L = c->launch;
   L.work = timed ? c->work_b.as<uint32_t>() : nullptr;
-  uint32_t* misc = c->misc.as<uint32_t>();
   // per-pair partials: gpart holds max_pairs, the bound of the live count the
   // kernels read on the device (no host read-back, so a graph can capture this)
   VKGS_HIP_CHECK(launch_diff_raster_backward(L, c->rec.as<float4>(), c->pvals.as<uint32_t>(),
                                              c->ranges.as<uint32_t>(), c->pslot.as<uint32_t>(),
-                                             c->tile_order.as<uint32_t>(), c->max_pairs, c->soffs.as<uint32_t>(),
-                                             c->scnt.as<uint32_t>(), misc + kMiscVisible, c->cap_items,
-                                             c->gpart.as<float>(), c->g9.as<float>(), s, timed ? c->ev[4] : nullptr));
+                                             c->tile_order.as<uint32_t>(), c->gpart.as<float>(), s));
+  // the per-slot reduction of the partials runs inside the projection
+  // backward: its stage time (ev[4] .. ev[5]) is empty
+  if (timed) VKGS_HIP_CHECK(hipEventRecord(c->ev[4], s));
   if (timed) VKGS_HIP_CHECK(hipEventRecord(c->ev[5], s));
   L.work = nullptr;
   VKGS_HIP_CHECK(launch_diff_project_backward(L, c->counts.as<uint32_t>(), c->masks.as<uint64_t>(),
-                                              c->g9.as<float>(), s));
+                                              c->soffs.as<uint32_t>(), c->scnt.as<uint32_t>(),
+                                              c->gpart.as<float>(), c->max_pairs, s));
   if (timed) VKGS_HIP_CHECK(hipEventRecord(c->ev[6], s));
   return VKGS_OK;
 }

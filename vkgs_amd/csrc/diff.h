@@ -141,18 +141,17 @@ hipError_t launch_diff_raster(const DiffLaunch& L, const float4* rec, const uint
                               const uint32_t* pslot, const uint32_t* tile_order, bool keep_state, hipStream_t s);
 // Blend backward: per sorted pair 9 partial gradient words written at the
 // pair's q (its slot-order index, the sorted value) in gpart (9 floats per
-// pair, pairs = live pair count; every pair written once, zeros for entries
-// no pixel reached), then g9[slot] = the sum over
-// the slot's pairs soffs[slot] .. soffs[slot] + scnt[slot] - 1 for slots <
-// *d_count.  No global atomics.
+// pair; every live pair written once, zeros for entries no pixel reached).
+// No global atomics.
 hipError_t launch_diff_raster_backward(const DiffLaunch& L, const float4* rec, const uint32_t* vals,
                                        const uint32_t* ranges, const uint32_t* pslot, const uint32_t* tile_order,
-                                       uint64_t pairs, const uint32_t* soffs, const uint32_t* scnt,
-                                       const uint32_t* d_count, uint32_t max_n, float* gpart, float* g9,
-                                       hipStream_t s, hipEvent_t mid = nullptr);
-// (mid, optional: recorded between the raster backward and the reduction.)
-// Writes (overwrites) every parameter gradient of every scene.
+                                       float* gpart, hipStream_t s);
+// Projection backward: a slot's 9 gradient words are the sum of its pairs'
+// partials soffs[slot] .. soffs[slot] + scnt[slot] - 1 (clipped at
+// max_pairs), chained through each view's projection.  Writes (overwrites)
+// every parameter gradient of every scene.
 hipError_t launch_diff_project_backward(const DiffLaunch& L, const uint32_t* offsets, const uint64_t* masks,
-                                        const float* g9, hipStream_t s);
+                                        const uint32_t* soffs, const uint32_t* scnt, const float* gpart,
+                                        uint64_t max_pairs, hipStream_t s);
 
 }  // namespace vkgs

@@ -364,6 +364,7 @@ class GraphedStep:
         check(_L().vkgs_diff_check(self.ctx.h, None, _stream()), "vkgs_diff_check")
 
 
+# backward_reduce is empty: the per-slot sum of the pair partials runs inside the projection backward
 DIFF_STAGES = ("forward_prep", "forward_raster", "forward_total", "backward_raster", "backward_reduce",
                "backward_project", "backward_total")
 
