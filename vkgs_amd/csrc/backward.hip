@@ -572,6 +572,22 @@ __global__ __launch_bounds__(256) void diff_bin_emit_packed_kernel(DiffLaunch L,
 // below it stays below); while it does, last = idx, so a pixel that stops
 // keeps the entry that stopped it.  The state is re-derived from T on every
 // step rather than carried as a flag, which keeps it in a lane mask.
+__device__ __forceinline__ void diff_step_at(float ux, float uy, const float4& a, const float4& c, uint32_t idx,
+                                             float t_min, float& T, float& cr, float& cg, float& cb, float& aa,
+                                             uint32_t& last) {
+  const bool alive = T >= t_min;
+  const float m = fmaxf(fabsf(ux), fabsf(uy));
+  const float e = __builtin_amdgcn_exp2f(-fmaf(ux, ux, fmaf(uy, uy, a.w)));
+  const float alpha = ((m <= kCoverScaled) & alive) ? fminf(e, kAlphaMax) : 0.f;
+  last = alive ? idx : last;
+  const float ta = T * alpha;
+  cr = fmaf(c.x, ta, cr);
+  cg = fmaf(c.y, ta, cg);
+  cb = fmaf(c.z, ta, cb);
+  aa = fmaf(alpha, ta, aa);
+  T = T - ta;
+}
+
 __device__ __forceinline__ void diff_step(const float4& a, const float4& b, const float4& c, float lx, float ly,
                                           uint32_t idx, float t_min, float& T, float& cr, float& cg, float& cb,
                                           float& aa, uint32_t& last) {
@@ -918,10 +934,11 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_backward_kernel(DiffLaunch
 //
 // The 8x8 forward and backward rasters also run with two tiles per wave:
 // lanes 0-31 take tile order[2i], lanes 32-63 tile order[2i + 1], each lane
-// two pixels of its tile (rows r and r + 4).  A record read from LDS then
-// serves two pixels, and in the backward each of the nine per-entry sums
-// first adds the lane's two pixels and then runs the same three DPP steps
-// over half rows of 8 lanes, so the reduction costs half as much per pixel.
+// two pixels of one row of its tile (columns c and c + 4, which share the
+// row's term of each frame coordinate).  A record read from LDS then serves
+// two pixels, and in the backward each of the nine per-entry sums first adds
+// the lane's two pixels and then runs the same three DPP steps over groups of
+// 8 lanes (two pixel rows), so the reduction costs half as much per pixel.
 // Consecutive tiles of the longest-first schedule have list lengths within a
 // bucket of each other; a half whose list is shorter walks records that
 // touch none of its pixels.  The per-pixel arithmetic is that of the one-tile
@@ -955,11 +972,13 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
   const uint32_t tile = gtile - view.tile_base;
   const uint32_t W = view.fc.width, H = view.fc.height;
   const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
-  const int px = (int)tx * 8 + l % 8, py0 = (int)ty * 8 + l / 8, py1 = py0 + 4;
-  const bool in0 = has_tile && px < (int)W && py0 < (int)H, in1 = has_tile && px < (int)W && py1 < (int)H;
+  // a lane's two pixels: columns c and c + 4 of one row, sharing the row's
+  // term of each frame coordinate (each pixel's arithmetic is diff_step's)
+  const int px0 = (int)tx * 8 + l % 4, px1 = px0 + 4, py = (int)ty * 8 + l / 4;
+  const bool in0 = has_tile && px0 < (int)W && py < (int)H, in1 = has_tile && px1 < (int)W && py < (int)H;
   const int bx0 = ((int)tx * 8) & ~(kRasterBlock - 1), by0 = ((int)ty * 8) & ~(kRasterBlock - 1);
   const float X0 = (float)bx0, Y0 = (float)by0;
-  const float lx = (float)(px - bx0) + 0.5f, ly0 = (float)(py0 - by0) + 0.5f, ly1 = ly0 + 4.f;
+  const float lx0 = (float)(px0 - bx0) + 0.5f, lx1 = lx0 + 4.f, ly = (float)(py - by0) + 0.5f;
   uint32_t end = 0, items = 0;
   if (has_tile) {
     end = ranges[2 * gtile + 1];
@@ -1007,8 +1026,9 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
     for (; e < cnt; ++e) {
       const float4 a = s0[e], bb = s1[e], c = s2[e];
       const uint32_t idx = done + e + 1;
-      diff_step(a, bb, c, lx, ly0, idx, t_min, T0, r0, g0, b0, a0, last0);
-      diff_step(a, bb, c, lx, ly1, idx, t_min, T1, r1, g1, b1, a1, last1);
+      const float jx = fmaf(a.y, ly, a.z), jy = fmaf(bb.y, ly, bb.z);
+      diff_step_at(fmaf(a.x, lx0, jx), fmaf(bb.x, lx0, jy), a, c, idx, t_min, T0, r0, g0, b0, a0, last0);
+      diff_step_at(fmaf(a.x, lx1, jx), fmaf(bb.x, lx1, jy), a, c, idx, t_min, T1, r1, g1, b1, a1, last1);
       if (__ballot((T0 >= t_min) | (T1 >= t_min)) == 0) {
         ++e;
         break;
@@ -1022,7 +1042,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
   if (T0 >= t_min) last0 = items;
   if (T1 >= t_min) last1 = items;
   if (in0) {
-    const size_t pix = (size_t)py0 * W + px;
+    const size_t pix = (size_t)py * W + px0;
     view.out[pix] = make_float4(r0, g0, b0, a0 + T0);
     if (keep_state) {
       view.final_t[pix] = T0;
@@ -1030,7 +1050,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
     }
   }
   if (in1) {
-    const size_t pix = (size_t)py1 * W + px;
+    const size_t pix = (size_t)py * W + px1;
     view.out[pix] = make_float4(r1, g1, b1, a1 + T1);
     if (keep_state) {
       view.final_t[pix] = T1;
@@ -1054,11 +1074,12 @@ struct PixCover {
   bool in;
 };
 
-__device__ __forceinline__ PixCover bwd_cover(const PixState& ps, const float4& a, const float4& bb, float lx, float ly,
-                                              uint32_t i) {
+// (ux, uy) = the pixel's frame coordinates, formed as diff_step forms them:
+// fma(a.x, x, fma(a.y, y, a.z)) and the same with bb.
+__device__ __forceinline__ PixCover bwd_cover(const PixState& ps, float ux, float uy, uint32_t i) {
   PixCover pc;
-  pc.ux = fmaf(a.x, lx, fmaf(a.y, ly, a.z));
-  pc.uy = fmaf(bb.x, lx, fmaf(bb.y, ly, bb.z));
+  pc.ux = ux;
+  pc.uy = uy;
   pc.in = (i < ps.last) & (fmaxf(fabsf(pc.ux), fabsf(pc.uy)) <= kCoverScaled);
   return pc;
 }
@@ -1094,7 +1115,7 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
     const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ pslot, const uint32_t* __restrict__ tile_order,
     float* __restrict__ gpart) {
   constexpr int kB = KB;  // list entries per batch of each tile
-  constexpr int kSlots = kPairHalf / 8;  // half rows of 8 lanes per tile
+  constexpr int kSlots = kPairHalf / 8;  // groups of 8 lanes (two pixel rows) per tile
   __shared__ float4 s_rec[2][3][kB];
   __shared__ float s_g[2][9][kSlots][kB + 1];
   __shared__ uint32_t s_j[2][kB];
@@ -1107,28 +1128,29 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
   const uint32_t tile = gtile - view.tile_base;
   const uint32_t W = view.fc.width, H = view.fc.height;
   const uint32_t tx = tile % view.fc.tiles_x, ty = tile / view.fc.tiles_x;
-  const int px = (int)tx * 8 + l % 8, py0 = (int)ty * 8 + l / 8, py1 = py0 + 4;
-  const bool in0 = has_tile && px < (int)W && py0 < (int)H, in1 = has_tile && px < (int)W && py1 < (int)H;
+  // the forward pair kernel's pixels: columns c and c + 4 of one row
+  const int px0 = (int)tx * 8 + l % 4, px1 = px0 + 4, py = (int)ty * 8 + l / 4;
+  const bool in0 = has_tile && px0 < (int)W && py < (int)H, in1 = has_tile && px1 < (int)W && py < (int)H;
   const int bx0 = ((int)tx * 8) & ~(kRasterBlock - 1), by0 = ((int)ty * 8) & ~(kRasterBlock - 1);
   const float X0 = (float)bx0, Y0 = (float)by0;
-  const float lx = (float)(px - bx0) + 0.5f, ly0 = (float)(py0 - by0) + 0.5f, ly1 = ly0 + 4.f;
+  const float lx0 = (float)(px0 - bx0) + 0.5f, lx1 = lx0 + 4.f, ly = (float)(py - by0) + 0.5f;
   uint32_t end = 0, items = 0;
   if (has_tile) {
     end = ranges[2 * gtile + 1];
     items = end - ranges[2 * gtile];
   }
-  const bool half_last = (__lane_id() & 7u) == 7u;  // the last lane of each half row of 8
-  const uint32_t row = (uint32_t)l >> 3;           // half-row slot 0..3 of the tile
+  const bool half_last = (__lane_id() & 7u) == 7u;  // the last lane of each group of 8
+  const uint32_t row = (uint32_t)l >> 3;           // the group's slot 0..3 of the tile
 
   PixState s0 = {1.f, 0.f, make_float4(0.f, 0.f, 0.f, 0.f), 0u}, s1 = s0;
   if (in0) {
-    const size_t pix = (size_t)py0 * W + px;
+    const size_t pix = (size_t)py * W + px0;
     s0.last = view.last[pix];
     s0.T = view.final_t[pix];
     s0.g = view.grad_out[pix];
   }
   if (in1) {
-    const size_t pix = (size_t)py1 * W + px;
+    const size_t pix = (size_t)py * W + px1;
     s1.last = view.last[pix];
     s1.T = view.final_t[pix];
     s1.g = view.grad_out[pix];
@@ -1169,12 +1191,14 @@ __global__ __launch_bounds__(kWave) void diff_raster_backward_pair_kernel(
     for (int e = 0; e < cnt; ++e) {
       const uint32_t i = hi - 1 - e;
       const float4 a = s_rec[h][0][e], bb = s_rec[h][1][e], c = s_rec[h][2][e];
-      const PixCover c0 = bwd_cover(s0, a, bb, lx, ly0, i), c1 = bwd_cover(s1, a, bb, lx, ly1, i);
+      const float jx = fmaf(a.y, ly, a.z), jy = fmaf(bb.y, ly, bb.z);
+      const PixCover c0 = bwd_cover(s0, fmaf(a.x, lx0, jx), fmaf(bb.x, lx0, jy), i);
+      const PixCover c1 = bwd_cover(s1, fmaf(a.x, lx1, jx), fmaf(bb.x, lx1, jy), i);
       if (__ballot(c0.in | c1.in) == 0) continue;  // no pixel of either tile: the slots stay zero
       walked_a += i < max_a ? 1u : 0u;
       walked_b += i < max_b ? 1u : 0u;
       const PixTerms t0 = bwd_pixel(s0, c0, a, c), t1 = bwd_pixel(s1, c1, a, c);
-      // the lane's two pixels summed (FMAs), then over its half row
+      // the lane's two pixels summed (FMAs), then over its group of 8 lanes
       float w[9];
       w[0] = fmaf(s1.g.x, t1.ta, s0.g.x * t0.ta);
       w[1] = fmaf(s1.g.y, t1.ta, s0.g.y * t0.ta);
