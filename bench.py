@@ -452,7 +452,9 @@ def main(argv=None) -> int:
     flags = cfg.flags if cfg is not None else 0
     pairs_form = (tw, th) == (8, 8) and r_variant == 0 and not (flags & PAIRS_NEVER) and (
         (flags & PAIRS_ALWAYS) or T >= 8192)
-    raster_name = ("rasterize_tri_kernel" if args.triangles else "rasterize_pair_kernel" if pairs_form else
+    # (the timed frames, whose stage times price the roofline, run the pair kernel's work-counting
+    # form rasterize_pair_kernel<true>; graph-replayed frames run <false>, which skips the counts)
+    raster_name = ("rasterize_tri_kernel" if args.triangles else "rasterize_pair_kernel<true>" if pairs_form else
                    "rasterize_kernel" if r_variant == 0 else "rasterize_wave_kernel")
     if t_raster >= t_proj:
         # vector fp32 work (no stage is a dense contraction, MFMA is unused)

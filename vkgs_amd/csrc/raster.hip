@@ -336,6 +336,7 @@ __global__ __launch_bounds__(TW* TH) void rasterize_kernel(const float4* __restr
 // lengths within a bucket of each other, so little of the wave idles.  Both
 // halves step through their lists together; a half whose list ended reads
 // records that cover nothing.
+template <bool kWork>
 __global__ __launch_bounds__(kWave) void rasterize_pair_kernel(const float4* __restrict__ rec,
                                                                const uint32_t* __restrict__ vals,
                                                                const uint32_t* __restrict__ ranges, RasterShape rs,
@@ -409,14 +410,16 @@ __global__ __launch_bounds__(kWave) void rasterize_pair_kernel(const float4* __r
         composite_at(fmaf(ra.x, fx0, jx), fmaf(rb.x, fx0, jy), ra, rc, T0, r0, g0, b0, a0);
         composite_at(fmaf(ra.x, fx1, jx), fmaf(rb.x, fx1, jy), ra, rc, T1, r1, g1, b1, a1);
       }
-      walked_a += (uint32_t)max(0, min(kStepGroup, cnt_a - e));
-      walked_b += (uint32_t)max(0, min(kStepGroup, cnt_b - e));
+      if (kWork) {
+        walked_a += (uint32_t)max(0, min(kStepGroup, cnt_a - e));
+        walked_b += (uint32_t)max(0, min(kStepGroup, cnt_b - e));
+      }
       if (__ballot((T0 >= t_min) | (T1 >= t_min)) == 0) break;
     }
   }
   if (in0) out[(size_t)py * rs.width + px0] = make_float4(r0, g0, b0, a0 + T0);
   if (in1) out[(size_t)py * rs.width + px1] = make_float4(r1, g1, b1, a1 + T1);
-  if (work && l == 0 && has_tile) work[slot] = h ? walked_b : walked_a;
+  if (kWork && l == 0 && has_tile) work[slot] = h ? walked_b : walked_a;
 }
 
 // ------------------------------------------------- triangle-exact raster ---
@@ -832,8 +835,14 @@ hipError_t launch_rasterize(const float4* rec, const uint32_t* vals, const uint3
   }
   // two tiles per wave halve the launch's waves: by default only where there are enough tiles to fill the GPU
   if (tw == 8 && th == 8 && rs.variant == 0 && use_pairs(rs.pairs, tiles)) {
-    const size_t pad = occupancy_lds_pad((const void*)rasterize_pair_kernel, 1, cap);
-    rasterize_pair_kernel<<<(tiles + 1) / 2, kWave, pad, s>>>(rec, vals, ranges, rs, tile_order, out, work);
+    if (work) {  // the work-counting form (timed frames)
+      const size_t pad = occupancy_lds_pad((const void*)rasterize_pair_kernel<true>, 1, cap);
+      rasterize_pair_kernel<true><<<(tiles + 1) / 2, kWave, pad, s>>>(rec, vals, ranges, rs, tile_order, out, work);
+    } else {
+      const size_t pad = occupancy_lds_pad((const void*)rasterize_pair_kernel<false>, 1, cap);
+      rasterize_pair_kernel<false><<<(tiles + 1) / 2, kWave, pad, s>>>(rec, vals, ranges, rs, tile_order, out,
+                                                                       nullptr);
+    }
     return hipGetLastError();
   }
   VKGS_RASTER_CASE(16, 16)
