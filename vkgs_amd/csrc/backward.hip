@@ -1022,17 +1022,24 @@ __global__ __launch_bounds__(kWave) void diff_raster_pair_kernel(DiffLaunch L, c
     const float4* s0 = s_rec[buf][0] + h * kStride;
     const float4* s1 = s_rec[buf][1] + h * kStride;
     const float4* s2 = s_rec[buf][2] + h * kStride;
+    // groups of kGroup entries between exit tests: a pixel that has stopped
+    // (T < t_min) composites nothing more and keeps its last, and entries
+    // past both lists are staged as records that cover nothing, so the extra
+    // steps change no pixel state
+    constexpr int kGroup = 4;
+    static_assert(kPairHalf % kGroup == 0, "batches hold whole groups");
     int e = 0;
-    for (; e < cnt; ++e) {
-      const float4 a = s0[e], bb = s1[e], c = s2[e];
-      const uint32_t idx = done + e + 1;
-      const float jx = fmaf(a.y, ly, a.z), jy = fmaf(bb.y, ly, bb.z);
-      diff_step_at(fmaf(a.x, lx0, jx), fmaf(bb.x, lx0, jy), a, c, idx, t_min, T0, r0, g0, b0, a0, last0);
-      diff_step_at(fmaf(a.x, lx1, jx), fmaf(bb.x, lx1, jy), a, c, idx, t_min, T1, r1, g1, b1, a1, last1);
-      if (__ballot((T0 >= t_min) | (T1 >= t_min)) == 0) {
-        ++e;
-        break;
+    for (; e < cnt;) {
+#pragma unroll
+      for (int j = 0; j < kGroup; ++j) {
+        const float4 a = s0[e + j], bb = s1[e + j], c = s2[e + j];
+        const uint32_t idx = done + e + j + 1;
+        const float jx = fmaf(a.y, ly, a.z), jy = fmaf(bb.y, ly, bb.z);
+        diff_step_at(fmaf(a.x, lx0, jx), fmaf(bb.x, lx0, jy), a, c, idx, t_min, T0, r0, g0, b0, a0, last0);
+        diff_step_at(fmaf(a.x, lx1, jx), fmaf(bb.x, lx1, jy), a, c, idx, t_min, T1, r1, g1, b1, a1, last1);
       }
+      e += kGroup;
+      if (__ballot((T0 >= t_min) | (T1 >= t_min)) == 0) break;
     }
     walked_a += (uint32_t)min(e, cnt_a);
     walked_b += (uint32_t)min(e, cnt_b);
