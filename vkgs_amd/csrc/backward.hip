@@ -662,6 +662,22 @@ __global__ __launch_bounds__(TW* TH) void diff_raster_kernel(DiffLaunch L, const
     }
     const int cnt = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kN, items - b * kN));
     if (__ballot(T >= t_min) == 0) continue;
+    if (kOneWave) {
+      // groups of 4 entries between exit tests (the batch's tail is staged as
+      // records that cover nothing; a stopped pixel composites nothing more
+      // and keeps its last)
+      static_assert(kN % 4 == 0, "batches hold whole groups");
+      for (int e = 0; e < cnt;) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          diff_step(s_rec[buf][0][e + j], s_rec[buf][1][e + j], s_rec[buf][2][e + j], lx, ly, b * kN + e + j + 1,
+                    t_min, T, cr, cg, cb, aa, last);
+        walked += (uint32_t)min(4, cnt - e);
+        e += 4;
+        if (__ballot(T >= t_min) == 0) break;
+      }
+      continue;
+    }
     for (int e = 0; e < cnt; ++e) {
       const float4 a = s_rec[buf][0][e], bb = s_rec[buf][1][e], c = s_rec[buf][2][e];
       if (!kOneWave) {
