@@ -205,14 +205,21 @@ def test_pygs_load_render_show_close(tmp_path):
     pygs._state["frame"] = None
 
 
-def test_stage_timing_sums_to_total_and_graph_replay_resumes():
+@pytest.mark.parametrize("pairs", [False, True])
+def test_stage_timing_sums_to_total_and_graph_replay_resumes(pairs):
     """vkgs_renderer_set_timing (the reference's per-stage timestamps, engine.cc:1031-1045): the stage
     times are positive and add up to the frame total; images equal the graph-replayed ones; with timing
-    off the frame is replayed from a hipGraph again (faster than timed eager frames)."""
+    off the frame is replayed from a hipGraph again (faster than timed eager frames).  With 8x8 tiles
+    two per wave forced (pairs), the timed frames run the pair raster's work-counting form and the
+    replayed ones the form without counts: the images must still be equal bit for bit."""
     sc = S.gaussian_cloud(200_000, seed=65)
     cam = S.orbit_camera(800, 600, 6.0)
     r = P.Renderer(sc.n)
     r.load_buffer(sc.ply_buffer(), sc.n)
+    if pairs:
+        cfg = P._lib.RasterConfig()
+        cfg.tile_w, cfg.tile_h, cfg.t_min, cfg.flags = 8, 8, 1.0 / 8192.0, 4  # VKGS_RASTER_PAIRS_ALWAYS
+        r.set_config(config=cfg)
     for _ in range(2):
         base = r.render(cam).cpu().numpy()
     r.set_timing(True)
