@@ -571,7 +571,7 @@ int vkgs_renderer_render(vkgs_renderer* r, const vkgs_camera* camera, const floa
     VKGS_HIP_CHECK(record_frame(r, d_rgba, s, true));
     VKGS_HIP_CHECK(hipEventSynchronize(r->ev[6]));
     float t[kStages] = {0};
-    hipEventElapsedTime(&t[0], r->ev[0], r->ev[1]);  // visible count + scan
+    hipEventElapsedTime(&t[0], r->ev[0], r->ev[1]);  // visible counts per group of workgroups
     hipEventElapsedTime(&t[1], r->ev[1], r->ev[2]);  // project + compact (one kernel)
     hipEventElapsedTime(&t[2], r->ev[2], r->ev[3]);  // depth sort
     hipEventElapsedTime(&t[3], r->ev[3], r->ev[4]);  // bin: count, scan, emit
