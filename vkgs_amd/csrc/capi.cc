@@ -162,14 +162,13 @@ hipError_t run_draw_packed(const uint64_t* val64, const uint32_t* d_count, uint3
                            hipEvent_t* marks) {
   hipError_t e;
   if (!b.status_cleared && (e = fill_u32_launch(b.status, 0u, 1, s)) != hipSuccess) return e;
-  if ((e = scan_rect_counts_launch(b.scan, val64, rb, b.offs, d_count, b.pairs, b.scan_storage, s)) != hipSuccess)
-    return e;
   const uint32_t tiles = rs.tiles_x * rs.tiles_y;
   const uint32_t tile_bits = (uint32_t)bits_for(tiles);
   uint32_t *pk, *pv;
   pair_targets(b, tile_bits, &pk, &pv);
-  if ((e = launch_bin_emit_packed(val64, b.offs, d_count, max_inst, rs, rb, max_pairs, pk, pv, b.status, s,
-                                  b.ranges, 2 * tiles)) != hipSuccess)
+  // offs (max_inst words) holds the emit workgroups' group offsets and sums
+  if ((e = launch_bin_grouped(val64, d_count, max_inst, rs, rb, max_pairs, pk, pv, b.pairs, b.status, b.offs, s,
+                              b.ranges, 2 * tiles)) != hipSuccess)
     return e;
   if (marks) (void)hipEventRecord(marks[0], s);
   if ((e = sort_pairs(b, tile_bits, s)) != hipSuccess) return e;

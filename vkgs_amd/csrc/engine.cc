@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <condition_variable>
 #include <mutex>
@@ -176,7 +177,9 @@ hipError_t ensure_frame(vkgs_renderer* r, uint32_t width, uint32_t height) {
     if ((e = r->rect.ensure((size_t)cap * sizeof(TileRect))) != hipSuccess) return e;
     if ((e = r->cnt.ensure((size_t)cap * 4)) != hipSuccess) return e;
   }
-  if ((e = r->offs.ensure((size_t)cap * 4)) != hipSuccess) return e;
+  // offs: the gather path's scanned counts, or the packed path's emit group sums
+  const size_t offs_bytes = std::max((size_t)cap * 4, bin_grouped_storage_bytes(cap));
+  if ((e = r->offs.ensure(offs_bytes)) != hipSuccess) return e;
   if ((e = r->scan_storage.ensure(r->scan.bytes)) != hipSuccess) return e;
   if ((e = r->pkeys.ensure((size_t)r->max_pairs * 4)) != hipSuccess) return e;
   if ((e = r->pvals.ensure((size_t)r->max_pairs * 4)) != hipSuccess) return e;

@@ -145,6 +145,150 @@ __global__ __launch_bounds__(256) void bin_emit_packed_kernel(const uint64_t* __
   }
 }
 
+// Binning of depth-sorted packed values without a scan launch.  One
+// workgroup per kBinGroup emit workgroups (kBinGroup x 256 items) sums each
+// emit workgroup's tile counts and writes, per emit workgroup, the pairs of
+// the group's earlier ones (boff) and the group's total (gsum); an emit
+// workgroup then adds the group totals before its group and scans its own 256
+// counts.  Integer sums: the offsets equal the exclusive scan's.
+constexpr uint32_t kBinGroup = 16;
+constexpr uint32_t kBinGroupThreads = 512;
+constexpr uint32_t kBinGroupIters = kBinGroup * 256 / kBinGroupThreads;
+static_assert(kBinGroup <= kWave && kBinGroupThreads % 256 == 0, "bin group layout");
+
+__device__ __forceinline__ uint32_t bin_wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v += (uint32_t)__shfl_xor((int)v, d, kWave);
+  return v;
+}
+
+__global__ __launch_bounds__(kBinGroupThreads) void bin_group_count_kernel(const uint64_t* __restrict__ val64,
+                                                                           const uint32_t* __restrict__ d_count,
+                                                                           uint32_t max_n, RectBits rb, uint32_t nb,
+                                                                           uint32_t* __restrict__ boff,
+                                                                           uint32_t* __restrict__ gsum) {
+  constexpr uint32_t kWaves = kBinGroupThreads / kWave;
+  constexpr uint32_t kPerIter = kBinGroupThreads / 256;  // emit workgroups per iteration
+  __shared__ uint32_t s_c[kBinGroupIters * kWaves];      // tiles of wave w's items in iteration it
+  const uint32_t n = min(*d_count, max_n);
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t base = blockIdx.x * kBinGroup * 256 + threadIdx.x;
+  uint64_t pv[kBinGroupIters];
+#pragma unroll
+  for (uint32_t it = 0; it < kBinGroupIters; ++it) {
+    const uint32_t k = base + it * kBinGroupThreads;
+    pv[it] = k < n ? val64[k] : 0ull;
+  }
+#pragma unroll
+  for (uint32_t it = 0; it < kBinGroupIters; ++it) {
+    uint32_t c = 0;
+    if (base + it * kBinGroupThreads < n) (void)unpack_rect((uint32_t)(pv[it] >> 32), rb, &c);
+    c = bin_wave_sum(c);
+    if (__lane_id() == 0) s_c[it * kWaves + w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    const uint32_t j = threadIdx.x;  // emit workgroup j of the group
+    uint32_t v = 0;
+    if (j < kBinGroup) {
+      const uint32_t* c = s_c + (j / kPerIter) * kWaves + (j % kPerIter) * (256 / kWave);
+#pragma unroll
+      for (uint32_t i = 0; i < 256 / kWave; ++i) v += c[i];
+    }
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)inc, d, kWave);
+      if ((int)j >= d) inc += t;
+    }
+    const uint32_t b = blockIdx.x * kBinGroup + j;
+    if (j < kBinGroup && b < nb) boff[b] = inc - v;
+    if (j == kBinGroup - 1) gsum[blockIdx.x] = inc;
+  }
+}
+
+// Pairs (tile, position) of the depth-sorted packed values, one item per
+// lane (emit_pairs_wave as in bin_emit_packed_kernel), kBinChunks units of
+// 256 consecutive items per workgroup: the first unit's offset from
+// bin_group_count_kernel's output, the rest by the workgroup's running scan.
+// Workgroup 0 stores the pair total in *d_pairs; the first n_clear words at
+// clear are zeroed on the way (the tile ranges of this draw).
+constexpr uint32_t kBinChunks = 4;
+static_assert(kBinGroup % kBinChunks == 0, "a workgroup's units share a group");
+
+__global__ __launch_bounds__(256) void bin_emit_grouped_kernel(
+    const uint64_t* __restrict__ val64, const uint32_t* __restrict__ d_count, uint32_t max_n, uint32_t tiles_x,
+    RectBits rb, uint64_t max_pairs, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ gsum,
+    uint32_t groups, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ d_pairs,
+    uint32_t* __restrict__ status, uint32_t* __restrict__ clear, uint32_t n_clear) {
+  constexpr uint32_t kWaves = 256 / kWave;
+  __shared__ uint32_t s_tot[kWaves];
+  __shared__ uint32_t s_base[kWaves];
+  __shared__ uint32_t s_wsum[2][kWaves];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_clear; i += gridDim.x * blockDim.x) clear[i] = 0u;
+  const uint32_t n = min(*d_count, max_n);
+  const uint32_t w = threadIdx.x / kWave, lane = __lane_id();
+  if (blockIdx.x == 0) {  // the pair total: every group's sum
+    uint32_t t = 0;
+    for (uint32_t i = threadIdx.x; i < groups; i += 256) t += gsum[i];
+    t = bin_wave_sum(t);
+    if (lane == 0) s_tot[w] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t all = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < kWaves; ++i) all += s_tot[i];
+      *d_pairs = all;
+    }
+  }
+  const uint32_t u0 = blockIdx.x * kBinChunks;
+  if (u0 * 256 >= n) return;  // workgroup-uniform
+  uint64_t pw[kBinChunks];
+#pragma unroll
+  for (uint32_t q = 0; q < kBinChunks; ++q) {
+    const uint32_t k = (u0 + q) * 256 + threadIdx.x;
+    pw[q] = k < n ? val64[k] : 0ull;
+  }
+  // the first unit's first pair: group totals before its group + its offset in the group
+  uint32_t acc = threadIdx.x == 0 ? boff[u0] : 0u;
+  for (uint32_t i = threadIdx.x; i < u0 / kBinGroup; i += 256) acc += gsum[i];
+  acc = bin_wave_sum(acc);
+  if (lane == 0) s_base[w] = acc;
+  __syncthreads();
+  uint32_t running = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kWaves; ++i) running += s_base[i];
+#pragma unroll
+  for (uint32_t q = 0; q < kBinChunks; ++q) {
+    if ((u0 + q) * 256 >= n) break;  // workgroup-uniform
+    const uint32_t k = (u0 + q) * 256 + threadIdx.x;
+    uint32_t c = 0;
+    TileRect r = {0u, 0u};
+    if (k < n) r = unpack_rect((uint32_t)(pw[q] >> 32), rb, &c);
+    uint32_t inc = c;  // inclusive scan of the wave's counts
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)inc, d, kWave);
+      if ((int)lane >= d) inc += t;
+    }
+    // two buffers: a wave writes unit q + 2's sums only after every wave has
+    // passed unit q + 1's barrier, which follows their reads of unit q's
+    if (lane == kWave - 1) s_wsum[q & 1][w] = inc;
+    __syncthreads();
+    uint32_t o = running, unit = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kWaves; ++i) {
+      const uint32_t x = s_wsum[q & 1][i];
+      o += i < w ? x : 0u;
+      unit += x;
+    }
+    o += inc - c;
+    running += unit;
+    if (c != 0 && (uint64_t)o + c > max_pairs) atomicOr(status, 1u);
+    emit_pairs_wave(c, (uint64_t)o, r, (uint32_t)pw[q], 0u, tiles_x, max_pairs, keys, vals);
+  }
+}
+
 __global__ void ranges_fill_kernel(uint32_t* __restrict__ ranges, uint32_t n2) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n2) ranges[i] = 0u;
@@ -772,6 +916,30 @@ hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, c
   bin_emit_packed_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(val64, offs, d_count, max_n, rs.tiles_x, rb,
                                                                      max_pairs, keys, vals, status, clear,
                                                                      clear ? n_clear : 0u);
+  return hipGetLastError();
+}
+
+size_t bin_grouped_storage_bytes(uint32_t max_n) {
+  const uint32_t nb = ceil_div(max_n ? max_n : 1u, 256u);
+  return ((size_t)nb + ceil_div(nb, kBinGroup)) * 4;
+}
+
+hipError_t launch_bin_grouped(const uint64_t* val64, const uint32_t* d_count, uint32_t max_n, const RasterShape& rs,
+                              const RectBits& rb, uint64_t max_pairs, uint32_t* keys, uint32_t* vals,
+                              uint32_t* d_pairs, uint32_t* status, uint32_t* storage, hipStream_t s, uint32_t* clear,
+                              uint32_t n_clear) {
+  if (max_n == 0) {
+    hipError_t e = fill_u32_launch(d_pairs, 0u, 1, s);
+    if (e == hipSuccess && clear && n_clear) e = fill_u32_launch(clear, 0u, n_clear, s);
+    return e;
+  }
+  const uint32_t nb = ceil_div(max_n, 256u), groups = ceil_div(nb, kBinGroup);
+  uint32_t* boff = storage;
+  uint32_t* gsum = storage + nb;
+  bin_group_count_kernel<<<groups, kBinGroupThreads, 0, s>>>(val64, d_count, max_n, rb, nb, boff, gsum);
+  bin_emit_grouped_kernel<<<ceil_div(nb, kBinChunks), 256, 0, s>>>(val64, d_count, max_n, rs.tiles_x, rb, max_pairs,
+                                                                    boff, gsum, groups, keys, vals, d_pairs, status,
+                                                                    clear, clear ? n_clear : 0u);
   return hipGetLastError();
 }
 
