@@ -108,20 +108,16 @@ hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint
                            const uint32_t* d_count, uint32_t max_n, const RasterShape& rs, uint64_t max_pairs,
                            uint32_t* keys, uint32_t* vals, uint32_t* status, hipStream_t s,
                            uint32_t* clear = nullptr, uint32_t n_clear = 0);
-// (tile, slot) pairs at offs[k] from depth-sorted values val64[k] = slot |
-// packed rect << 32 (offs = exclusive scan of the packed tile counts).
-// Binning of depth-sorted packed values in two launches and no scan: pair
+// Binning of depth-sorted packed values val64[k] = slot | packed rect << 32
+// in two launches and no scan: pair
 // offsets from per-workgroup group sums (storage: bin_grouped_storage_bytes(
-// max_n) bytes), *d_pairs = the pair total; pairs as launch_bin_emit_packed's.
+// max_n) bytes), *d_pairs = the pair total; pairs (tile, slot) at keys / vals
+// in depth order, each splat's tiles row-major.
 size_t bin_grouped_storage_bytes(uint32_t max_n);
 hipError_t launch_bin_grouped(const uint64_t* val64, const uint32_t* d_count, uint32_t max_n, const RasterShape& rs,
                               const RectBits& rb, uint64_t max_pairs, uint32_t* keys, uint32_t* vals,
                               uint32_t* d_pairs, uint32_t* status, uint32_t* storage, hipStream_t s,
                               uint32_t* clear = nullptr, uint32_t n_clear = 0);
-hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
-                                  const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
-                                  uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear = nullptr,
-                                  uint32_t n_clear = 0);
 // [start, end) per tile from the tile-sorted keys; `cleared`: the ranges were
 // already zeroed (by the emit launch).  Keys are tile ids, or with view_tiles
 // view << local_bits | tile in the view, for tile view_tiles[view] + tile in

@@ -8,7 +8,8 @@
 //              rectangle) into a 48-byte raster record rec[k], plus its tile
 //              rectangle and tile count (the fused renderer writes the record
 //              at projection time instead and carries the packed rectangle in
-//              the depth sort's value: bin_emit_packed);
+//              the depth sort's value: bin_group_count + bin_emit_grouped,
+//              which form the pair offsets without a scan launch);
 // scan       : exclusive scan of the tile counts;
 // bin_emit   : (tile, k) pairs, tiles of one splat in row-major order (the
 //              launch also zeroes the tile ranges);
@@ -113,44 +114,13 @@ __global__ __launch_bounds__(256) void bin_emit_kernel(const TileRect* __restric
   }
 }
 
-// Same from depth-sorted 64-bit values slot | packed rect << 32: pair value =
-// slot, offsets from the scan of the packed rectangles' tile counts.
-__global__ __launch_bounds__(256) void bin_emit_packed_kernel(const uint64_t* __restrict__ val64,
-                                                              const uint32_t* __restrict__ offs,
-                                                              const uint32_t* __restrict__ d_count, uint32_t max_n,
-                                                              uint32_t tiles_x, RectBits rb, uint64_t max_pairs,
-                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                              uint32_t* __restrict__ status,
-                                                              uint32_t* __restrict__ clear, uint32_t n_clear) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_clear; i += gridDim.x * blockDim.x) clear[i] = 0u;
-  const uint32_t n = min(*d_count, max_n);
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  // wave-uniform loop: emit_pairs_wave needs every lane
-  for (uint32_t base = wave * kWave; base < n; base += stride) {
-    const uint32_t k = base + __lane_id();
-    uint32_t c = 0, v = 0;
-    uint64_t o = 0;
-    TileRect r = {0u, 0u};
-    if (k < n) {
-      const uint64_t w = val64[k];
-      r = unpack_rect((uint32_t)(w >> 32), rb, &c);
-      if (c != 0) {
-        o = offs[k];
-        v = (uint32_t)w;
-        if (o + c > max_pairs) atomicOr(status, 1u);
-      }
-    }
-    emit_pairs_wave(c, o, r, v, 0u, tiles_x, max_pairs, keys, vals);
-  }
-}
-
 // Binning of depth-sorted packed values without a scan launch.  One
 // workgroup per kBinGroup emit workgroups (kBinGroup x 256 items) sums each
 // emit workgroup's tile counts and writes, per emit workgroup, the pairs of
 // the group's earlier ones (boff) and the group's total (gsum); an emit
-// workgroup then adds the group totals before its group and scans its own 256
-// counts.  Integer sums: the offsets equal the exclusive scan's.
+// workgroup then adds the group totals before its group to its first unit's
+// boff and scans its own units' counts.  Integer sums: the offsets equal the
+// exclusive scan's.
 constexpr uint32_t kBinGroup = 16;
 constexpr uint32_t kBinGroupThreads = 512;
 constexpr uint32_t kBinGroupIters = kBinGroup * 256 / kBinGroupThreads;
@@ -207,8 +177,9 @@ __global__ __launch_bounds__(kBinGroupThreads) void bin_group_count_kernel(const
   }
 }
 
-// Pairs (tile, position) of the depth-sorted packed values, one item per
-// lane (emit_pairs_wave as in bin_emit_packed_kernel), kBinChunks units of
+// Pairs (tile, position) of the depth-sorted packed values val64[k] = k's
+// record slot | packed tile rectangle << 32, one item per lane (its tiles in
+// row-major order at consecutive pair slots, emit_pairs_wave), kBinChunks units of
 // 256 consecutive items per workgroup: the first unit's offset from
 // bin_group_count_kernel's output, the rest by the workgroup's running scan.
 // Workgroup 0 stores the pair total in *d_pairs; the first n_clear words at
@@ -905,17 +876,6 @@ hipError_t launch_bin_emit(const TileRect* rect, const uint32_t* cnt, const uint
   bin_emit_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(rect, cnt, offs, order, d_count, max_n, rs.tiles_x,
                                                               max_pairs, keys, vals, status, clear,
                                                               clear ? n_clear : 0u);
-  return hipGetLastError();
-}
-
-hipError_t launch_bin_emit_packed(const uint64_t* val64, const uint32_t* offs, const uint32_t* d_count, uint32_t max_n,
-                                  const RasterShape& rs, const RectBits& rb, uint64_t max_pairs, uint32_t* keys,
-                                  uint32_t* vals, uint32_t* status, hipStream_t s, uint32_t* clear,
-                                  uint32_t n_clear) {
-  if (max_n == 0) return clear && n_clear ? fill_u32_launch(clear, 0u, n_clear, s) : hipSuccess;
-  bin_emit_packed_kernel<<<grid_for(max_n, 256, 8192), 256, 0, s>>>(val64, offs, d_count, max_n, rs.tiles_x, rb,
-                                                                     max_pairs, keys, vals, status, clear,
-                                                                     clear ? n_clear : 0u);
   return hipGetLastError();
 }
 
