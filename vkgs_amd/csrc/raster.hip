@@ -179,9 +179,11 @@ __global__ __launch_bounds__(kBinGroupThreads) void bin_group_count_kernel(const
 
 // Pairs (tile, position) of the depth-sorted packed values val64[k] = k's
 // record slot | packed tile rectangle << 32, one item per lane (its tiles in
-// row-major order at consecutive pair slots, emit_pairs_wave), kBinChunks units of
-// 256 consecutive items per workgroup: the first unit's offset from
-// bin_group_count_kernel's output, the rest by the workgroup's running scan.
+// row-major order at consecutive pair slots, emit_pairs_wave), `chunks` (1 or
+// kBinChunks) units of 256 consecutive items per workgroup: the first unit's
+// offset from bin_group_count_kernel's output, the rest by the workgroup's
+// running scan.  Several units per workgroup amortize the offset sums on large
+// draws; one keeps enough waves emitting on small ones.
 // Workgroup 0 stores the pair total in *d_pairs; the first n_clear words at
 // clear are zeroed on the way (the tile ranges of this draw).
 constexpr uint32_t kBinChunks = 4;
@@ -190,8 +192,8 @@ static_assert(kBinGroup % kBinChunks == 0, "a workgroup's units share a group");
 __global__ __launch_bounds__(256) void bin_emit_grouped_kernel(
     const uint64_t* __restrict__ val64, const uint32_t* __restrict__ d_count, uint32_t max_n, uint32_t tiles_x,
     RectBits rb, uint64_t max_pairs, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ gsum,
-    uint32_t groups, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ d_pairs,
-    uint32_t* __restrict__ status, uint32_t* __restrict__ clear, uint32_t n_clear) {
+    uint32_t groups, uint32_t chunks, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+    uint32_t* __restrict__ d_pairs, uint32_t* __restrict__ status, uint32_t* __restrict__ clear, uint32_t n_clear) {
   constexpr uint32_t kWaves = 256 / kWave;
   __shared__ uint32_t s_tot[kWaves];
   __shared__ uint32_t s_base[kWaves];
@@ -212,13 +214,13 @@ __global__ __launch_bounds__(256) void bin_emit_grouped_kernel(
       *d_pairs = all;
     }
   }
-  const uint32_t u0 = blockIdx.x * kBinChunks;
+  const uint32_t u0 = blockIdx.x * chunks;
   if (u0 * 256 >= n) return;  // workgroup-uniform
   uint64_t pw[kBinChunks];
 #pragma unroll
   for (uint32_t q = 0; q < kBinChunks; ++q) {
     const uint32_t k = (u0 + q) * 256 + threadIdx.x;
-    pw[q] = k < n ? val64[k] : 0ull;
+    pw[q] = q < chunks && k < n ? val64[k] : 0ull;
   }
   // the first unit's first pair: group totals before its group + its offset in the group
   uint32_t acc = threadIdx.x == 0 ? boff[u0] : 0u;
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(256) void bin_emit_grouped_kernel(
   for (uint32_t i = 0; i < kWaves; ++i) running += s_base[i];
 #pragma unroll
   for (uint32_t q = 0; q < kBinChunks; ++q) {
-    if ((u0 + q) * 256 >= n) break;  // workgroup-uniform
+    if (q >= chunks || (u0 + q) * 256 >= n) break;  // workgroup-uniform
     const uint32_t k = (u0 + q) * 256 + threadIdx.x;
     uint32_t c = 0;
     TileRect r = {0u, 0u};
@@ -897,9 +899,12 @@ hipError_t launch_bin_grouped(const uint64_t* val64, const uint32_t* d_count, ui
   uint32_t* boff = storage;
   uint32_t* gsum = storage + nb;
   bin_group_count_kernel<<<groups, kBinGroupThreads, 0, s>>>(val64, d_count, max_n, rb, nb, boff, gsum);
-  bin_emit_grouped_kernel<<<ceil_div(nb, kBinChunks), 256, 0, s>>>(val64, d_count, max_n, rs.tiles_x, rb, max_pairs,
-                                                                    boff, gsum, groups, keys, vals, d_pairs, status,
-                                                                    clear, clear ? n_clear : 0u);
+  // several units per emit workgroup from 2M items up (measured: 6M splats
+  // gain; 100k-1M lose waves in flight)
+  const uint32_t chunks = max_n >= (1u << 21) ? kBinChunks : 1u;
+  bin_emit_grouped_kernel<<<ceil_div(nb, chunks), 256, 0, s>>>(val64, d_count, max_n, rs.tiles_x, rb, max_pairs, boff,
+                                                               gsum, groups, chunks, keys, vals, d_pairs, status,
+                                                               clear, clear ? n_clear : 0u);
   return hipGetLastError();
 }
 
