@@ -490,7 +490,8 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"plain forward {wl.name}: {wl.n} splats, {wl.width}x{wl.height}, orbit r={wl.radius} m",
                    "splats": n, "width": wl.width, "height": wl.height, "visible": V, "pairs": K,
-                   "tile": f"{tw}x{th}", "raster_kernel": raster_name,
+                   "tile": f"{tw}x{th}",
+                   "raster_kernel": raster_name.replace("<true>", "<false>"),  # the graph-replayed frames' form
                    "waves_per_cu": (cfg.waves_per_cu if cfg is not None else 0) or "max",
                    "t_min": args.t_min, "graph": True},
         "frame_ms": frame_ms,
